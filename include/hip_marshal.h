@@ -1,0 +1,162 @@
+/*
+ * hip_marshal.h - C ABI of the MI355X (gfx950) in-place layout-conversion library.
+ *
+ * Every entry point of the reference's OpenCL header include/cl_marshal.h:26-49 has a
+ * `hip_`-prefixed twin here with the same argument meaning and the same error convention:
+ *
+ *   - cl_command_queue  -> hipStream_t (0 = the null stream)
+ *   - cl_mem            -> void* device pointer owned by the caller
+ *   - cl_ulong *        -> uint64_t * (elapsed nanoseconds, ACCUMULATED with +=)
+ *   - return value      -> false on success, true on error (diagnostic on stderr)
+ *
+ * The plain twins move 4-byte elements, as the reference does when compiled with SP=1
+ * (src/cl_marshal.cc:37-43). The `_es` variants take the element size in bytes
+ * (1, 2, 4, 8 or 16; elements are opaque bit patterns) and 64-bit dimensions.
+ *
+ * Timing: when profiling is on (the default, like LIBMARSHAL_OCL_PROFILE in the reference's
+ * src/CMakeLists.txt:2,8-10) a non-NULL `elapsed_ns` receives the wall time of the whole call
+ * measured with HIP events (the call then synchronizes its stream), and a NULL pointer makes
+ * the call print "[Profile] <name> <ns>ns; <GB/s> GB/s" to stderr (include/cl_profile.h:43-56).
+ * With profiling off (hip_marshal_set_profiling(0)) calls are fully stream-ordered and never
+ * synchronize, and `elapsed_ns` is left untouched.
+ *
+ * Scratch: cycle-following stages need visited-bit flags. They come from a per-device pool
+ * owned by the library that only grows (hipMalloc happens on growth, never per call once
+ * the pool is large enough; hip_marshal_reserve() pre-sizes it). Flags are cleared with a
+ * fill kernel on the caller's stream. Calls that overlap on two streams of one device must
+ * not share the pool: the library serializes pool use per device with a mutex held until
+ * the call's last launch has been enqueued, and the flags of one call live in a region no
+ * concurrent call can be handed (see hip_marshal_reserve()).
+ */
+#ifndef LIBMARSHAL_AMD_HIP_MARSHAL_H_
+#define LIBMARSHAL_AMD_HIP_MARSHAL_H_
+
+#include <stddef.h>
+#include <stdint.h>
+#include <stdbool.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------------
+ * Twins of include/cl_marshal.h (4-byte elements, 32-bit dimensions).
+ * ------------------------------------------------------------------------------------- */
+
+/* AoS [height][width] -> ASTA [height/tile][width][tile] with the LDS-staged kernel.
+ * Replaces cl_aos_asta_bs, include/cl_marshal.h:26-27 (src/cl_marshal.cc:793-800). */
+bool hip_aos_asta_bs(hipStream_t stream, void *src, int height, int width, int tile_size);
+
+/* Same conversion by cycle following with LDS bit flags; R spreads flags over R times more
+ * words, P pads one word per 32. Replaces cl_aos_asta_pttwac, include/cl_marshal.h:28-29
+ * (src/cl_marshal.cc:803-812). */
+bool hip_aos_asta_pttwac(hipStream_t stream, void *src, int height, int width, int tile_size,
+                         int R, int P);
+
+/* LDS-staged form, falling back to cycle following when it fails.
+ * Replaces cl_aos_asta, include/cl_marshal.h:30-31 (src/cl_marshal.cc:814-818). */
+bool hip_aos_asta(hipStream_t stream, void *src, int height, int width, int tile_size, int R);
+
+/* Full in-place transpose of an (A*a) x (B*b) row-major matrix into (B*b) x (A*a).
+ * stages selects the elementary-stage sequence: 2, 22, 3, 32, 4 as in the reference
+ * (src/cl_marshal.cc:569-767); an infeasible choice falls back to the element-granular
+ * route (src/cl_marshal.cc:769-775). stages = 0 lets the planner choose.
+ * Replaces cl_transpose, include/cl_marshal.h:32-33 (src/cl_marshal.cc:561-776). */
+bool hip_transpose(hipStream_t stream, void *src, int A, int a, int B, int b, int R, int stages,
+                   uint64_t *elapsed_ns);
+
+/* 010: [A][a][B] -> [A][B][a], whole tile staged in LDS.
+ * Replaces cl_transpose_010_bs, include/cl_marshal.h:35-36 (src/cl_marshal.cc:136-277). */
+bool hip_transpose_010_bs(hipStream_t stream, void *src, int A, int a, int B,
+                          uint64_t *elapsed_ns);
+
+/* 010 by cycle following with the tile's visited bits in LDS.
+ * Replaces cl_transpose_010_pttwac, include/cl_marshal.h:37-38 (src/cl_marshal.cc:280-336). */
+bool hip_transpose_010_pttwac(hipStream_t stream, void *src, int A, int a, int B,
+                              uint64_t *elapsed_ns, int R, int P);
+
+/* 100: [A][B][b] -> [B][A][b], cycle following over super-elements of b elements with
+ * global visited bits. Replaces cl_transpose_100, include/cl_marshal.h:40-41
+ * (src/cl_marshal.cc:779-782 -> 340-558). */
+bool hip_transpose_100(hipStream_t stream, void *src, int A, int B, int b, uint64_t *elapsed_ns);
+
+/* 0100: [A][a][B][b] -> [A][B][a][b] (A independent 100 instances).
+ * Replaces cl_transpose_0100, include/cl_marshal.h:43-44 (src/cl_marshal.cc:785-788). */
+bool hip_transpose_0100(hipStream_t stream, void *src, int A, int a, int B, int b,
+                        uint64_t *elapsed_ns);
+
+/* In-place row padding [y_size][x_size] -> [y_size][pad_size] (new columns zeroed) and its
+ * inverse [y_size][pad_size] -> [y_size][x_size] (tail zeroed). The buffer must hold
+ * y_size*pad_size elements. Replace cl_padding / cl_unpadding, include/cl_marshal.h:46-49
+ * (src/cl_marshal.cc:821-956, src/cl/cl_aos_asta.cl:813-938). */
+bool hip_padding(hipStream_t stream, void *src, int x_size, int y_size, int pad_size,
+                 uint64_t *elapsed_ns);
+bool hip_unpadding(hipStream_t stream, void *src, int x_size, int y_size, int pad_size,
+                   uint64_t *elapsed_ns);
+
+/* Releases the scratch pools of every device. Replaces cl_marshal_finalize,
+ * src/cl_marshal.cc:116-119 (exported there but missing from its header). */
+void hip_marshal_finalize(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Element-size-generic variants (elem_bytes in {1,2,4,8,16}, 64-bit dimensions).
+ * Same semantics as the twins above; no counterpart in the reference, whose element type
+ * is fixed at compile time (src/cl_marshal.cc:37-43).
+ * ------------------------------------------------------------------------------------- */
+bool hip_transpose_010_bs_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t A,
+                             int64_t a, int64_t B, uint64_t *elapsed_ns);
+bool hip_transpose_010_pttwac_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t A,
+                                 int64_t a, int64_t B, uint64_t *elapsed_ns, int R, int P);
+bool hip_transpose_100_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t A,
+                          int64_t B, int64_t b, uint64_t *elapsed_ns);
+bool hip_transpose_0100_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t A,
+                           int64_t a, int64_t B, int64_t b, uint64_t *elapsed_ns);
+bool hip_transpose_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t A, int64_t a,
+                      int64_t B, int64_t b, int R, int stages, uint64_t *elapsed_ns);
+/* Full transpose of an M x N matrix; the planner picks the tile factors and the route. */
+bool hip_transpose_full_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t M,
+                           int64_t N, uint64_t *elapsed_ns);
+/* AoS [height][width] -> ASTA [height/tile][width][tile] (010) and back (ASTA -> AoS is
+ * 010 with a and B exchanged). method: 0 = LDS-staged with cycle-following fallback,
+ * 1 = LDS-staged only, 2 = cycle following only. */
+bool hip_aos_asta_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t height,
+                     int64_t width, int64_t tile_size, int method, uint64_t *elapsed_ns);
+bool hip_asta_aos_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t height,
+                     int64_t width, int64_t tile_size, int method, uint64_t *elapsed_ns);
+/* SoA [width][height] -> ASTA [height/tile][width][tile] is 100(width, height/tile, tile)
+ * (test/cl_unittest.cc:200); ASTA -> SoA is 100(height/tile, width, tile). */
+bool hip_soa_asta_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t height,
+                     int64_t width, int64_t tile_size, uint64_t *elapsed_ns);
+bool hip_asta_soa_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t height,
+                     int64_t width, int64_t tile_size, uint64_t *elapsed_ns);
+bool hip_padding_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t x_size,
+                    int64_t y_size, int64_t pad_size, uint64_t *elapsed_ns);
+bool hip_unpadding_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t x_size,
+                      int64_t y_size, int64_t pad_size, uint64_t *elapsed_ns);
+
+/* ---------------------------------------------------------------------------------------
+ * Library configuration and introspection (no counterpart in the reference, where these
+ * are compile-time macros: MAX_MEM src/cl_marshal.cc:45-59, LIBMARSHAL_OCL_PROFILE).
+ * ------------------------------------------------------------------------------------- */
+/* LDS bytes a stage may use per workgroup (default: the device's per-workgroup maximum,
+ * 160 KiB on gfx950). Lowering it forces cycle-following routes; 0 restores the default. */
+void hip_marshal_set_lds_budget(size_t bytes);
+size_t hip_marshal_get_lds_budget(void);
+/* 1 = time calls with events and synchronize (default), 0 = asynchronous. */
+void hip_marshal_set_profiling(int on);
+/* Route tag of the last hip_transpose* call on this thread ("3.1-010_BS" ...). */
+const char *hip_marshal_last_route(void);
+/* Grows the current device's flag pool to at least `bytes` (false on success). */
+bool hip_marshal_reserve(size_t bytes);
+/* Bytes of visited-bit flags the given stage needs (0 for LDS-staged stages). */
+size_t hip_marshal_flag_bytes_0100(int64_t A, int64_t a, int64_t B, int64_t b,
+                                   size_t elem_bytes);
+/* Library version string. */
+const char *hip_marshal_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIBMARSHAL_AMD_HIP_MARSHAL_H_ */

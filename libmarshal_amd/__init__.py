@@ -1,0 +1,240 @@
+"""libmarshal_amd - in-place layout conversion on AMD Instinct MI355X (gfx950).
+
+Python binding of the C ABI in ``include/hip_marshal.h`` and ``include/marshal.h``. The
+compute path is the HIP library ``libmarshal_hip.so`` built next to this file; there is no
+fallback: if the library is missing, importing the binding raises.
+
+The functions below mirror the reference's operator API (include/cl_marshal.h:26-49 and the
+PyOpenCL layer src/py/transpose.py): same names without the ``cl_`` prefix, same argument
+meaning, ``False`` on success and ``True`` on error. Device buffers are torch tensors on a
+ROCm device (any dtype; elements are moved as opaque bytes); the stream is torch's current
+stream unless one is given.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmarshal_hip.so")
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m libmarshal_amd.build` "
+            "(or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, u64p, sz, i32, b = (ctypes.c_void_p, ctypes.c_int64,
+                                 ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t,
+                                 ctypes.c_int, ctypes.c_bool)
+    sig = {
+        "hip_aos_asta_bs": (b, [vp, vp, i32, i32, i32]),
+        "hip_aos_asta_pttwac": (b, [vp, vp, i32, i32, i32, i32, i32]),
+        "hip_aos_asta": (b, [vp, vp, i32, i32, i32, i32]),
+        "hip_transpose": (b, [vp, vp, i32, i32, i32, i32, i32, i32, u64p]),
+        "hip_transpose_010_bs": (b, [vp, vp, i32, i32, i32, u64p]),
+        "hip_transpose_010_pttwac": (b, [vp, vp, i32, i32, i32, u64p, i32, i32]),
+        "hip_transpose_100": (b, [vp, vp, i32, i32, i32, u64p]),
+        "hip_transpose_0100": (b, [vp, vp, i32, i32, i32, i32, u64p]),
+        "hip_padding": (b, [vp, vp, i32, i32, i32, u64p]),
+        "hip_unpadding": (b, [vp, vp, i32, i32, i32, u64p]),
+        "hip_marshal_finalize": (None, []),
+        "hip_transpose_010_bs_es": (b, [vp, vp, sz, i64, i64, i64, u64p]),
+        "hip_transpose_010_pttwac_es": (b, [vp, vp, sz, i64, i64, i64, u64p, i32, i32]),
+        "hip_transpose_100_es": (b, [vp, vp, sz, i64, i64, i64, u64p]),
+        "hip_transpose_0100_es": (b, [vp, vp, sz, i64, i64, i64, i64, u64p]),
+        "hip_transpose_es": (b, [vp, vp, sz, i64, i64, i64, i64, i32, i32, u64p]),
+        "hip_transpose_full_es": (b, [vp, vp, sz, i64, i64, u64p]),
+        "hip_aos_asta_es": (b, [vp, vp, sz, i64, i64, i64, i32, u64p]),
+        "hip_asta_aos_es": (b, [vp, vp, sz, i64, i64, i64, i32, u64p]),
+        "hip_soa_asta_es": (b, [vp, vp, sz, i64, i64, i64, u64p]),
+        "hip_asta_soa_es": (b, [vp, vp, sz, i64, i64, i64, u64p]),
+        "hip_padding_es": (b, [vp, vp, sz, i64, i64, i64, u64p]),
+        "hip_unpadding_es": (b, [vp, vp, sz, i64, i64, i64, u64p]),
+        "hip_marshal_set_lds_budget": (None, [sz]),
+        "hip_marshal_get_lds_budget": (sz, []),
+        "hip_marshal_set_profiling": (None, [i32]),
+        "hip_marshal_last_route": (ctypes.c_char_p, []),
+        "hip_marshal_reserve": (b, [sz]),
+        "hip_marshal_flag_bytes_0100": (sz, [i64, i64, i64, i64, sz]),
+        "hip_marshal_version": (ctypes.c_char_p, []),
+        "gpu_aos_asta_bs_float": (b, [vp, i32, i32, i32, vp]),
+        "gpu_aos_asta_bs_double": (b, [vp, i32, i32, i32, vp]),
+        "gpu_aos_asta_pttwac": (b, [vp, i32, i32, i32, vp]),
+        "gpu_soa_asta_pttwac": (b, [vp, i32, i32, i32, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+# Symbols declared by include/hip_marshal.h and include/marshal.h.
+EXPORTED = [
+    "hip_aos_asta_bs", "hip_aos_asta_pttwac", "hip_aos_asta", "hip_transpose",
+    "hip_transpose_010_bs", "hip_transpose_010_pttwac", "hip_transpose_100",
+    "hip_transpose_0100", "hip_padding", "hip_unpadding", "hip_marshal_finalize",
+    "hip_transpose_010_bs_es", "hip_transpose_010_pttwac_es", "hip_transpose_100_es",
+    "hip_transpose_0100_es", "hip_transpose_es", "hip_transpose_full_es", "hip_aos_asta_es",
+    "hip_asta_aos_es", "hip_soa_asta_es", "hip_asta_soa_es", "hip_padding_es",
+    "hip_unpadding_es", "hip_marshal_set_lds_budget", "hip_marshal_get_lds_budget",
+    "hip_marshal_set_profiling", "hip_marshal_last_route", "hip_marshal_reserve",
+    "hip_marshal_flag_bytes_0100", "hip_marshal_version",
+    "gpu_aos_asta_bs_float", "gpu_aos_asta_bs_double", "gpu_aos_asta_pttwac",
+    "gpu_soa_asta_pttwac",
+]
+
+
+def _ptr(t):
+    import torch
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError("expected a torch tensor on a ROCm device")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class Elapsed:
+    """Accumulating nanosecond counter (the reference's ``cl_ulong *elapsed_time``)."""
+
+    def __init__(self):
+        self._v = ctypes.c_uint64(0)
+
+    @property
+    def ns(self):
+        return self._v.value
+
+    def reset(self):
+        self._v.value = 0
+
+    def ref(self):
+        return ctypes.byref(self._v)
+
+
+def _et(elapsed):
+    return None if elapsed is None else elapsed.ref()
+
+
+def _es(t, elem_bytes):
+    return t.element_size() if elem_bytes is None else elem_bytes
+
+
+# ---- mirrors of include/cl_marshal.h (element size taken from the tensor) --------------------
+
+def transpose(buf, A, a, B, b, R=1, stages=3, elapsed=None, elem_bytes=None, stream=None):
+    """Full in-place transpose of an (A*a) x (B*b) matrix (cl_transpose)."""
+    return lib().hip_transpose_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), A, a, B, b,
+                                  R, stages, _et(elapsed))
+
+
+def transpose_full(buf, M, N, elapsed=None, elem_bytes=None, stream=None):
+    """M x N transpose with planner-chosen tile factors and route."""
+    return lib().hip_transpose_full_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), M, N,
+                                       _et(elapsed))
+
+
+def transpose_010_bs(buf, A, a, B, elapsed=None, elem_bytes=None, stream=None):
+    return lib().hip_transpose_010_bs_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), A,
+                                         a, B, _et(elapsed))
+
+
+def transpose_010_pttwac(buf, A, a, B, elapsed=None, R=1, P=1, elem_bytes=None, stream=None):
+    return lib().hip_transpose_010_pttwac_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes),
+                                             A, a, B, _et(elapsed), R, P)
+
+
+def transpose_100(buf, A, B, b, elapsed=None, elem_bytes=None, stream=None):
+    return lib().hip_transpose_100_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), A, B,
+                                      b, _et(elapsed))
+
+
+def transpose_0100(buf, A, a, B, b, elapsed=None, elem_bytes=None, stream=None):
+    return lib().hip_transpose_0100_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), A, a,
+                                       B, b, _et(elapsed))
+
+
+def aos_asta(buf, height, width, tile_size, method=0, elapsed=None, elem_bytes=None,
+             stream=None):
+    """AoS [h][w] -> ASTA [h/t][w][t]; method 0 = BS then PTTWAC, 1 = BS, 2 = PTTWAC."""
+    return lib().hip_aos_asta_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), height,
+                                 width, tile_size, method, _et(elapsed))
+
+
+def asta_aos(buf, height, width, tile_size, method=0, elapsed=None, elem_bytes=None,
+             stream=None):
+    return lib().hip_asta_aos_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), height,
+                                 width, tile_size, method, _et(elapsed))
+
+
+def soa_asta(buf, height, width, tile_size, elapsed=None, elem_bytes=None, stream=None):
+    """SoA [w][h] -> ASTA [h/t][w][t] (100(w, h/t, t))."""
+    return lib().hip_soa_asta_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), height,
+                                 width, tile_size, _et(elapsed))
+
+
+def asta_soa(buf, height, width, tile_size, elapsed=None, elem_bytes=None, stream=None):
+    return lib().hip_asta_soa_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), height,
+                                 width, tile_size, _et(elapsed))
+
+
+def padding(buf, x_size, y_size, pad_size, elapsed=None, elem_bytes=None, stream=None):
+    return lib().hip_padding_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), x_size,
+                                y_size, pad_size, _et(elapsed))
+
+
+def unpadding(buf, x_size, y_size, pad_size, elapsed=None, elem_bytes=None, stream=None):
+    return lib().hip_unpadding_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), x_size,
+                                  y_size, pad_size, _et(elapsed))
+
+
+def set_lds_budget(nbytes):
+    lib().hip_marshal_set_lds_budget(nbytes)
+
+
+def lds_budget():
+    return lib().hip_marshal_get_lds_budget()
+
+
+def set_profiling(on):
+    lib().hip_marshal_set_profiling(1 if on else 0)
+
+
+def last_route():
+    return lib().hip_marshal_last_route().decode()
+
+
+def finalize():
+    lib().hip_marshal_finalize()
+
+
+def TransposeArrayInPlace(arr, a=1, b=1, stages=None, stream=None):
+    """Transposes a 2-D tensor in place and returns the same storage viewed as the transpose.
+
+    Mirrors TransposeArrayInPlace, src/py/transpose.py:108-116: the input's shape is left as it
+    is, its data is re-laid out as the transpose, and the returned tensor shares the storage
+    with shape (cols, rows). a and b must divide rows and cols. With the default a = b = 1
+    the planner picks the tile factors; otherwise the 3-stage route runs with the given ones.
+    """
+    if arr.dim() != 2:
+        raise TypeError("only allows transposing 2D arrays")
+    rows, cols = arr.shape
+    if rows % a or cols % b:
+        raise TypeError("tile size must evenly divide array dimensions")
+    if stages is None:
+        stages = 0 if (a == 1 and b == 1) else 3
+    if transpose(arr, rows // a, a, cols // b, b, 1, stages, stream=stream):
+        raise RuntimeError("in-place transpose failed")
+    return arr.view(-1)[: rows * cols].view(cols, rows)

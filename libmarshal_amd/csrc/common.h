@@ -1,0 +1,95 @@
+// common.h - shared device/host helpers of the gfx950 layout-conversion library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+
+namespace lm {
+
+// ---------------------------------------------------------------------------------------
+// Division by a run-time invariant divisor with one high multiply.
+// Granlund & Montgomery, "Division by Invariant Integers using Multiplication" (PLDI 1994),
+// Figure 4.1: with l = ceil(log2 d), m' = floor(2^N (2^l - d) / d) + 1,
+//   t1 = mulhi(m', n);  q = (t1 + ((n - t1) >> sh1)) >> sh2,  sh1 = min(l,1), sh2 = max(l-1,0)
+// exact for every N-bit n and 1 <= d < 2^N. The reference divides with the hardware divide
+// in its index map (include/plan.hpp:54-56); here the divisor is fixed per launch.
+// ---------------------------------------------------------------------------------------
+struct FastDiv32 {
+  uint32_t d = 1, m = 0, sh1 = 0, sh2 = 0;
+  FastDiv32() = default;
+  __host__ explicit FastDiv32(uint32_t dd) : d(dd) {
+    uint32_t l = 0;
+    while (l < 32 && (uint64_t(1) << l) < dd) l++;
+    uint64_t num = ((uint64_t(1) << l) - dd) << 32;  // < 2^64 since 2^l - d < 2^32
+    m = uint32_t(num / dd + 1);
+    sh1 = l < 1 ? l : 1;
+    sh2 = l > 1 ? l - 1 : 0;
+  }
+  __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    uint32_t t = uint32_t((uint64_t(m) * n) >> 32);
+    return (t + ((n - t) >> sh1)) >> sh2;
+  }
+};
+
+struct FastDiv64 {
+  uint64_t d = 1, m = 0;
+  uint32_t sh1 = 0, sh2 = 0;
+  FastDiv64() = default;
+  __host__ explicit FastDiv64(uint64_t dd) : d(dd) {
+    uint32_t l = 0;
+    while (l < 64 && (uint64_t(1) << l) < dd) l++;
+    unsigned __int128 two_l = (unsigned __int128)1 << l;
+    unsigned __int128 num = (two_l - dd) << 64;
+    m = uint64_t(num / dd + 1);
+    sh1 = l < 1 ? l : 1;
+    sh2 = l > 1 ? l - 1 : 0;
+  }
+  __host__ __device__ __forceinline__ uint64_t div(uint64_t n) const {
+    uint64_t t = uint64_t(((unsigned __int128)m * n) >> 64);
+    return (t + ((n - t) >> sh1)) >> sh2;
+  }
+};
+
+// ---------------------------------------------------------------------------------------
+// Element storage by size. Elements are opaque: they are only loaded and stored.
+// ---------------------------------------------------------------------------------------
+template <int ES> struct Elem;
+template <> struct Elem<1> { using T = uint8_t; };
+template <> struct Elem<2> { using T = uint16_t; };
+template <> struct Elem<4> { using T = uint32_t; };
+template <> struct Elem<8> { using T = uint64_t; };
+struct alignas(16) U128 { uint64_t lo, hi; };
+template <> struct Elem<16> { using T = U128; };
+
+// Sizes of the device this library targets (gfx950 / MI355X).
+constexpr int kWave = 64;
+
+// Host-side error reporting in the reference's style: a message on stderr, `true` returned.
+#define LM_CHECK_HIP(expr)                                                              \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess) {                                                             \
+      fprintf(stderr, "[libmarshal] %s:%d %s: %s\n", __FILE__, __LINE__, #expr,         \
+              hipGetErrorString(_e));                                                   \
+      return true;                                                                      \
+    }                                                                                   \
+  } while (0)
+
+// Wait until every outstanding vector-memory load of this wave has returned its data.
+__device__ __forceinline__ void wait_vm_loads() {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// Visited-bit flags in global memory are only touched with device-scope (agent) atomics so
+// that every XCD sees one coherent copy; the data moves themselves need no ordering beyond
+// "a start element is read before its bit is tested" (see the cycle-following kernels).
+__device__ __forceinline__ uint32_t flag_load(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t flag_or(uint32_t *p, uint32_t mask) {
+  return __hip_atomic_fetch_or(p, mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace lm
