@@ -1,0 +1,178 @@
+"""Benchmark: in-place AoS -> ASTA -> AoS on one MI355X.
+
+Workload: 2^24 structs of 16 f32 fields (1 GiB), ASTA tile width 64 (one wave64), converted
+AoS -> ASTA (010(2^18, 64, 16)) and back ASTA -> AoS (010(2^18, 16, 64)) by the LDS-staged
+kernel behind hip_aos_asta_es / hip_asta_aos_es. One step = both conversions; inputs are
+resident in HBM before timing starts. Effective bandwidth = 2 x bytes / time per conversion.
+
+Printed: one JSON line with the metric, a roofline object for the dominant kernel (its
+launches timed with HIP events on the stream they run on) and a CPU baseline (the oracle's
+multi-threaded restatement of the reference's barrier-sync kernel on a bounded sample of the
+same workload, on this host's cores). Extra measurements of the other forms ride along in
+"extra".
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "in-place GB/s (2x bytes/time) and % HBM peak; batched/served p50/p99 latency"
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_010_bs_aos_asta_2e24x16_f32_pmc.json")
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--cpu-seconds", type=float, default=10.0,
+                   help="approximate CPU work for the baseline sample")
+    p.add_argument("--no-extra", action="store_true")
+    return p.parse_args()
+
+
+def time_calls(fn, reps, stream):
+    """Average device time of fn() over reps calls, bracketed by events on `stream`."""
+    import torch
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(reps)]
+    with torch.cuda.stream(stream):
+        for e0, e1 in evs:
+            e0.record(stream)
+            if fn():
+                raise RuntimeError("library call failed")
+            e1.record(stream)
+    stream.synchronize()
+    return sum(e0.elapsed_time(e1) for e0, e1 in evs) / reps / 1e3  # seconds
+
+
+def cpu_baseline(n_structs, w, t, seconds):
+    import numpy as np
+    import oracle
+    cores = oracle.num_cpus()
+    # Bounded sample of the same workload: a slice of the structs, converted repeatedly.
+    sample = min(n_structs, 1 << 22)
+    x = oracle.tagged(sample * w, 4, seed=11).view(np.uint32)
+    t0 = time.perf_counter()
+    oracle.par_010_bs(x, sample // t, t, w, nthreads=cores)
+    one = time.perf_counter() - t0
+    reps = max(1, int(seconds / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        oracle.par_010_bs(x, sample // t, t, w, nthreads=cores)
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": round(2 * x.nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": cores,
+            "kind": "port",
+            "sample": f"AoS->ASTA of {sample} of the 2^24 structs (16 x f32, tile 64), "
+                      f"{reps} repetitions, oracle/oracle.c orc_par_010_bs on {cores} threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                          "(the multi-GPU sharding design is described in DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+    import libmarshal_amd as lm
+
+    torch.cuda.set_device(0)
+    lm.set_profiling(False)  # asynchronous calls: no per-call synchronization
+    n, w, t = 1 << 24, 16, 64
+    nbytes = n * w * 4
+    stream = torch.cuda.current_stream()
+    g = torch.Generator(device="cuda").manual_seed(1234)
+    d = torch.rand(n * w, device="cuda", generator=g) * 2 - 1  # U[-1, 1)
+    ref = d.clone()
+
+    def step():
+        return lm.aos_asta(d, n, w, t, method=1, elem_bytes=4) or \
+            lm.asta_aos(d, n, w, t, method=1, elem_bytes=4)
+
+    for _ in range(args.warmup):
+        if step():
+            raise RuntimeError("warm-up failed")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        if step():
+            raise RuntimeError("step failed")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_s = e0.elapsed_time(e1) / 1e3 / args.steps
+    if not torch.equal(d.view(torch.int32), ref.view(torch.int32)):
+        raise RuntimeError("round trip is not bit-exact")
+
+    # Dominant kernel: one LDS-staged 010 launch per conversion call.
+    launch_s = time_calls(lambda: lm.aos_asta(d, n, w, t, method=1, elem_bytes=4) or
+                          lm.asta_aos(d, n, w, t, method=1, elem_bytes=4), 10, stream) / 2
+    achieved = 2 * nbytes / launch_s / 1e9
+    traffic = None
+    if os.path.exists(PMC_SUMMARY):
+        try:
+            traffic = json.load(open(PMC_SUMMARY)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    value = 2 * 2 * nbytes / step_s / 1e9
+
+    extra = {}
+    if not args.no_extra:
+        reps = 5
+        s = time_calls(lambda: lm.aos_asta(d, n, w, t, method=2, elem_bytes=4) or
+                       lm.asta_aos(d, n, w, t, method=2, elem_bytes=4), reps, stream) / 2
+        extra["aos_asta_cycle_following_GBs"] = round(2 * nbytes / s / 1e9, 1)
+        s = time_calls(lambda: lm.soa_asta(d, n, w, t, elem_bytes=4) or
+                       lm.asta_soa(d, n, w, t, elem_bytes=4), reps, stream) / 2
+        extra["soa_asta_GBs"] = round(2 * nbytes / s / 1e9, 1)
+        if not torch.equal(d.view(torch.int32), ref.view(torch.int32)):
+            raise RuntimeError("cycle-following round trips are not bit-exact")
+        del ref
+        # Full 1800 x 7200 int32 transpose, a = 18, b = 72, 3-stage route (paper shape).
+        M, N = 1800, 7200
+        x = torch.randint(-2**31, 2**31 - 1, (M * N,), dtype=torch.int32, device="cuda")
+        s = time_calls(lambda: lm.transpose(x, M // 18, 18, N // 72, 72, 1, 3), reps, stream)
+        extra["transpose_1800x7200_i32_3stage_GBs"] = round(2 * M * N * 4 / s / 1e9, 1)
+        s = time_calls(lambda: lm.transpose_full(x, M, N), reps, stream)
+        extra["transpose_1800x7200_i32_planner_GBs"] = round(2 * M * N * 4 / s / 1e9, 1)
+        extra["planner_route"] = lm.last_route()
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: U[-1,1) f32 from a seeded generator, resident in HBM",
+        "config": {"workload": "AoS->ASTA->AoS in place, 2^24 structs x 16 f32 (1 GiB), "
+                               "ASTA width 64, LDS-staged 010 kernel",
+                   "structs": n, "fields": w, "tile": t, "elem_bytes": 4,
+                   "parallelism": "single GPU"},
+        "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "k_010_bs<4,1,256>",
+                     "launch_us": round(launch_s * 1e6, 2),
+                     "algorithmic_bytes_per_launch": 2 * nbytes},
+        "wall_s": round(wall, 3),
+        "extra": extra,
+    }
+    out["cpu_baseline"] = cpu_baseline(n, w, t, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

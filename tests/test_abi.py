@@ -1,0 +1,86 @@
+"""C-ABI library: loads without a GPU, exports every symbol the headers declare, host logic."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from tests.conftest import ROOT
+
+
+def header_symbols():
+    syms = set()
+    for h in ("hip_marshal.h", "marshal.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(?:bool|void|size_t|const char\s*\*)\s*\**\s*(\w+)\s*\(", text):
+            syms.add(m.group(1))
+    return syms
+
+
+def test_library_loads_and_exports_all_declared_symbols():
+    import libmarshal_amd as lm
+    L = lm.lib()
+    declared = header_symbols()
+    assert len(declared) >= 30
+    for s in declared:
+        assert hasattr(L, s), s
+    assert set(lm.EXPORTED) == declared
+    nm = subprocess.run(["nm", "-D", "--defined-only", lm.LIB_PATH], capture_output=True,
+                        text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in nm.splitlines() if " T " in ln}
+    assert declared <= exported
+    assert b"gfx950" in L.hip_marshal_version()
+
+
+def test_library_contains_gfx950_code_object():
+    import libmarshal_amd as lm
+    data = open(lm.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in data
+
+
+def test_fastdiv_host_check(tmp_path):
+    src = os.path.join(ROOT, "tests", "host", "fastdiv_check.cpp")
+    exe = str(tmp_path / "fdc")
+    subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-o", exe, src], check=True,
+                   capture_output=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout
+    assert "fails=0" in r.stdout
+
+
+def test_flag_bytes_host_logic():
+    import libmarshal_amd as lm
+    L = lm.lib()
+    # 100(16, 2^18, 64) f32: one bit per 256-byte super-element
+    assert L.hip_marshal_flag_bytes_0100(1, 16, 1 << 18, 64, 4) == 16 * (1 << 18) // 8
+    # element-granular fallback on 250007 x 199999 bytes: one bit per element
+    n = 250007 * 199999
+    assert L.hip_marshal_flag_bytes_0100(1, 250007, 199999, 1, 1) == (n + 31) // 32 * 4
+    # invalid input gives 0
+    assert L.hip_marshal_flag_bytes_0100(1, 0, 5, 1, 4) == 0
+    assert L.hip_marshal_flag_bytes_0100(1, 2, 5, 1, 3) == 0
+
+
+def test_invalid_arguments_rejected_without_gpu_work():
+    """Argument validation happens before any device call (error convention: True)."""
+    import ctypes
+    import libmarshal_amd as lm
+    L = lm.lib()
+    nul = ctypes.c_void_p(0)
+    assert L.hip_transpose_es(nul, nul, 4, 2, 3, 4, 5, 1, 3, None) is True
+    assert L.hip_transpose_es(nul, ctypes.c_void_p(256), 3, 2, 3, 4, 5, 1, 3, None) is True
+    assert L.hip_transpose_es(nul, ctypes.c_void_p(256), 4, 0, 3, 4, 5, 1, 3, None) is True
+    assert L.hip_aos_asta_es(nul, ctypes.c_void_p(256), 4, 10, 3, 4, 0, None) is True  # 10 % 4
+    assert L.hip_padding_es(nul, ctypes.c_void_p(256), 4, 9, 3, 8, None) is True       # x > pad
+    assert L.hip_transpose_es(nul, ctypes.c_void_p(258), 4, 2, 3, 4, 5, 1, 3, None) is True
+
+
+def test_lds_budget_knob():
+    import libmarshal_amd as lm
+    default = lm.lds_budget()
+    assert default >= 64 * 1024
+    lm.set_lds_budget(4096)
+    assert lm.lds_budget() == 4096
+    lm.set_lds_budget(0)
+    assert lm.lds_budget() == default

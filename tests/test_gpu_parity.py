@@ -1,0 +1,296 @@
+"""Parity of the HIP path (called through the C ABI) with the CPU oracle and the golden vectors.
+
+Shapes follow the reference's own suites: test/cl_unittest.cc (bug533/536/537, test_0100,
+full) and test/unittest.cu (bug523-538). Every comparison is bit-exact on tagged elements.
+"""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+from tests.golden import vectors
+
+pytestmark = pytest.mark.gpu
+
+ES = (1, 2, 4, 8, 16)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def check(fn, x, want, *args, **kw):
+    t = dev(x)
+    r = fn(t, *args, **kw)
+    got = host(t)
+    assert r is False, "library returned an error"
+    if not np.array_equal(got, want):
+        bad = np.nonzero(got != want)[0]
+        raise AssertionError(f"{bad.size} bytes differ, first at byte {bad[0]}")
+
+
+@pytest.mark.parametrize("op,es,dims,x,y", list(vectors.cases()),
+                         ids=lambda v: str(v) if isinstance(v, (str, int)) else None)
+def test_golden_vectors(hip, op, es, dims, x, y):
+    say("golden", op, es, dims)
+    if op == "010":
+        A, a, B = dims
+        check(hip.transpose_010_bs, x, y, A, a, B, elem_bytes=es)
+        check(hip.transpose_010_pttwac, x, y, A, a, B, R=2, P=1, elem_bytes=es)
+    elif op == "100":
+        A, B, b = dims
+        check(hip.transpose_100, x, y, A, B, b, elem_bytes=es)
+    elif op == "0100":
+        A, a, B, b = dims
+        check(hip.transpose_0100, x, y, A, a, B, b, elem_bytes=es)
+    elif op == "T":
+        M, N = dims
+        check(hip.transpose_full, x, y, M, N, elem_bytes=es)
+        check(hip.transpose, x, y, 1, M, 1, N, 1, 1, elem_bytes=es)  # fallback route
+    elif op == "PAD":
+        xs, ys, pad = dims
+        check(hip.padding, x, y, xs, ys, pad, elem_bytes=es)
+    elif op == "UNPAD":
+        xs, ys, pad = dims
+        check(hip.unpadding, x, y, xs, ys, pad, elem_bytes=es)
+
+
+def test_010_bs_bug533_shapes(hip, orc):
+    # test/cl_unittest.cc:295-325: w = 3..384 doubling, t = 1..8, h = (500/w+1)*ceil(13000/t)*t
+    for w in (3, 6, 12, 24, 48, 96, 192, 384):
+        for t in range(1, 9):
+            h = (500 // w + 1) * ((100 * 130 + t - 1) // t) * t
+            es = (4, 1, 2, 8, 16)[(w + t) % 5]
+            say("bug533", w, t, h, es)
+            x = orc.tagged(h * w, es, seed=w * 10 + t)
+            want = orc.op_010(x, h // t, t, w, es=es)
+            check(hip.transpose_010_bs, x, want, h // t, t, w, elem_bytes=es)
+
+
+def test_010_pttwac_bug536_shapes(hip, orc):
+    # test/cl_unittest.cc:226-292: w in {197, 215}, t in {16, 32, 64}, R = 1..16, P = 1
+    for w, h0 in ((197, 35588), (215, 44609)):
+        for t in (16, 32, 64):
+            h = (h0 + t - 1) // t * t
+            x = orc.tagged(h * w, 4, seed=w + t)
+            want = orc.op_010(x, h // t, t, w, es=4)
+            for R in (1, 2, 4, 8, 16):
+                say("bug536", w, t, R)
+                check(hip.transpose_010_pttwac, x, want, h // t, t, w, R=R, P=1, elem_bytes=4)
+
+
+@pytest.mark.parametrize("es", [1, 4, 8, 16])
+def test_010_pttwac_element_sizes(hip, orc, es):
+    for (A, a, B) in ((50, 33, 47), (7, 64, 100), (3, 300, 211), (1000, 2, 3)):
+        say("pttwac", es, A, a, B)
+        x = orc.tagged(A * a * B, es, seed=A + a + B)
+        check(hip.transpose_010_pttwac, x, orc.op_010(x, A, a, B, es=es), A, a, B, R=1, P=1,
+              elem_bytes=es)
+
+
+def test_100_bug537_shapes(hip, orc):
+    # test/cl_unittest.cc:171-223: 100(w, h/t, t), w in {197, 215}, t = 1..1024 powers of two
+    for w, h0 in ((197, 35588), (215, 44609)):
+        t = 1
+        while t <= 1024:
+            h = (h0 + t - 1) // t * t
+            say("bug537", w, t)
+            x = orc.tagged(h * w, 4, seed=w * t)
+            check(hip.transpose_100, x, orc.op_100(x, w, h // t, t, es=4), w, h // t, t,
+                  elem_bytes=4)
+            t *= 2
+
+
+@pytest.mark.parametrize("es", ES)
+def test_100_element_sizes(hip, orc, es):
+    for (A, B, b) in ((16, 4096, 64), (37, 53, 3), (5, 7, 1000), (1000, 3, 1), (2, 2, 5000)):
+        say("100", es, A, B, b)
+        x = orc.tagged(A * B * b, es, seed=A * B + b)
+        check(hip.transpose_100, x, orc.op_100(x, A, B, b, es=es), A, B, b, elem_bytes=es)
+
+
+def test_0100_reference_case(hip, orc):
+    # test_0100, test/cl_unittest.cc:666-703: A=128, a=62, B=57, b=256
+    A, a, B, b = 128, 62, 57, 256
+    say("test_0100")
+    x = orc.tagged(A * a * B * b, 4, seed=3)
+    check(hip.transpose_0100, x, orc.op_0100(x, A, a, B, b, es=4), A, a, B, b, elem_bytes=4)
+
+
+def test_marshal_h_twins(hip, orc):
+    """The include/marshal.h entry points on the reference's CUDA suite shapes
+    (test/unittest.cu:101-246)."""
+    import ctypes
+    import torch
+    L = hip.lib()
+
+    def run(fn, x, h, w, t):
+        d = dev(x)
+        r = fn(ctypes.c_void_p(d.data_ptr()), h, w, t, None)
+        out = host(d)
+        assert r is False
+        return out
+
+    # bug538: soa, h = 65538, t = 2, w = 2
+    h, t, w = 65538, 2, 2
+    x = orc.tagged(h * w, 4, seed=538)
+    assert np.array_equal(run(L.gpu_soa_asta_pttwac, x, h, w, t), orc.op_100(x, w, h // t, t, es=4))
+    # bug528: soa, h = 1024, t = 64, w = 1..99
+    for w in range(1, 100, 7):
+        say("bug528", w)
+        h, t = 1024, 64
+        x = orc.tagged(h * w, 4, seed=w)
+        assert np.array_equal(run(L.gpu_soa_asta_pttwac, x, h, w, t),
+                              orc.op_100(x, w, h // t, t, es=4))
+    # bug525 / bug524: aos pttwac, h = 16384, t = 16, w = 1..99 and 66
+    for w in list(range(1, 100, 7)) + [66]:
+        say("bug525", w)
+        h, t = 16 * 1024, 16
+        x = orc.tagged(h * w, 4, seed=w)
+        assert np.array_equal(run(L.gpu_aos_asta_pttwac, x, h, w, t),
+                              orc.op_010(x, h // t, t, w, es=4))
+    # bug523 / bug523d: BS float and double, w = 6, t = 16
+    h, t, w = 16 * 1024, 16, 6
+    x = orc.tagged(h * w, 4, seed=523)
+    assert np.array_equal(run(L.gpu_aos_asta_bs_float, x, h, w, t), orc.op_010(x, h // t, t, w, es=4))
+    x = orc.tagged(h * w, 8, seed=524)
+    assert np.array_equal(run(L.gpu_aos_asta_bs_double, x, h, w, t), orc.op_010(x, h // t, t, w, es=8))
+    # errors: height not a multiple of the tile
+    d = torch.zeros(100, dtype=torch.float32, device="cuda")
+    assert L.gpu_aos_asta_bs_float(ctypes.c_void_p(d.data_ptr()), 10, 10, 3, None) is True
+
+
+def _divisors(n):
+    return [d for d in range(1, n + 1) if n % d == 0]
+
+
+@pytest.mark.parametrize("stages", [2, 22, 3, 32, 4, 1])
+def test_transpose_routes_all_factorisations(hip, orc, stages):
+    """cl_transpose's stage sequences (src/cl_marshal.cc:561-776) on every factorisation of
+    small shapes, for 1-, 4- and 16-byte elements."""
+    for (M, N) in ((12, 18), (30, 8), (7, 20)):
+        for es in (1, 4, 16):
+            x = orc.tagged(M * N, es, seed=M * N + es)
+            want = orc.op_transpose(x, M, N, es=es)
+            for a, b in itertools.product(_divisors(M), _divisors(N)):
+                check(hip.transpose, x, want, M // a, a, N // b, b, 1, stages, elem_bytes=es)
+
+
+@pytest.mark.parametrize("hw", [(1800, 7200), (2500, 5100), (3200, 4000), (3900, 3300),
+                                (5100, 2500), (7200, 1800)])
+def test_transpose_ppopp_sizes(hip, orc, hw):
+    """The PPoPP'14 sizes of the reference's `full` test (test/cl_unittest.cc:386-387) with the
+    tile factors of its Alg. 5 and the 3-stage routes of its Alg. 3, plus the planner route."""
+    h, w = hw
+    aa, bb, ph, pw = orc.choose_tiles(h, w, 24, 110)
+    assert ph == 0 and pw == 0
+    x = orc.tagged(h * w, 4, seed=h)
+    want = orc.op_transpose(x, h, w, es=4)
+    say("ppopp", h, w, aa, bb)
+    check(hip.transpose, x, want, h // aa, aa, w // bb, bb, 1, 3 if aa <= bb else 32,
+          elem_bytes=4)
+    check(hip.transpose_full, x, want, h, w, elem_bytes=4)
+
+
+def test_config1_three_stage_exact(hip, orc):
+    # 1800 x 7200 int32, a = 18, b = 72 (A = B = 100), stages = 3
+    M, N = 1800, 7200
+    x = (orc.splitmix64(2024, M * N) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    want = x.reshape(M, N).T.copy()
+    t = dev(x)
+    et = hip.Elapsed()
+    assert hip.transpose(t, M // 18, 18, N // 72, 72, 1, 3, elapsed=et) is False
+    assert hip.last_route().endswith("3.1-")
+    assert np.array_equal(host(t).reshape(N, M), want)
+    assert et.ns > 0
+
+
+def test_full_flow_with_padding(hip, orc):
+    """The reference's `full` flow for a dimension with no divisor in range: pad the columns,
+    transpose, unpad (test/cl_unittest.cc:526-543 / 607-628)."""
+    h0, w0 = 1009, 1013  # both prime
+    aa, bb, ph, pw = orc.choose_tiles(h0, w0, 24, 110)
+    h, w = h0 + ph, w0 + pw
+    x = orc.tagged(h * w, 4, seed=9)
+    src = x.view(np.uint32)[: h0 * w0].reshape(h0, w0).copy()
+    t = dev(x)
+    assert hip.padding(t, w0, h0, w, elem_bytes=4) is False
+    assert hip.transpose(t, h // aa, aa, w // bb, bb, 1, 3 if aa <= bb else 32,
+                         elem_bytes=4) is False
+    assert hip.unpadding(t, h0, w0, h, elem_bytes=4) is False
+    got = host(t).view(np.uint32)[: h0 * w0].reshape(w0, h0)
+    assert np.array_equal(got, src.T)
+
+
+@pytest.mark.parametrize("es", ES)
+def test_forced_routes_match(hip, orc, es):
+    """Shrinking the LDS budget forces the LDS-flag and global-flag forms of 010; every route
+    must give the same bytes (SURVEY: host-forced fallbacks)."""
+    A, a, B = 40, 37, 29
+    x = orc.tagged(A * a * B, es, seed=es)
+    want = orc.op_010(x, A, a, B, es=es)
+    try:
+        for budget in (0, 2048, 64):
+            hip.set_lds_budget(budget)
+            say("forced", es, budget)
+            check(hip.aos_asta, x, want, A * a, B, a, 0, elem_bytes=es)
+        hip.set_lds_budget(64)
+        assert hip.aos_asta(dev(x), A * a, B, a, 1, elem_bytes=es) is True  # BS cannot run
+    finally:
+        hip.set_lds_budget(0)
+
+
+@pytest.mark.parametrize("es", ES)
+def test_aos_soa_round_trips(hip, orc, es):
+    h, w, t = 4096, 13, 64
+    x = orc.tagged(h * w, es, seed=77)
+    d = dev(x)
+    assert hip.aos_asta(d, h, w, t, elem_bytes=es) is False
+    assert np.array_equal(host(d), orc.op_010(x, h // t, t, w, es=es))
+    assert hip.asta_aos(d, h, w, t, elem_bytes=es) is False
+    assert np.array_equal(host(d), x)
+    assert hip.soa_asta(d, h, w, t, elem_bytes=es) is False
+    assert np.array_equal(host(d), orc.op_100(x, w, h // t, t, es=es))
+    assert hip.asta_soa(d, h, w, t, elem_bytes=es) is False
+    assert np.array_equal(host(d), x)
+
+
+def test_config2_full_size_round_trip(hip, orc):
+    """2^24 structs of 16 f32 fields (1 GiB), ASTA width 64: AoS->ASTA with the LDS-staged and
+    the cycle-following kernel, back, and SoA->ASTA and back; bit-exact."""
+    import torch
+    n, w, t = 1 << 24, 16, 64
+    g = torch.Generator(device="cuda").manual_seed(5)
+    d = torch.rand(n * w, device="cuda", generator=g) * 2 - 1
+    orig = d.clone()
+    sample = torch.randint(0, n * w, (1 << 16,), device="cuda", generator=g)
+    for method in (1, 2):
+        say("config2 aos method", method)
+        assert hip.aos_asta(d, n, w, t, method=method) is False
+        # element (r, c) of AoS lands at ASTA (r // t, c, r % t)
+        r, c = sample // w, sample % w
+        dst = (r // t) * (w * t) + c * t + (r % t)
+        assert torch.equal(d[dst].view(torch.int32), orig[sample].view(torch.int32))
+        assert hip.asta_aos(d, n, w, t, method=method) is False
+        assert torch.equal(d.view(torch.int32), orig.view(torch.int32))
+    say("config2 soa")
+    assert hip.soa_asta(d, n, w, t) is False
+    # SoA element (c, r) (field-major) lands at ASTA (r // t, c, r % t)
+    c, r = sample // n, sample % n
+    dst = (r // t) * (w * t) + c * t + (r % t)
+    assert torch.equal(d[dst].view(torch.int32), orig[sample].view(torch.int32))
+    assert hip.asta_soa(d, n, w, t) is False
+    assert torch.equal(d.view(torch.int32), orig.view(torch.int32))
