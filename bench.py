@@ -33,22 +33,25 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="approximate CPU work for the baseline sample")
     p.add_argument("--no-extra", action="store_true")
+    p.add_argument("--profile", type=int, default=0,
+                   help="only run N LDS-staged AoS->ASTA launches and N plain 1 GiB device "
+                        "copies (the calibration kernel for HBM byte counters), then exit")
     return p.parse_args()
 
 
 def time_calls(fn, reps, stream):
-    """Average device time of fn() over reps calls, bracketed by events on `stream`."""
+    """Average device time of fn() over `reps` back-to-back calls, bracketed by one pair of
+    HIP events on `stream` (the stream the library launches on)."""
     import torch
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(reps)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     with torch.cuda.stream(stream):
-        for e0, e1 in evs:
-            e0.record(stream)
+        e0.record(stream)
+        for _ in range(reps):
             if fn():
                 raise RuntimeError("library call failed")
-            e1.record(stream)
+        e1.record(stream)
     stream.synchronize()
-    return sum(e0.elapsed_time(e1) for e0, e1 in evs) / reps / 1e3  # seconds
+    return e0.elapsed_time(e1) / reps / 1e3  # seconds
 
 
 def cpu_baseline(n_structs, w, t, seconds):
@@ -90,6 +93,16 @@ def main():
     d = torch.rand(n * w, device="cuda", generator=g) * 2 - 1  # U[-1, 1)
     ref = d.clone()
 
+    if args.profile:
+        d2 = torch.empty_like(d)
+        for _ in range(args.profile):
+            if lm.aos_asta(d, n, w, t, method=1, elem_bytes=4):
+                raise RuntimeError("conversion failed")
+            d2.copy_(d)
+        torch.cuda.synchronize()
+        print(f"profile run: {args.profile} x k_010_bs and {args.profile} x 1 GiB copy", flush=True)
+        return
+
     def step():
         return lm.aos_asta(d, n, w, t, method=1, elem_bytes=4) or \
             lm.asta_aos(d, n, w, t, method=1, elem_bytes=4)
@@ -113,7 +126,7 @@ def main():
 
     # Dominant kernel: one LDS-staged 010 launch per conversion call.
     launch_s = time_calls(lambda: lm.aos_asta(d, n, w, t, method=1, elem_bytes=4) or
-                          lm.asta_aos(d, n, w, t, method=1, elem_bytes=4), 10, stream) / 2
+                          lm.asta_aos(d, n, w, t, method=1, elem_bytes=4), 20, stream) / 2
     achieved = 2 * nbytes / launch_s / 1e9
     traffic = None
     if os.path.exists(PMC_SUMMARY):
@@ -164,7 +177,7 @@ def main():
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_010_bs<4,1,256>",
+                     "traffic": traffic, "kernel": "k_010_bs<4,1,256,16>",
                      "launch_us": round(launch_s * 1e6, 2),
                      "algorithmic_bytes_per_launch": 2 * nbytes},
         "wall_s": round(wall, 3),

@@ -145,6 +145,13 @@ void hip_marshal_set_lds_budget(size_t bytes);
 size_t hip_marshal_get_lds_budget(void);
 /* 1 = time calls with events and synchronize (default), 0 = asynchronous. */
 void hip_marshal_set_profiling(int on);
+/* Launch-shape knobs: "bs_upt" (4, 8, 16 units per lane per round of the LDS-staged 010),
+ * "bs_blocks_per_cu", "pt010_blocks_per_cu", "pt100_blocks_per_cu" (grid caps). Replaces the
+ * reference's hard-coded work-group tables (src/cl_marshal.cc:165-231, 388-465). The setter
+ * returns true on an unknown key or a value outside [1, 1024]; the getter returns 0 for an
+ * unknown key. */
+bool hip_marshal_set_tuning(const char *key, int64_t value);
+int64_t hip_marshal_get_tuning(const char *key);
 /* Route tag of the last hip_transpose* call on this thread ("3.1-010_BS" ...). */
 const char *hip_marshal_last_route(void);
 /* Grows the current device's flag pool to at least `bytes` (false on success). */

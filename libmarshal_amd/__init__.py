@@ -63,6 +63,8 @@ def lib():
         "hip_marshal_reserve": (b, [sz]),
         "hip_marshal_flag_bytes_0100": (sz, [i64, i64, i64, i64, sz]),
         "hip_marshal_version": (ctypes.c_char_p, []),
+        "hip_marshal_set_tuning": (b, [ctypes.c_char_p, i64]),
+        "hip_marshal_get_tuning": (i64, [ctypes.c_char_p]),
         "gpu_aos_asta_bs_float": (b, [vp, i32, i32, i32, vp]),
         "gpu_aos_asta_bs_double": (b, [vp, i32, i32, i32, vp]),
         "gpu_aos_asta_pttwac": (b, [vp, i32, i32, i32, vp]),
@@ -86,7 +88,8 @@ EXPORTED = [
     "hip_asta_aos_es", "hip_soa_asta_es", "hip_asta_soa_es", "hip_padding_es",
     "hip_unpadding_es", "hip_marshal_set_lds_budget", "hip_marshal_get_lds_budget",
     "hip_marshal_set_profiling", "hip_marshal_last_route", "hip_marshal_reserve",
-    "hip_marshal_flag_bytes_0100", "hip_marshal_version",
+    "hip_marshal_flag_bytes_0100", "hip_marshal_version", "hip_marshal_set_tuning",
+    "hip_marshal_get_tuning",
     "gpu_aos_asta_bs_float", "gpu_aos_asta_bs_double", "gpu_aos_asta_pttwac",
     "gpu_soa_asta_pttwac",
 ]
@@ -210,6 +213,15 @@ def lds_budget():
 
 def set_profiling(on):
     lib().hip_marshal_set_profiling(1 if on else 0)
+
+
+def set_tuning(key, value):
+    if lib().hip_marshal_set_tuning(key.encode(), int(value)):
+        raise ValueError(f"bad tuning {key}={value}")
+
+
+def get_tuning(key):
+    return lib().hip_marshal_get_tuning(key.encode())
 
 
 def last_route():

@@ -57,6 +57,35 @@ static size_t lds_budget() {
 }
 
 // ---------------------------------------------------------------------------------------
+// Launch-shape knobs. Defaults come from measurements on MI355X; hip_marshal_set_tuning
+// overrides them (the reference hard-codes its equivalents per vendor,
+// src/cl_marshal.cc:165-231 and 388-465).
+// ---------------------------------------------------------------------------------------
+namespace {
+struct Knob {
+  const char *key;
+  std::atomic<int64_t> value;
+};
+Knob g_knobs[] = {
+    {"bs_upt", {16}},
+    {"bs_blocks_per_cu", {12}},
+    {"pt010_blocks_per_cu", {8}},
+    {"pt100_blocks_per_cu", {16}},
+};
+Knob *find_knob(const char *key) {
+  if (!key) return nullptr;
+  for (Knob &k : g_knobs)
+    if (strcmp(k.key, key) == 0) return &k;
+  return nullptr;
+}
+}  // namespace
+
+int64_t tuning(const char *key) {
+  Knob *k = find_knob(key);
+  return k ? k->value.load() : 0;
+}
+
+// ---------------------------------------------------------------------------------------
 // Flag pool: one growable device buffer per device. A call holds the pool's mutex while it
 // enqueues; use by a different stream than the previous user is ordered after that user's
 // last enqueued work with an event, so flags are never cleared under a running stage.
@@ -232,7 +261,8 @@ static bool stage_0100(unsigned char *buf, size_t es, uint64_t inst, uint64_t R,
   uint32_t *flags = lease.get(pl.flag_words * 4);
   if (!flags) return true;
   LM_CHECK_HIP(launch_fill_u32(flags, pl.flag_words, 0, st));
-  LM_CHECK_HIP(launch_100_pttwac(buf, flags, inst, R, C, b * es, pl, grid_cap(16), st));
+  LM_CHECK_HIP(launch_100_pttwac(buf, flags, inst, R, C, b * es, pl,
+                                 grid_cap(uint32_t(tuning("pt100_blocks_per_cu"))), st));
   return false;
 }
 
@@ -240,7 +270,8 @@ static bool stage_0100(unsigned char *buf, size_t es, uint64_t inst, uint64_t R,
 static M010 choose_010(size_t es, uint64_t A, uint64_t a, uint64_t B, int R, int P,
                        const void *buf) {
   Bs010Plan pl{};
-  if (plan_010_bs(es, A, a, B, lds_budget(), reinterpret_cast<uintptr_t>(buf), 16384, &pl))
+  if (plan_010_bs(es, A, a, B, lds_budget(), reinterpret_cast<uintptr_t>(buf), int(tuning("bs_upt")),
+                  &pl))
     return M010::BS;
   if (a * B < (1ull << 31) && pt010_flag_words(a, B, R, P) * 4 <= lds_budget())
     return M010::PTTWAC;
@@ -256,14 +287,15 @@ static bool stage_010(unsigned char *buf, size_t es, uint64_t A, uint64_t a, uin
   if (m == M010::Auto) m = choose_010(es, A, a, B, R, P, buf);
   if (m == M010::BS) {
     Bs010Plan pl{};
-    if (!plan_010_bs(es, A, a, B, lds_budget(), reinterpret_cast<uintptr_t>(buf), 16384,
-                     &pl)) {
+    if (!plan_010_bs(es, A, a, B, lds_budget(), reinterpret_cast<uintptr_t>(buf),
+                     int(tuning("bs_upt")), &pl)) {
       fprintf(stderr, "[libmarshal] 010 BS: tile %llux%llu does not fit the LDS budget\n",
               (unsigned long long)a, (unsigned long long)B);
       return true;
     }
     if (tag) *tag = "010_BS-";
-    LM_CHECK_HIP(launch_010_bs(buf, es, A, a, B, pl, grid_cap(32), st));
+    LM_CHECK_HIP(launch_010_bs(buf, es, A, a, B, pl,
+                               grid_cap(uint32_t(tuning("bs_blocks_per_cu"))), st));
     return false;
   }
   if (m == M010::PTTWAC) {
@@ -274,7 +306,8 @@ static bool stage_010(unsigned char *buf, size_t es, uint64_t A, uint64_t a, uin
       return true;
     }
     if (tag) *tag = "010_PTTWAC-";
-    LM_CHECK_HIP(launch_010_pttwac(buf, es, A, a, B, R, P, grid_cap(8), st));
+    LM_CHECK_HIP(launch_010_pttwac(buf, es, A, a, B, R, P,
+                                   grid_cap(uint32_t(tuning("pt010_blocks_per_cu"))), st));
     return false;
   }
   // 010 on [A][a][B] is 0100 with single-element super-elements: global visited bits.
@@ -361,7 +394,7 @@ double cost_pt(uint64_t S) { return 1.0 + 96.0 / double(S); }
 double cost_010(size_t es, uint64_t A, uint64_t a, uint64_t B) {
   if (a <= 1 || B <= 1) return 0.0;
   Bs010Plan pl{};
-  if (plan_010_bs(es, A, a, B, lds_budget(), 0, 16384, &pl)) return 1.0;
+  if (plan_010_bs(es, A, a, B, lds_budget(), 0, 16, &pl)) return 1.0;
   return 3.0 * cost_pt(es);
 }
 
@@ -414,6 +447,18 @@ extern "C" {
 const char *hip_marshal_version(void) { return "libmarshal-amd 0.1 (gfx950)"; }
 
 void hip_marshal_set_lds_budget(size_t bytes) { g_lds_budget.store(bytes); }
+
+bool hip_marshal_set_tuning(const char *key, int64_t value) {
+  Knob *k = find_knob(key);
+  if (!k || value < 1 || value > 1024) {
+    fprintf(stderr, "[libmarshal] hip_marshal_set_tuning: bad key or value\n");
+    return true;
+  }
+  k->value.store(value);
+  return false;
+}
+
+int64_t hip_marshal_get_tuning(const char *key) { return tuning(key); }
 size_t hip_marshal_get_lds_budget(void) { return lds_budget(); }
 void hip_marshal_set_profiling(int on) { g_profiling.store(on ? 1 : 0); }
 const char *hip_marshal_last_route(void) { return t_route.c_str(); }

@@ -60,7 +60,7 @@ template <> struct Elem<1> { using T = uint8_t; };
 template <> struct Elem<2> { using T = uint16_t; };
 template <> struct Elem<4> { using T = uint32_t; };
 template <> struct Elem<8> { using T = uint64_t; };
-struct alignas(16) U128 { uint64_t lo, hi; };
+typedef uint32_t U128 __attribute__((ext_vector_type(4)));
 template <> struct Elem<16> { using T = U128; };
 
 // Sizes of the device this library targets (gfx950 / MI355X).

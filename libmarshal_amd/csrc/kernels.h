@@ -18,6 +18,7 @@ struct Bs010Params {
 
 struct Bs010Plan {
   int G;              // elements per lane access
+  int upt;            // units per thread per round (4, 8 or 16)
   uint64_t g;         // tiles per chunk
   uint32_t Bp;        // LDS row pitch (elements)
   size_t lds_bytes;   // dynamic LDS per workgroup
@@ -27,7 +28,7 @@ struct Bs010Plan {
 uint32_t bs010_pitch(uint32_t B);
 // Chooses the chunking of an LDS-staged 010; false when one tile does not fit `lds_budget`.
 bool plan_010_bs(size_t es, uint64_t A, uint64_t a, uint64_t B, size_t lds_budget,
-                 uint64_t base_addr, size_t target_chunk_bytes, Bs010Plan *out);
+                 uint64_t base_addr, int upt, Bs010Plan *out);
 hipError_t launch_010_bs(unsigned char *base, size_t es, uint64_t A, uint64_t a, uint64_t B,
                          const Bs010Plan &pl, uint32_t max_blocks, hipStream_t st);
 
@@ -76,6 +77,10 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStrea
 uint64_t pad_flag_words(uint64_t total_elems);
 hipError_t launch_padding(unsigned char *base, size_t es, uint64_t x, uint64_t y, uint64_t pad,
                           bool unpad, uint32_t *flags, hipStream_t st);
+
+// Run-time tuning knobs (launch shapes). Keys: "bs_upt", "bs_blocks_per_cu",
+// "pt010_blocks_per_cu", "pt100_blocks_per_cu".
+int64_t tuning(const char *key);
 
 // Device properties used for launch sizing.
 struct DeviceInfo {

@@ -51,7 +51,23 @@ __device__ __forceinline__ void unit_put(typename UnitOf<ES, G>::T &v, int k,
   }
 }
 
-template <int ES, int G, int NT>
+// Units of chunk `c` owned by this thread in round `r` are u = r*NT*UPT + tid + k*NT.
+template <int ES, int G, int NT, int UPT>
+__device__ __forceinline__ void bs_load_round(const Bs010Params &p, uint64_t c, uint32_t r,
+                                              typename UnitOf<ES, G>::T (&v)[UPT]) {
+  using U = typename UnitOf<ES, G>::T;
+  const uint64_t t0 = c * p.g;
+  const uint32_t tiles = uint32_t(std::min<uint64_t>(p.g, p.A - t0));
+  const uint32_t nU = tiles * p.n / G;
+  const U *gu = reinterpret_cast<const U *>(p.base + t0 * uint64_t(p.n) * ES);
+#pragma unroll
+  for (int k = 0; k < UPT; k++) {
+    uint32_t u = r * (NT * UPT) + threadIdx.x + k * NT;
+    if (u < nU) v[k] = __builtin_nontemporal_load(gu + u);
+  }
+}
+
+template <int ES, int G, int NT, int UPT>
 __global__ __launch_bounds__(NT) void k_010_bs(Bs010Params p) {
   using E = typename Elem<ES>::T;
   using U = typename UnitOf<ES, G>::T;
@@ -59,75 +75,92 @@ __global__ __launch_bounds__(NT) void k_010_bs(Bs010Params p) {
   E *lds = reinterpret_cast<E *>(lds_raw);
   const uint32_t tid = threadIdx.x;
   const uint32_t a = p.a, B = p.B, Bp = p.Bp, n = p.n;
+  constexpr uint32_t RU = NT * UPT;  // units per round
 
-  for (uint64_t c = blockIdx.x; c < p.nchunks; c += gridDim.x) {
+  uint64_t c = blockIdx.x;
+  if (c >= p.nchunks) return;
+  U v[UPT];
+  bs_load_round<ES, G, NT, UPT>(p, c, 0, v);
+
+  while (true) {
     const uint64_t t0 = c * p.g;
     const uint32_t tiles = uint32_t(std::min<uint64_t>(p.g, p.A - t0));
     const uint32_t C = tiles * n;
     const uint32_t nU = C / G;
     unsigned char *gbase = p.base + t0 * uint64_t(n) * ES;
     U *gu = reinterpret_cast<U *>(gbase);
+    const uint32_t rounds = (nU + RU - 1) / RU;
 
-    // ---- stage in: coalesced units -> padded LDS rows
-    {
-      uint32_t f = tid * G;
+    // ---- stage in: registers -> padded LDS rows (round 0 was loaded ahead of time)
+    for (uint32_t r = 0; r < rounds; r++) {
+      if (r > 0) bs_load_round<ES, G, NT, UPT>(p, c, r, v);
+      uint32_t f = (r * RU + tid) * G;
       uint32_t t = p.div_n.div(f);
-      uint32_t r = f - t * n;
-      uint32_t i = p.div_B.div(r);
-      uint32_t j = r - i * B;
-      for (uint32_t u = tid; u < nU; u += NT) {
-        U v = gu[u];
+      uint32_t rr = f - t * n;
+      uint32_t i = p.div_B.div(rr);
+      uint32_t j = rr - i * B;
 #pragma unroll
-        for (int k = 0; k < G; k++) {
-          lds[(t * a + i) * Bp + j] = unit_get<ES, G>(v, k);
-          if (k < G - 1) {
-            if (++j == B) { j = 0; if (++i == a) { i = 0; ++t; } }
+      for (int k = 0; k < UPT; k++) {
+        uint32_t u = r * RU + tid + k * NT;
+        if (u < nU) {
+#pragma unroll
+          for (int e = 0; e < G; e++) {
+            lds[(t * a + i) * Bp + j] = unit_get<ES, G>(v[k], e);
+            if (e < G - 1) {
+              if (++j == B) { j = 0; if (++i == a) { i = 0; ++t; } }
+            }
           }
         }
         j += p.din_j; if (j >= B) { j -= B; ++i; }
         i += p.din_i; if (i >= a) { i -= a; ++t; }
         t += p.din_t;
       }
-      if constexpr (G > 1) {
-        for (uint32_t f2 = nU * G + tid; f2 < C; f2 += NT) {
-          uint32_t tt = p.div_n.div(f2), rr = f2 - tt * n;
-          uint32_t ii = p.div_B.div(rr), jj = rr - ii * B;
-          lds[(tt * a + ii) * Bp + jj] = reinterpret_cast<E *>(gbase)[f2];
-        }
+    }
+    if constexpr (G > 1) {
+      for (uint32_t f2 = nU * G + tid; f2 < C; f2 += NT) {
+        uint32_t tt = p.div_n.div(f2), rr = f2 - tt * n;
+        uint32_t ii = p.div_B.div(rr), jj = rr - ii * B;
+        lds[(tt * a + ii) * Bp + jj] = reinterpret_cast<E *>(gbase)[f2];
       }
     }
     __syncthreads();
+
+    // ---- prefetch the next chunk (a disjoint region) while this one is written back
+    const uint64_t cn = c + gridDim.x;
+    if (cn < p.nchunks) bs_load_round<ES, G, NT, UPT>(p, cn, 0, v);
 
     // ---- gather out: output position o = t*n + j*a + i reads LDS (t, i, j)
     {
       uint32_t o = tid * G;
       uint32_t t = p.div_n.div(o);
-      uint32_t r = o - t * n;
-      uint32_t j = p.div_a.div(r);
-      uint32_t i = r - j * a;
+      uint32_t rr = o - t * n;
+      uint32_t j = p.div_a.div(rr);
+      uint32_t i = rr - j * a;
       for (uint32_t u = tid; u < nU; u += NT) {
-        U v{};
+        U w{};
 #pragma unroll
-        for (int k = 0; k < G; k++) {
-          unit_put<ES, G>(v, k, lds[(t * a + i) * Bp + j]);
-          if (k < G - 1) {
+        for (int e = 0; e < G; e++) {
+          unit_put<ES, G>(w, e, lds[(t * a + i) * Bp + j]);
+          if (e < G - 1) {
             if (++i == a) { i = 0; if (++j == B) { j = 0; ++t; } }
           }
         }
-        gu[u] = v;
+        __builtin_nontemporal_store(w, gu + u);
         i += p.dout_i; if (i >= a) { i -= a; ++j; }
         j += p.dout_j; if (j >= B) { j -= B; ++t; }
         t += p.dout_t;
       }
       if constexpr (G > 1) {
         for (uint32_t o2 = nU * G + tid; o2 < C; o2 += NT) {
-          uint32_t tt = p.div_n.div(o2), rr = o2 - tt * n;
-          uint32_t jj = p.div_a.div(rr), ii = rr - jj * a;
+          uint32_t tt = p.div_n.div(o2), rr2 = o2 - tt * n;
+          uint32_t jj = p.div_a.div(rr2), ii = rr2 - jj * a;
           reinterpret_cast<E *>(gbase)[o2] = lds[(tt * a + ii) * Bp + jj];
         }
       }
     }
     __syncthreads();
+    if (cn >= p.nchunks) break;
+    c = cn;
   }
 }
 
@@ -194,10 +227,10 @@ __global__ __launch_bounds__(NT) void k_010_pttwac(Pt010Params p) {
 // ---------------------------------------------------------------------------------------
 namespace {
 
-template <int ES, int G, int NT>
+template <int ES, int G, int NT, int UPT>
 hipError_t launch_bs_t(const Bs010Params &p, size_t lds_bytes, uint32_t blocks,
                        hipStream_t st) {
-  auto k = k_010_bs<ES, G, NT>;
+  auto k = k_010_bs<ES, G, NT, UPT>;
   if (lds_bytes > 65536) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -227,8 +260,9 @@ hipError_t launch_pt_t(const Pt010Params &p, size_t lds_bytes, uint32_t blocks,
 uint32_t bs010_pitch(uint32_t B) { return (B & 1u) ? B : B + 1; }
 
 bool plan_010_bs(size_t es, uint64_t A, uint64_t a, uint64_t B, size_t lds_budget,
-                 uint64_t base_addr, size_t target_chunk_bytes, Bs010Plan *out) {
+                 uint64_t base_addr, int upt, Bs010Plan *out) {
   if (A == 0 || a == 0 || B == 0) return false;
+  if (upt != 4 && upt != 8 && upt != 16) upt = 16;
   uint64_t Bp = bs010_pitch(uint32_t(std::min<uint64_t>(B, 0xffffffffu)));
   uint64_t tile_lds = a * Bp * es;
   if (a > 0xffffu * 16 || B > 0xffffu * 16 || tile_lds > lds_budget) return false;
@@ -243,18 +277,33 @@ bool plan_010_bs(size_t es, uint64_t A, uint64_t a, uint64_t B, size_t lds_budge
     if ((base_addr % 4) == 0 && g0 * tile_lds <= lds_budget) G = int(4 / es);
     else g0 = 1;
   }
-  uint64_t g = std::max<uint64_t>(1, target_chunk_bytes / std::max<uint64_t>(1, n * es));
+  // One round of 256 threads x upt units per chunk when tiles are small.
+  uint64_t target = uint64_t(256) * upt * G;
+  uint64_t g = std::max<uint64_t>(1, target / n);
   g = std::min<uint64_t>(g, lds_budget / tile_lds);
   g = std::min<uint64_t>(g, A);
   g = std::max<uint64_t>(g0, (g / g0) * g0);
   if (g * tile_lds > lds_budget) g = g0;
   out->G = G;
   out->g = g;
+  out->upt = upt;
   out->Bp = uint32_t(Bp);
   out->lds_bytes = size_t(g * tile_lds);
   out->nchunks = (A + g - 1) / g;
   return true;
 }
+
+namespace {
+template <int ES, int G>
+hipError_t launch_bs_g(const Bs010Params &p, const Bs010Plan &pl, uint32_t blocks,
+                       hipStream_t st) {
+  switch (pl.upt) {
+    case 4: return launch_bs_t<ES, G, 256, 4>(p, pl.lds_bytes, blocks, st);
+    case 8: return launch_bs_t<ES, G, 256, 8>(p, pl.lds_bytes, blocks, st);
+    default: return launch_bs_t<ES, G, 256, 16>(p, pl.lds_bytes, blocks, st);
+  }
+}
+}  // namespace
 
 hipError_t launch_010_bs(unsigned char *base, size_t es, uint64_t A, uint64_t a, uint64_t B,
                          const Bs010Plan &pl, uint32_t max_blocks, hipStream_t st) {
@@ -271,7 +320,7 @@ hipError_t launch_010_bs(unsigned char *base, size_t es, uint64_t A, uint64_t a,
   p.div_n = FastDiv32(p.n);
   p.div_B = FastDiv32(p.B);
   p.div_a = FastDiv32(p.a);
-  // Per-iteration advance: NT units of G elements, minus the G-1 single steps taken inside.
+  // Per-unit advance: NT units of G elements, minus the G-1 single steps taken inside.
   uint32_t D = uint32_t(NT * pl.G - (pl.G - 1));
   p.din_t = D / p.n;
   p.din_i = (D % p.n) / p.B;
@@ -281,13 +330,13 @@ hipError_t launch_010_bs(unsigned char *base, size_t es, uint64_t A, uint64_t a,
   p.dout_i = (D % p.n) % p.a;
   uint32_t blocks = uint32_t(std::min<uint64_t>(pl.nchunks, max_blocks));
   switch (es) {
-    case 1: return pl.G == 4 ? launch_bs_t<1, 4, NT>(p, pl.lds_bytes, blocks, st)
-                             : launch_bs_t<1, 1, NT>(p, pl.lds_bytes, blocks, st);
-    case 2: return pl.G == 2 ? launch_bs_t<2, 2, NT>(p, pl.lds_bytes, blocks, st)
-                             : launch_bs_t<2, 1, NT>(p, pl.lds_bytes, blocks, st);
-    case 4: return launch_bs_t<4, 1, NT>(p, pl.lds_bytes, blocks, st);
-    case 8: return launch_bs_t<8, 1, NT>(p, pl.lds_bytes, blocks, st);
-    case 16: return launch_bs_t<16, 1, NT>(p, pl.lds_bytes, blocks, st);
+    case 1: return pl.G == 4 ? launch_bs_g<1, 4>(p, pl, blocks, st)
+                             : launch_bs_g<1, 1>(p, pl, blocks, st);
+    case 2: return pl.G == 2 ? launch_bs_g<2, 2>(p, pl, blocks, st)
+                             : launch_bs_g<2, 1>(p, pl, blocks, st);
+    case 4: return launch_bs_g<4, 1>(p, pl, blocks, st);
+    case 8: return launch_bs_g<8, 1>(p, pl, blocks, st);
+    case 16: return launch_bs_g<16, 1>(p, pl, blocks, st);
   }
   return hipErrorInvalidValue;
 }

@@ -1,0 +1,72 @@
+"""Turns rocprofv3 CSV output into the short summaries kept under profiles/.
+
+    python -m libmarshal_amd.profsummary stats  <kernel_stats.csv> <out.txt> <command...>
+    python -m libmarshal_amd.profsummary pmc    <fetch_counter_collection.csv>
+                                                <write_counter_collection.csv>
+                                                <kernel-substring> <copy-bytes> <out.json>
+
+`pmc` scales the FETCH_SIZE / WRITE_SIZE readings (KiB per dispatch) of the named kernel by
+the factor that maps the same counter on the calibration copy kernel (__amd_rocclr_copyBuffer,
+a device copy of <copy-bytes> bytes: it reads and writes exactly that many) to its known
+traffic, and reports HBM bytes per launch.
+"""
+import csv
+import json
+import statistics
+import sys
+
+
+def stats(csv_path, out_path, command):
+    rows = list(csv.DictReader(open(csv_path)))
+    lines = [f"# rocprofv3 kernel statistics", f"# command: {command}", "",
+             f"{'avg_us':>10} {'min_us':>10} {'max_us':>10} {'calls':>6}  kernel"]
+    for r in rows:
+        lines.append(f"{float(r['AverageNs']) / 1e3:10.2f} {float(r['MinNs']) / 1e3:10.2f} "
+                     f"{float(r['MaxNs']) / 1e3:10.2f} {int(r['Calls']):6d}  {r['Name'][:110]}")
+    open(out_path, "w").write("\n".join(lines) + "\n")
+
+
+def _per_kernel(path, counter):
+    vals = {}
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return vals
+
+
+def pmc(fetch_csv, write_csv, kernel_sub, copy_bytes, out_path):
+    f = _per_kernel(fetch_csv, "FETCH_SIZE")
+    w = _per_kernel(write_csv, "WRITE_SIZE")
+
+    def pick(d, sub):
+        ks = [k for k in d if sub in k]
+        if not ks:
+            raise SystemExit(f"no kernel matching {sub!r}")
+        return ks[0], statistics.mean(d[ks[0]])
+
+    kname, kf = pick(f, kernel_sub)
+    _, kw = pick(w, kernel_sub)
+    _, cf = pick(f, "__amd_rocclr_copyBuffer")
+    _, cw = pick(w, "__amd_rocclr_copyBuffer")
+    fs = copy_bytes / (cf * 1024.0)
+    ws = copy_bytes / (cw * 1024.0)
+    res = {
+        "kernel": kname,
+        "fetch_kib_per_dispatch": kf, "write_kib_per_dispatch": kw,
+        "copy_fetch_kib": cf, "copy_write_kib": cw, "copy_bytes": copy_bytes,
+        "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+        "hbm_read_bytes_per_launch": round(kf * 1024 * fs),
+        "hbm_write_bytes_per_launch": round(kw * 1024 * ws),
+    }
+    res["hbm_bytes_per_launch"] = res["hbm_read_bytes_per_launch"] + \
+        res["hbm_write_bytes_per_launch"]
+    json.dump(res, open(out_path, "w"), indent=1)
+    return res
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "stats":
+        stats(sys.argv[2], sys.argv[3], " ".join(sys.argv[4:]))
+    elif sys.argv[1] == "pmc":
+        print(pmc(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]), sys.argv[6]))
