@@ -79,7 +79,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                          "(the multi-GPU sharding design is described in DESIGN.md)"}))
+                          "(a transpose larger than one GPU shards by rows with one all-to-all exchange)"}))
         sys.exit(2)
     import torch
     import libmarshal_amd as lm
