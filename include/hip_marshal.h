@@ -136,6 +136,42 @@ bool hip_unpadding_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t 
                       int64_t y_size, int64_t pad_size, uint64_t *elapsed_ns);
 
 /* ---------------------------------------------------------------------------------------
+ * Batched form: many conversions of uneven shapes and element sizes in one graph launch.
+ * No counterpart in the reference (it launches one kernel per stage per matrix).
+ * hip_marshal_batch_create plans every descriptor (the planner picks tile factors, routes and
+ * kernel forms), groups the stages of all matrices by level and kernel form, and uploads the
+ * work lists; it returns NULL on an invalid descriptor. hip_marshal_batch_run converts every
+ * buffer in place: with use_graph the sequence (bit clearing + one persistent kernel per
+ * group and level) is captured into a hipGraph on the first run and replayed afterwards.
+ * Each buffer ends bit-identical to what the plain entry point gives for it alone. Buffers
+ * must not overlap; a batch's own visited-bit space is private to it, so different batches
+ * may run concurrently on different streams, but one batch must not run twice at once.
+ * ------------------------------------------------------------------------------------- */
+enum {
+  HIP_MARSHAL_OP_TRANSPOSE = 0, /* rows x cols -> cols x rows                     */
+  HIP_MARSHAL_OP_AOS_ASTA = 1,  /* [rows][cols] -> [rows/tile][cols][tile]        */
+  HIP_MARSHAL_OP_SOA_ASTA = 2,  /* [cols][rows] -> [rows/tile][cols][tile]        */
+  HIP_MARSHAL_OP_ASTA_AOS = 3,  /* inverse of AOS_ASTA                            */
+  HIP_MARSHAL_OP_ASTA_SOA = 4   /* inverse of SOA_ASTA                            */
+};
+typedef struct hip_marshal_desc {
+  void *ptr;          /* device buffer, naturally aligned to min(elem_bytes, 16) */
+  int32_t op;         /* HIP_MARSHAL_OP_* */
+  int32_t elem_bytes; /* 1, 2, 4, 8 or 16 */
+  int64_t rows;       /* transpose: M; AoS/SoA/ASTA: height (number of structures) */
+  int64_t cols;       /* transpose: N; AoS/SoA/ASTA: width (fields per structure) */
+  int64_t tile;       /* ASTA tile width (divides rows); ignored by TRANSPOSE */
+} hip_marshal_desc;
+typedef struct hip_marshal_batch hip_marshal_batch;
+hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64_t count,
+                                            int use_graph);
+bool hip_marshal_batch_run(hip_marshal_batch *batch, hipStream_t stream, uint64_t *elapsed_ns);
+/* "levels", "stages", "kernels", "flag_bytes", "bytes" (algorithmic bytes of one run). */
+int64_t hip_marshal_batch_info(const hip_marshal_batch *batch, const char *key);
+const char *hip_marshal_batch_route(const hip_marshal_batch *batch, int64_t index);
+void hip_marshal_batch_destroy(hip_marshal_batch *batch);
+
+/* ---------------------------------------------------------------------------------------
  * Library configuration and introspection (no counterpart in the reference, where these
  * are compile-time macros: MAX_MEM src/cl_marshal.cc:45-59, LIBMARSHAL_OCL_PROFILE).
  * ------------------------------------------------------------------------------------- */
@@ -148,10 +184,16 @@ void hip_marshal_set_profiling(int on);
 /* Launch-shape knobs: "bs_upt" (4, 8, 16 units per lane per round of the LDS-staged 010),
  * "bs_blocks_per_cu", "pt010_blocks_per_cu", "pt100_blocks_per_cu" (grid caps). Replaces the
  * reference's hard-coded work-group tables (src/cl_marshal.cc:165-231, 388-465). The setter
- * returns true on an unknown key or a value outside [1, 1024]; the getter returns 0 for an
+ * returns true on an unknown key or a value outside [1, 2^30]; the getter returns 0 for an
  * unknown key. */
 bool hip_marshal_set_tuning(const char *key, int64_t value);
 int64_t hip_marshal_get_tuning(const char *key);
+/* The planner's choice for an M x N transpose of elem_bytes elements: tile factors a | M,
+ * b | N and the stage sequence (2, 22, 3, 32; -1 = element-granular fallback). Host-only
+ * (no device work). Replaces the tile heuristic of the reference's test
+ * (test/cl_unittest.cc:396-495, "Alg. 5" and "Alg. 3"). */
+bool hip_marshal_plan_transpose(int64_t M, int64_t N, size_t elem_bytes, int64_t *a,
+                                int64_t *b, int *stages);
 /* Route tag of the last hip_transpose* call on this thread ("3.1-010_BS" ...). */
 const char *hip_marshal_last_route(void);
 /* Grows the current device's flag pool to at least `bytes` (false on success). */

@@ -18,6 +18,14 @@ LIB_PATH = os.path.join(_HERE, "libmarshal_hip.so")
 
 _lib = None
 
+OP_TRANSPOSE, OP_AOS_ASTA, OP_SOA_ASTA, OP_ASTA_AOS, OP_ASTA_SOA = 0, 1, 2, 3, 4
+
+
+class Desc(ctypes.Structure):
+    """hip_marshal_desc of include/hip_marshal.h."""
+    _fields_ = [("ptr", ctypes.c_void_p), ("op", ctypes.c_int32), ("elem_bytes", ctypes.c_int32),
+                ("rows", ctypes.c_int64), ("cols", ctypes.c_int64), ("tile", ctypes.c_int64)]
+
 
 def lib():
     """Loads the HIP library (raises if it has not been built)."""
@@ -64,6 +72,14 @@ def lib():
         "hip_marshal_flag_bytes_0100": (sz, [i64, i64, i64, i64, sz]),
         "hip_marshal_version": (ctypes.c_char_p, []),
         "hip_marshal_set_tuning": (b, [ctypes.c_char_p, i64]),
+        "hip_marshal_plan_transpose": (b, [i64, i64, sz, ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int)]),
+        "hip_marshal_batch_create": (vp, [ctypes.POINTER(Desc), i64, i32]),
+        "hip_marshal_batch_run": (b, [vp, vp, u64p]),
+        "hip_marshal_batch_info": (i64, [vp, ctypes.c_char_p]),
+        "hip_marshal_batch_route": (ctypes.c_char_p, [vp, i64]),
+        "hip_marshal_batch_destroy": (None, [vp]),
         "hip_marshal_get_tuning": (i64, [ctypes.c_char_p]),
         "gpu_aos_asta_bs_float": (b, [vp, i32, i32, i32, vp]),
         "gpu_aos_asta_bs_double": (b, [vp, i32, i32, i32, vp]),
@@ -89,7 +105,8 @@ EXPORTED = [
     "hip_unpadding_es", "hip_marshal_set_lds_budget", "hip_marshal_get_lds_budget",
     "hip_marshal_set_profiling", "hip_marshal_last_route", "hip_marshal_reserve",
     "hip_marshal_flag_bytes_0100", "hip_marshal_version", "hip_marshal_set_tuning",
-    "hip_marshal_get_tuning",
+    "hip_marshal_get_tuning", "hip_marshal_plan_transpose", "hip_marshal_batch_create", "hip_marshal_batch_run",
+    "hip_marshal_batch_info", "hip_marshal_batch_route", "hip_marshal_batch_destroy",
     "gpu_aos_asta_bs_float", "gpu_aos_asta_bs_double", "gpu_aos_asta_pttwac",
     "gpu_soa_asta_pttwac",
 ]
@@ -224,12 +241,60 @@ def get_tuning(key):
     return lib().hip_marshal_get_tuning(key.encode())
 
 
+def plan_transpose(M, N, elem_bytes):
+    """(a, b, stages) the planner picks for an M x N transpose (host only)."""
+    a, b, st = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+    if lib().hip_marshal_plan_transpose(M, N, elem_bytes, ctypes.byref(a), ctypes.byref(b),
+                                        ctypes.byref(st)):
+        raise ValueError("bad arguments")
+    return a.value, b.value, st.value
+
+
 def last_route():
     return lib().hip_marshal_last_route().decode()
 
 
 def finalize():
     lib().hip_marshal_finalize()
+
+
+class Batch:
+    """Many conversions in one graph launch (hip_marshal_batch_*).
+
+    items: iterable of (tensor, op, elem_bytes, rows, cols, tile) with op one of OP_*.
+    The tensors must stay alive while the batch exists.
+    """
+
+    def __init__(self, items, use_graph=True):
+        items = list(items)
+        arr = (Desc * len(items))()
+        self._keep = []
+        for i, (t, op, es, rows, cols, tile) in enumerate(items):
+            arr[i] = Desc(_ptr(t).value, op, es, rows, cols, tile)
+            self._keep.append(t)
+        self._h = lib().hip_marshal_batch_create(arr, len(items), 1 if use_graph else 0)
+        if not self._h:
+            raise ValueError("hip_marshal_batch_create failed")
+
+    def run(self, stream=None, elapsed=None):
+        return lib().hip_marshal_batch_run(self._h, _stream(stream), _et(elapsed))
+
+    def info(self, key):
+        return lib().hip_marshal_batch_info(self._h, key.encode())
+
+    def route(self, i):
+        return lib().hip_marshal_batch_route(self._h, i).decode()
+
+    def close(self):
+        if self._h:
+            lib().hip_marshal_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def TransposeArrayInPlace(arr, a=1, b=1, stages=None, stream=None):

@@ -14,7 +14,8 @@ OUT = os.path.join(HERE, "libmarshal_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["kernels_010.hip", "kernels_100.hip", "kernels_pad.hip", "batch.hip", "api.hip"]
+SOURCES = ["kernels_010.hip", "kernels_100.hip", "kernels_rc.hip", "kernels_pad.hip", "batch.hip",
+           "api.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -70,7 +70,12 @@ Knob g_knobs[] = {
     {"bs_upt", {16}},
     {"bs_blocks_per_cu", {12}},
     {"pt010_blocks_per_cu", {8}},
-    {"pt100_blocks_per_cu", {16}},
+    {"pt100_vw_max", {8}},
+    {"pt100_blocks_per_cu", {32}},
+    {"lds0100_min_inst", {32}},
+    {"rc_row_blocks_per_cu", {2}},
+    {"rc_max_super_bytes", {64}},
+    {"rc_col_blocks_per_cu", {1}},
 };
 Knob *find_knob(const char *key) {
   if (!key) return nullptr;
@@ -247,135 +252,235 @@ uint32_t grid_cap(uint32_t per_cu) { return uint32_t(device_info().cus) * per_cu
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
-// Stages (no timing, no argument checks beyond what the kernels need)
+// Stage records: a route is a list of elementary stages with their chosen form. A plain call
+// builds the list and runs it on its stream; a batch collects the lists of many matrices.
 // ---------------------------------------------------------------------------------------
-enum class M010 { Auto, BS, PTTWAC, Global };
-
-static bool stage_0100(unsigned char *buf, size_t es, uint64_t inst, uint64_t R, uint64_t C,
-                       uint64_t b, hipStream_t st) {
-  if (inst == 0 || R * C <= 1) return false;
-  if (R == 1 || C == 1) return false;  // 1 x C or R x 1 grids are their own transpose
-  Pt100Plan pl{};
-  plan_100(es, inst, R, C, b, &pl);
-  Lease lease(pool_here(), st);
-  uint32_t *flags = lease.get(pl.flag_words * 4);
-  if (!flags) return true;
-  LM_CHECK_HIP(launch_fill_u32(flags, pl.flag_words, 0, st));
-  LM_CHECK_HIP(launch_100_pttwac(buf, flags, inst, R, C, b * es, pl,
-                                 grid_cap(uint32_t(tuning("pt100_blocks_per_cu"))), st));
-  return false;
-}
+size_t current_lds_budget() { return lds_budget(); }
 
 // Which form a 010 step can use under the current LDS budget.
-static M010 choose_010(size_t es, uint64_t A, uint64_t a, uint64_t B, int R, int P,
-                       const void *buf) {
+M010 choose_010(size_t es, uint64_t A, uint64_t a, uint64_t B, int R, int P,
+                uint64_t base_addr) {
   Bs010Plan pl{};
-  if (plan_010_bs(es, A, a, B, lds_budget(), reinterpret_cast<uintptr_t>(buf), int(tuning("bs_upt")),
-                  &pl))
+  if (plan_010_bs(es, A, a, B, lds_budget(), base_addr, int(tuning("bs_upt")), &pl))
     return M010::BS;
   if (a * B < (1ull << 31) && pt010_flag_words(a, B, R, P) * 4 <= lds_budget())
     return M010::PTTWAC;
   return M010::Global;
 }
 
-static bool stage_010(unsigned char *buf, size_t es, uint64_t A, uint64_t a, uint64_t B,
-                      M010 m, int R, int P, hipStream_t st, const char **tag) {
-  if (A == 0 || a <= 1 || B <= 1) {  // 1 x B and a x 1 tiles are unchanged by 010
-    if (tag) *tag = "";
-    return false;
-  }
-  if (m == M010::Auto) m = choose_010(es, A, a, B, R, P, buf);
+void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t inst,
+               uint64_t R, uint64_t C, uint64_t b) {
+  if (inst == 0 || R <= 1 || C <= 1) return;  // 1 x C and R x 1 grids are unchanged
+  // A single 100 over short super-elements is a full R x C transpose of (b*es)-byte
+  // elements: the LDS-staged row / column decomposition beats cycle following there.
+  if (inst == 1 && b * es < uint64_t(tuning("rc_max_super_bytes")) &&
+      push_rc(out, buf, es, R, C, b * es))
+    return;
+  StageRec r{};
+  r.es = es;
+  r.buf = buf;
+  r.d0 = inst;
+  r.d1 = R;
+  r.d2 = C;
+  r.d3 = b;
+  plan_100(es, inst, R, C, b, &r.pt);
+  // Many small instances: visited bits of each instance in LDS, one workgroup per instance.
+  if (inst >= uint64_t(tuning("lds0100_min_inst")) && R * C * r.pt.nsub < (1ull << 31) &&
+      lds0100_bytes(R, C, r.pt) <= lds_budget())
+    r.kind = StageKind::PT0100L;
+  else
+    r.kind = StageKind::PT100G;
+  out.push_back(r);
+}
+
+bool push_010(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t A,
+              uint64_t a, uint64_t B, M010 m, int R, int P, const char **tag) {
+  if (tag) *tag = "";
+  if (A == 0 || a <= 1 || B <= 1) return true;  // 1 x B and a x 1 tiles are unchanged
+  if (m == M010::Auto) m = choose_010(es, A, a, B, R, P, reinterpret_cast<uintptr_t>(buf));
+  StageRec r{};
+  r.es = es;
+  r.buf = buf;
+  r.d0 = A;
+  r.d1 = a;
+  r.d2 = B;
+  r.R = R;
+  r.P = P;
   if (m == M010::BS) {
-    Bs010Plan pl{};
     if (!plan_010_bs(es, A, a, B, lds_budget(), reinterpret_cast<uintptr_t>(buf),
-                     int(tuning("bs_upt")), &pl)) {
+                     int(tuning("bs_upt")), &r.bs)) {
       fprintf(stderr, "[libmarshal] 010 BS: tile %llux%llu does not fit the LDS budget\n",
               (unsigned long long)a, (unsigned long long)B);
-      return true;
+      return false;
     }
+    r.kind = StageKind::BS010;
     if (tag) *tag = "010_BS-";
-    LM_CHECK_HIP(launch_010_bs(buf, es, A, a, B, pl,
-                               grid_cap(uint32_t(tuning("bs_blocks_per_cu"))), st));
-    return false;
+    out.push_back(r);
+    return true;
   }
   if (m == M010::PTTWAC) {
     if (R < 1 || P < 0 || a * B >= (1ull << 31) ||
         pt010_flag_words(a, B, R, P) * 4 > lds_budget()) {
       fprintf(stderr, "[libmarshal] 010 PTTWAC: flags of a %llux%llu tile do not fit LDS\n",
               (unsigned long long)a, (unsigned long long)B);
-      return true;
+      return false;
     }
+    r.kind = StageKind::PT010;
     if (tag) *tag = "010_PTTWAC-";
-    LM_CHECK_HIP(launch_010_pttwac(buf, es, A, a, B, R, P,
-                                   grid_cap(uint32_t(tuning("pt010_blocks_per_cu"))), st));
-    return false;
+    out.push_back(r);
+    return true;
   }
   // 010 on [A][a][B] is 0100 with single-element super-elements: global visited bits.
   if (tag) *tag = "010_G-";
-  return stage_0100(buf, es, A, a, B, 1, st);
+  push_0100(out, buf, es, A, a, B, 1);
+  return true;
 }
 
-// ---------------------------------------------------------------------------------------
-// Composer: stage sequences of cl_transpose (src/cl_marshal.cc:561-776).
-// ---------------------------------------------------------------------------------------
-static bool compose(unsigned char *buf, size_t es, uint64_t A, uint64_t a, uint64_t B,
-                    uint64_t b, int R, int stages, hipStream_t st, std::string &route) {
+bool push_rc(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t M,
+             uint64_t N, uint64_t S) {
+  if (M <= 1 || N <= 1) return true;  // a vector is its own transpose
+  RcPlan pl{};
+  if (!plan_rc(es, M, N, S, lds_budget(), &pl)) return false;
+  StageRec r{};
+  r.es = es;
+  r.buf = buf;
+  r.d1 = M;
+  r.d2 = N;
+  r.d3 = S;
+  r.rc = pl;
+  if (pl.c > 1) {
+    r.kind = StageKind::RC_COL1;
+    out.push_back(r);
+  }
+  r.kind = StageKind::RC_ROW;
+  out.push_back(r);
+  r.kind = StageKind::RC_COL2;
+  out.push_back(r);
+  return true;
+}
+
+// Composer: the stage sequences of cl_transpose (src/cl_marshal.cc:561-776).
+bool route_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t A,
+                  uint64_t a, uint64_t B, uint64_t b, int R, int stages, std::string &route) {
   route.clear();
   const char *tag = "";
   auto s010 = [&](uint64_t A_, uint64_t a_, uint64_t B_) {
-    bool e = stage_010(buf, es, A_, a_, B_, M010::Auto, R, 1, st, &tag);
+    bool ok = push_010(out, buf, es, A_, a_, B_, M010::Auto, R, 1, &tag);
     route += tag;
-    return e;
+    return ok;
   };
   switch (stages) {
     case 2:  // AaBb -> ABba: 010(A, a, Bb), then 100(A, Bb, a)   (src/cl_marshal.cc:569-601)
-      if (s010(A, a, B * b)) return true;
-      if (stage_0100(buf, es, 1, A, B * b, a, st)) return true;
+      if (!s010(A, a, B * b)) return false;
+      push_0100(out, buf, es, 1, A, B * b, a);
       route += "2.1-";
-      return false;
+      return true;
     case 22:  // 100(Aa, B, b), then 010(B, Aa, b)               (src/cl_marshal.cc:602-634)
-      if (stage_0100(buf, es, 1, A * a, B, b, st)) return true;
-      if (s010(B, A * a, b)) return true;
+      push_0100(out, buf, es, 1, A * a, B, b);
+      if (!s010(B, A * a, b)) return false;
       route += "2.2-";
-      return false;
+      return true;
     case 3:  // AaBb -> BAab -> BAba -> BbAa                      (src/cl_marshal.cc:636-677)
       if (a > 1 && b > 1) {
-        if (stage_0100(buf, es, 1, A * a, B, b, st)) return true;
-        if (s010(B * A, a, b)) return true;
-        if (stage_0100(buf, es, B, A, b, a, st)) return true;
+        push_0100(out, buf, es, 1, A * a, B, b);
+        if (!s010(B * A, a, b)) return false;
+        push_0100(out, buf, es, B, A, b, a);
         route += "3.1-";
-        return false;
+        return true;
       }
       break;
     case 32:  // AaBb -> ABab -> ABba -> BbAa                     (src/cl_marshal.cc:679-720)
       if (a > 1 && b > 1) {
-        if (stage_0100(buf, es, A, a, B, b, st)) return true;
-        if (s010(A * B, a, b)) return true;
-        if (stage_0100(buf, es, 1, A, B * b, a, st)) return true;
+        push_0100(out, buf, es, A, a, B, b);
+        if (!s010(A * B, a, b)) return false;
+        push_0100(out, buf, es, 1, A, B * b, a);
         route += "3.2-";
-        return false;
+        return true;
+      }
+      break;
+    case 5:  // row / column decomposition (no counterpart in the reference)
+      if (push_rc(out, buf, es, A * a, B * b, es)) {
+        route += "RC-";
+        return true;
       }
       break;
     case 4:  // Karlsson: 0100, 010, 100(A, B, ba), 0100          (src/cl_marshal.cc:722-767)
-      if (stage_0100(buf, es, A, a, B, b, st)) return true;
-      if (s010(B * A, a, b)) return true;
-      if (stage_0100(buf, es, 1, A, B, b * a, st)) return true;
-      if (stage_0100(buf, es, B, A, b, a, st)) return true;
+      push_0100(out, buf, es, A, a, B, b);
+      if (!s010(B * A, a, b)) return false;
+      push_0100(out, buf, es, 1, A, B, b * a);
+      push_0100(out, buf, es, B, A, b, a);
       route += "4-";
-      return false;
+      return true;
     default:
       break;
   }
   // Element-granular fallback: 100 over the whole matrix (src/cl_marshal.cc:769-775).
   route += "Fallback-";
-  return stage_0100(buf, es, 1, A * a, B * b, 1, st);
+  push_0100(out, buf, es, 1, A * a, B * b, 1);
+  return true;
+}
+
+// Runs stage records in order on one stream.
+bool run_stages(const std::vector<StageRec> &stages, hipStream_t st) {
+  for (const StageRec &r : stages) {
+    switch (r.kind) {
+      case StageKind::BS010:
+        LM_CHECK_HIP(launch_010_bs(r.buf, r.es, r.d0, r.d1, r.d2, r.bs,
+                                   grid_cap(uint32_t(tuning("bs_blocks_per_cu"))), st));
+        break;
+      case StageKind::PT010:
+        LM_CHECK_HIP(launch_010_pttwac(r.buf, r.es, r.d0, r.d1, r.d2, r.R, r.P,
+                                       grid_cap(uint32_t(tuning("pt010_blocks_per_cu"))), st));
+        break;
+      case StageKind::PT0100L:
+        LM_CHECK_HIP(launch_0100_lds(r.buf, r.d0, r.d1, r.d2, r.d3 * r.es, r.pt, grid_cap(8),
+                                     st));
+        break;
+      case StageKind::RC_ROW:
+      case StageKind::RC_COL1:
+      case StageKind::RC_COL2: {
+        int pass = r.kind == StageKind::RC_ROW ? 0 : r.kind == StageKind::RC_COL1 ? 1 : 2;
+        LM_CHECK_HIP(launch_rc(r.buf, r.d1, r.d2, r.rc, pass,
+                               grid_cap(uint32_t(tuning(pass ? "rc_col_blocks_per_cu"
+                                                             : "rc_row_blocks_per_cu"))),
+                               st));
+        break;
+      }
+      case StageKind::PT100G: {
+        Lease lease(pool_here(), st);
+        uint32_t *flags = lease.get(r.pt.flag_words * 4);
+        if (!flags) return true;
+        LM_CHECK_HIP(launch_fill_u32(flags, r.pt.flag_words, 0, st));
+        LM_CHECK_HIP(launch_100_pttwac(r.buf, flags, r.d0, r.d1, r.d2, r.d3 * r.es, r.pt,
+                                       grid_cap(uint32_t(tuning("pt100_blocks_per_cu"))), st));
+        break;
+      }
+    }
+  }
+  return false;
+}
+
+static bool stage_0100(unsigned char *buf, size_t es, uint64_t inst, uint64_t R, uint64_t C,
+                       uint64_t b, hipStream_t st) {
+  std::vector<StageRec> v;
+  push_0100(v, buf, es, inst, R, C, b);
+  return run_stages(v, st);
+}
+
+static bool stage_010(unsigned char *buf, size_t es, uint64_t A, uint64_t a, uint64_t B,
+                      M010 m, int R, int P, hipStream_t st) {
+  std::vector<StageRec> v;
+  if (!push_010(v, buf, es, A, a, B, m, R, P, nullptr)) return true;
+  return run_stages(v, st);
 }
 
 // ---------------------------------------------------------------------------------------
 // Planner: tile factors and route for an M x N transpose.
-// Cost model in "HBM pass equivalents": an LDS-staged pass costs 1; a cycle-following pass
-// over super-elements of S bytes costs 1 + 96/S (short moves waste most of each line);
-// LDS-flag and global-flag 010 passes move single elements and cost like S = es.
+// Each stage is priced as bytes / modelled bandwidth: LDS-staged 010 at the measured
+// ~5.5 TB/s (less for 1- and 2-byte elements), cycle following over super-elements of S
+// bytes at a bandwidth that grows with S (short moves waste most of each line) and with the
+// number of independent start points. Candidates: routes 2, 22, 3, 32 over all divisor
+// pairs, and the element-granular fallback.
 // ---------------------------------------------------------------------------------------
 namespace {
 std::vector<uint64_t> divisors(uint64_t n) {
@@ -389,50 +494,70 @@ std::vector<uint64_t> divisors(uint64_t n) {
   return lo;
 }
 
-double cost_pt(uint64_t S) { return 1.0 + 96.0 / double(S); }
-
-double cost_010(size_t es, uint64_t A, uint64_t a, uint64_t B) {
+// Seconds per byte of matrix for each stage form (relative units).
+double t_bs(size_t es) { return 1.0 / (es >= 4 ? 5.5 : es == 2 ? 4.5 : 2.8); }
+double t_pt(uint64_t S) {
+  double bw = 0.25 * std::pow(double(S), 0.45);  // ~0.5 TB/s at 4 B, ~3 TB/s at 256 B
+  return 1.0 / std::min(bw, 3.0);
+}
+double t_rc_row(size_t es) { return 1.0 / (es >= 4 ? 4.0 : es == 2 ? 3.0 : 1.5); }
+double t_rc_col(size_t es) { return 1.0 / (es >= 4 ? 2.5 : es == 2 ? 2.0 : 1.0); }
+double t_010(size_t es, uint64_t A, uint64_t a, uint64_t B) {
   if (a <= 1 || B <= 1) return 0.0;
   Bs010Plan pl{};
-  if (plan_010_bs(es, A, a, B, lds_budget(), 0, 16, &pl)) return 1.0;
-  return 3.0 * cost_pt(es);
+  if (plan_010_bs(es, A, a, B, lds_budget(), 0, 16, &pl)) return t_bs(es);
+  if (a * B < (1ull << 31) && pt010_flag_words(a, B, 1, 1) * 4 <= lds_budget())
+    return 2.0 * t_pt(es);
+  return 1.5 * t_pt(es);
 }
-
-struct Choice {
-  double cost = 1e300;
-  uint64_t a = 1, b = 1;
-  int stages = 0;
-};
 }  // namespace
 
-static Choice plan_full(size_t es, uint64_t M, uint64_t N) {
+Choice plan_full(size_t es, uint64_t M, uint64_t N) {
   Choice best;
-  best.cost = cost_pt(es);  // fallback
+  best.cost = t_pt(es);  // fallback
   best.a = M;
   best.b = N;
   best.stages = -1;
+  if (M <= 1 || N <= 1) return best;
+  RcPlan rp{};
+  if (plan_rc(es, M, N, es, lds_budget(), &rp)) {  // row / column decomposition
+    double c = t_rc_row(es) + t_rc_col(es) * (rp.c > 1 ? 2.0 : 1.0);
+    if (c < best.cost) best = {c, M, N, 5};
+  }
   std::vector<uint64_t> dm = divisors(M), dn = divisors(N);
   for (uint64_t a : dm) {
     uint64_t A = M / a;
-    // route 2: 010(A, a, N) + 100(A, N, a)
-    if (a > 1) {
-      double c = cost_010(es, A, a, N) + cost_pt(a * es);
+    if (a > 1 && a < M) {  // route 2: 010(A, a, N) + 100(A, N, a)
+      double c = t_010(es, A, a, N) + t_pt(a * es);
       if (c < best.cost) best = {c, a, N, 2};
     }
     for (uint64_t b : dn) {
       uint64_t B = N / b;
-      if (a == 1 && b > 1) {
-        // route 22 with a = M: 100(M, B, b) + 010(B, M, b)
-        double c = cost_pt(b * es) + cost_010(es, B, M, b);
+      if (a == 1 && b > 1 && b < N) {  // route 22 with a = M: 100(M, B, b) + 010(B, M, b)
+        double c = t_pt(b * es) + t_010(es, B, M, b);
         if (c < best.cost) best = {c, M, b, 22};
       }
       if (a > 1 && b > 1 && a < M && b < N) {
-        double c3 = cost_pt(b * es) + cost_010(es, B * A, a, b) + cost_pt(a * es);
+        double c3 = t_pt(b * es) + t_010(es, B * A, a, b) + t_pt(a * es);
         if (c3 < best.cost) best = {c3, a, b, 3};
+        double c32 = t_pt(b * es) + t_010(es, A * B, a, b) + t_pt(a * es);
+        if (c32 < best.cost) best = {c32, a, b, 32};
       }
     }
   }
   return best;
+}
+
+// Stage list of an M x N transpose as chosen by the planner (stages = 0).
+bool plan_transpose_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es,
+                           uint64_t M, uint64_t N, std::string &route) {
+  Choice c = plan_full(es, M, N);
+  if (c.stages == 2) return route_stages(out, buf, es, M / c.a, c.a, 1, N, 1, 2, route);
+  if (c.stages == 22) return route_stages(out, buf, es, 1, M, N / c.b, c.b, 1, 22, route);
+  if (c.stages == 3 || c.stages == 32)
+    return route_stages(out, buf, es, M / c.a, c.a, N / c.b, c.b, 1, c.stages, route);
+  if (c.stages == 5) return route_stages(out, buf, es, 1, M, 1, N, 1, 5, route);
+  return route_stages(out, buf, es, 1, M, 1, N, 1, -1, route);
 }
 
 }  // namespace lm
@@ -448,9 +573,19 @@ const char *hip_marshal_version(void) { return "libmarshal-amd 0.1 (gfx950)"; }
 
 void hip_marshal_set_lds_budget(size_t bytes) { g_lds_budget.store(bytes); }
 
+bool hip_marshal_plan_transpose(int64_t M, int64_t N, size_t es, int64_t *a, int64_t *b,
+                                int *stages) {
+  if (M < 1 || N < 1 || !valid_es(es)) return true;
+  Choice c = plan_full(es, uint64_t(M), uint64_t(N));
+  if (a) *a = int64_t(c.a);
+  if (b) *b = int64_t(c.b);
+  if (stages) *stages = c.stages;
+  return false;
+}
+
 bool hip_marshal_set_tuning(const char *key, int64_t value) {
   Knob *k = find_knob(key);
-  if (!k || value < 1 || value > 1024) {
+  if (!k || value < 1 || value > (int64_t(1) << 30)) {
     fprintf(stderr, "[libmarshal] hip_marshal_set_tuning: bad key or value\n");
     return true;
   }
@@ -507,7 +642,7 @@ bool hip_transpose_010_bs_es(hipStream_t st, void *src, size_t es, int64_t A, in
     return true;
   Timer tm(st, "AOS-ASTA BS", 2.0 * n * es, ns);
   return tm.finish(stage_010(static_cast<unsigned char *>(src), es, A, a, B, M010::BS, 1, 0,
-                             st, nullptr));
+                             st));
 }
 
 bool hip_transpose_010_pttwac_es(hipStream_t st, void *src, size_t es, int64_t A, int64_t a,
@@ -522,7 +657,7 @@ bool hip_transpose_010_pttwac_es(hipStream_t st, void *src, size_t es, int64_t A
   }
   Timer tm(st, "AOS-ASTA PTTWAC", 2.0 * n * es, ns);
   return tm.finish(stage_010(static_cast<unsigned char *>(src), es, A, a, B, M010::PTTWAC, R,
-                             P, st, nullptr));
+                             P, st));
 }
 
 bool hip_transpose_0100_es(hipStream_t st, void *src, size_t es, int64_t A, int64_t a,
@@ -550,24 +685,16 @@ bool hip_transpose_es(hipStream_t st, void *src, size_t es, int64_t A, int64_t a
     fprintf(stderr, "[libmarshal] hip_transpose: need 1 <= R <= 32\n");
     return true;
   }
-  uint64_t A_ = A, a_ = a, B_ = B, b_ = b;
-  int stg = stages;
-  if (stg == 0) {
-    Choice c = plan_full(es, A_ * a_, B_ * b_);
-    if (c.stages == -1) {
-      stg = -1;
-    } else {
-      stg = c.stages;
-      if (stg == 2) { A_ = A_ * a_ / c.a; a_ = c.a; b_ = B_ * b_; B_ = 1; }
-      else if (stg == 22) { b_ = c.b; B_ = uint64_t(B) * b / c.b; A_ = 1; a_ = uint64_t(A) * a; }
-      else { uint64_t M = A_ * a_, N = B_ * b_; a_ = c.a; A_ = M / c.a; b_ = c.b; B_ = N / c.b; }
-    }
-  }
-  Timer tm(st, "Transpose", 2.0 * n * es, ns);
+  unsigned char *buf = static_cast<unsigned char *>(src);
+  std::vector<StageRec> list;
   std::string route;
-  bool err = compose(static_cast<unsigned char *>(src), es, A_, a_, B_, b_, R, stg, st, route);
+  bool ok = stages == 0
+                ? plan_transpose_stages(list, buf, es, uint64_t(A) * a, uint64_t(B) * b, route)
+                : route_stages(list, buf, es, A, a, B, b, R, stages, route);
   t_route = route;
-  return tm.finish(err);
+  if (!ok) return true;
+  Timer tm(st, "Transpose", 2.0 * n * es, ns);
+  return tm.finish(run_stages(list, st));
 }
 
 bool hip_transpose_full_es(hipStream_t st, void *src, size_t es, int64_t M, int64_t N,
@@ -591,12 +718,12 @@ static bool aos_common(const char *who, hipStream_t st, void *src, size_t es, in
   unsigned char *buf = static_cast<unsigned char *>(src);
   bool err;
   if (method == 1) {
-    err = stage_010(buf, es, A, a, B, M010::BS, 1, 1, st, nullptr);
+    err = stage_010(buf, es, A, a, B, M010::BS, 1, 1, st);
   } else if (method == 2) {
-    err = stage_010(buf, es, A, a, B, M010::PTTWAC, 1, 1, st, nullptr);
+    err = stage_010(buf, es, A, a, B, M010::PTTWAC, 1, 1, st);
   } else {
     // LDS-staged first, cycle following when it cannot run (src/cl_marshal.cc:814-818).
-    err = stage_010(buf, es, A, a, B, M010::Auto, 1, 1, st, nullptr);
+    err = stage_010(buf, es, A, a, B, M010::Auto, 1, 1, st);
   }
   return tm.finish(err);
 }
@@ -722,7 +849,7 @@ static bool gpu_010(void *src, size_t es, int h, int w, int t, M010 m) {
   }
   if (!aligned_ok("gpu_aos_asta", src, es)) return true;
   return stage_010(static_cast<unsigned char *>(src), es, uint64_t(h / t), uint64_t(t),
-                   uint64_t(w), m, 1, 0, nullptr, nullptr);
+                   uint64_t(w), m, 1, 0, nullptr);
 }
 
 bool gpu_aos_asta_bs_float(float *src, int h, int w, int t, clock_t *) {

@@ -1,0 +1,659 @@
+// batch.hip - many layout conversions of uneven shapes and element sizes in one graph launch.
+//
+// A batch is planned once: every matrix's conversion becomes its route (a list of stage
+// records, the same ones a plain call runs). Stage k of every matrix forms level k. Within a
+// level, stages that use the same kernel form (and template instantiation) form a group,
+// executed by ONE persistent kernel over a work list: each workgroup (or wave) claims work
+// units (chunks, tiles, instances, walker sets) of any matrix in the group, looks up its
+// matrix by binary search over a prefix sum of units, and runs the same device body as the
+// single-matrix kernel. Levels are ordered; groups inside a level are independent. The whole
+// sequence - clearing every matrix's visited bits, then the group kernels level by level - is
+// captured once into a hipGraph, so a run is one graph launch.
+//
+// No counterpart in the reference, which launches one kernel per stage per matrix and waits
+// after each (src/cl_marshal.cc:561-776, include/cl_profile.h:23-24).
+#include "kernels_dev.h"
+#include "../../include/hip_marshal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace lm {
+
+__device__ __forceinline__ uint32_t find_item(const uint64_t *prefix, uint32_t n, uint64_t q) {
+  uint32_t lo = 0, hi = n;  // prefix[lo] <= q < prefix[hi]
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---- group kernels -------------------------------------------------------------------------
+
+template <int ES, int G, int UPT>
+__global__ __launch_bounds__(256) void kg_010_bs(const Bs010Params *items,
+                                                 const uint64_t *prefix, uint32_t n,
+                                                 uint64_t total) {
+  using E = typename Elem<ES>::T;
+  using U = typename UnitOf<ES, G>::T;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  E *lds = reinterpret_cast<E *>(lds_raw);
+  uint64_t q = blockIdx.x;
+  if (q >= total) return;
+  uint32_t it = find_item(prefix, n, q);
+  Bs010Params p = items[it];
+  uint64_t c = q - prefix[it];
+  U v[UPT];
+  bs_load_round<ES, G, 256, UPT>(p, c, 0, v);
+  while (true) {
+    bs_stage_in<ES, G, 256, UPT>(p, c, v, lds);
+    __syncthreads();
+    // Prefetch the next claimed chunk (a disjoint region, possibly of another matrix).
+    const uint64_t qn = q + gridDim.x;
+    Bs010Params pn = p;
+    uint64_t cn = 0;
+    if (qn < total) {
+      uint32_t in = find_item(prefix, n, qn);
+      pn = items[in];
+      cn = qn - prefix[in];
+      bs_load_round<ES, G, 256, UPT>(pn, cn, 0, v);
+    }
+    bs_gather_out<ES, G, 256>(p, c, lds);
+    __syncthreads();
+    if (qn >= total) break;
+    q = qn;
+    p = pn;
+    c = cn;
+  }
+}
+
+template <int ES>
+__global__ __launch_bounds__(256) void kg_010_pttwac(const Pt010Params *items,
+                                                     const uint64_t *prefix, uint32_t n,
+                                                     uint64_t total) {
+  extern __shared__ __align__(16) uint32_t lflags[];
+  for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
+    uint32_t it = find_item(prefix, n, q);
+    const Pt010Params p = items[it];
+    pt010_tile<ES, 256>(p, q - prefix[it], lflags);
+  }
+}
+
+template <int UB, int UPL, bool I64>
+__global__ __launch_bounds__(256) void kg_100(const Pt100Params *items, const uint64_t *prefix,
+                                              uint32_t n, uint64_t tasks) {
+  const uint64_t wave = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const uint64_t nwaves = uint64_t(gridDim.x) * 4;
+  for (uint64_t task = wave; task < tasks; task += nwaves) {
+    uint32_t it = find_item(prefix, n, task);
+    const Pt100Params p = items[it];
+    const uint64_t gpw = 64u >> p.vw_log2;
+    pt100_walkers<UB, UPL, I64>(p, (task - prefix[it]) * gpw);
+  }
+}
+
+template <int UB, int UPL>
+__global__ __launch_bounds__(256) void kg_0100_lds(const Pt100Params *items,
+                                                   const uint64_t *prefix, uint32_t n,
+                                                   uint64_t total) {
+  extern __shared__ __align__(16) uint32_t lflags[];
+  for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
+    uint32_t it = find_item(prefix, n, q);
+    const Pt100Params p = items[it];
+    pt0100_instance<UB, UPL, 256>(p, q - prefix[it], lflags);
+  }
+}
+
+template <int UB>
+__global__ __launch_bounds__(1024) void kg_rc_row(const RcParams *items, const uint64_t *prefix,
+                                                 uint32_t n, uint64_t total) {
+  using V = typename Elem<UB>::T;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  V *lds = reinterpret_cast<V *>(lds_raw);
+  for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
+    uint32_t it = find_item(prefix, n, q);
+    const RcParams p = items[it];
+    rc_row_block<UB, 1024>(p, q - prefix[it], lds);
+  }
+}
+
+// Column blocks: every item's unit count is padded to a multiple of 8 and the grid is a
+// multiple of 8, so (unit mod 8) is this workgroup's XCD and each XCD sweeps a contiguous
+// range of an item's column blocks.
+template <int UB, bool SHUFFLE>
+__global__ __launch_bounds__(1024) void kg_rc_col(const RcParams *items, const uint64_t *prefix,
+                                                 uint32_t n, uint64_t total) {
+  using V = typename Elem<UB>::T;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  V *lds = reinterpret_cast<V *>(lds_raw);
+  for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
+    uint32_t it = find_item(prefix, n, q);
+    const RcParams p = items[it];
+    const uint32_t local = uint32_t(q - prefix[it]);
+    const uint32_t per = (p.ncolblk + 7) / 8;
+    const uint32_t cb = (local & 7) * per + (local >> 3);
+    if (cb < p.ncolblk) rc_col_block<UB, 1024, SHUFFLE>(p, cb, lds);
+  }
+}
+
+// ---- host side -----------------------------------------------------------------------------
+
+namespace {
+
+// Kernel form + template instantiation of a group.
+struct GroupKey {
+  int kind, a, b, c;
+  bool operator<(const GroupKey &o) const {
+    return std::tie(kind, a, b, c) < std::tie(o.kind, o.a, o.b, o.c);
+  }
+};
+
+struct Group {
+  GroupKey key;
+  std::vector<unsigned char> items;  // packed parameter blocks
+  size_t item_size = 0;
+  std::vector<uint64_t> prefix{0};
+  size_t lds = 0;
+  // device copies (offsets into the batch's parameter buffer)
+  size_t items_off = 0, prefix_off = 0;
+  const void *func = nullptr;
+  uint32_t grid = 0;
+  uint32_t block = 256;
+};
+
+template <typename P>
+void add_item(Group &g, const P &p, uint64_t units) {
+  if (units == 0) return;
+  g.item_size = sizeof(P);
+  size_t off = g.items.size();
+  g.items.resize(off + sizeof(P));
+  std::memcpy(g.items.data() + off, &p, sizeof(P));
+  g.prefix.push_back(g.prefix.back() + units);
+}
+
+Bs010Params make_bs(const StageRec &r) {
+  constexpr int NT = 256;
+  Bs010Params p{};
+  p.base = r.buf;
+  p.A = r.d0;
+  p.a = uint32_t(r.d1);
+  p.B = uint32_t(r.d2);
+  p.Bp = r.bs.Bp;
+  p.n = uint32_t(r.d1 * r.d2);
+  p.g = uint32_t(r.bs.g);
+  p.nchunks = r.bs.nchunks;
+  p.div_n = FastDiv32(p.n);
+  p.div_B = FastDiv32(p.B);
+  p.div_a = FastDiv32(p.a);
+  uint32_t D = uint32_t(NT * r.bs.G - (r.bs.G - 1));
+  p.din_t = D / p.n;
+  p.din_i = (D % p.n) / p.B;
+  p.din_j = (D % p.n) % p.B;
+  p.dout_t = D / p.n;
+  p.dout_j = (D % p.n) / p.a;
+  p.dout_i = (D % p.n) % p.a;
+  return p;
+}
+
+Pt010Params make_pt010(const StageRec &r) {
+  Pt010Params p{};
+  p.base = r.buf;
+  p.A = r.d0;
+  p.a = uint32_t(r.d1);
+  p.B = uint32_t(r.d2);
+  p.n = uint32_t(r.d1 * r.d2);
+  p.sh = uint32_t(pt010_flag_words(r.d1, r.d2, r.R, r.P));
+  p.div_B = FastDiv32(p.B);
+  p.div_sh = FastDiv32(p.sh);
+  return p;
+}
+
+Pt100Params make_pt100(const StageRec &r, uint32_t *flags) {
+  Pt100Params p{};
+  p.base = r.buf;
+  p.flags = flags;
+  p.R = r.d1;
+  p.C = r.d2;
+  p.RC = r.d1 * r.d2;
+  p.S = r.d3 * r.es;
+  p.nsub = r.pt.nsub;
+  p.sub_bytes = r.pt.sub_bytes;
+  p.walkers = r.d0 * p.RC * r.pt.nsub;
+  p.vw_log2 = r.pt.vw_log2;
+  p.div_C64 = FastDiv64(p.C);
+  p.div_RC = FastDiv64(p.RC);
+  p.div_nsub = FastDiv64(p.nsub);
+  p.div_C32 = FastDiv32(uint32_t(std::min<uint64_t>(p.C, 0xffffffffu)));
+  return p;
+}
+
+template <int ES, int G>
+const void *bs_func(int upt) {
+  switch (upt) {
+    case 4: return reinterpret_cast<const void *>(kg_010_bs<ES, G, 4>);
+    case 8: return reinterpret_cast<const void *>(kg_010_bs<ES, G, 8>);
+    default: return reinterpret_cast<const void *>(kg_010_bs<ES, G, 16>);
+  }
+}
+
+const void *bs_func_es(int es, int G, int upt) {
+  switch (es) {
+    case 1: return G == 4 ? bs_func<1, 4>(upt) : bs_func<1, 1>(upt);
+    case 2: return G == 2 ? bs_func<2, 2>(upt) : bs_func<2, 1>(upt);
+    case 4: return bs_func<4, 1>(upt);
+    case 8: return bs_func<8, 1>(upt);
+    default: return bs_func<16, 1>(upt);
+  }
+}
+
+const void *pt010_func(int es) {
+  switch (es) {
+    case 1: return reinterpret_cast<const void *>(kg_010_pttwac<1>);
+    case 2: return reinterpret_cast<const void *>(kg_010_pttwac<2>);
+    case 4: return reinterpret_cast<const void *>(kg_010_pttwac<4>);
+    case 8: return reinterpret_cast<const void *>(kg_010_pttwac<8>);
+    default: return reinterpret_cast<const void *>(kg_010_pttwac<16>);
+  }
+}
+
+template <int UB>
+const void *pt100_func_ub(int upl, bool i64) {
+  switch (upl) {
+    case 1: return i64 ? reinterpret_cast<const void *>(kg_100<UB, 1, true>)
+                       : reinterpret_cast<const void *>(kg_100<UB, 1, false>);
+    case 2: return i64 ? reinterpret_cast<const void *>(kg_100<UB, 2, true>)
+                       : reinterpret_cast<const void *>(kg_100<UB, 2, false>);
+    case 4: return i64 ? reinterpret_cast<const void *>(kg_100<UB, 4, true>)
+                       : reinterpret_cast<const void *>(kg_100<UB, 4, false>);
+    default: return i64 ? reinterpret_cast<const void *>(kg_100<UB, 8, true>)
+                        : reinterpret_cast<const void *>(kg_100<UB, 8, false>);
+  }
+}
+
+const void *pt100_func(int ub, int upl, bool i64) {
+  switch (ub) {
+    case 1: return pt100_func_ub<1>(upl, i64);
+    case 2: return pt100_func_ub<2>(upl, i64);
+    case 4: return pt100_func_ub<4>(upl, i64);
+    case 8: return pt100_func_ub<8>(upl, i64);
+    default: return pt100_func_ub<16>(upl, i64);
+  }
+}
+
+template <int UB>
+const void *lds_func_ub(int upl) {
+  switch (upl) {
+    case 1: return reinterpret_cast<const void *>(kg_0100_lds<UB, 1>);
+    case 2: return reinterpret_cast<const void *>(kg_0100_lds<UB, 2>);
+    case 4: return reinterpret_cast<const void *>(kg_0100_lds<UB, 4>);
+    default: return reinterpret_cast<const void *>(kg_0100_lds<UB, 8>);
+  }
+}
+
+template <int UB>
+const void *rc_func_ub(int pass) {
+  switch (pass) {
+    case 0: return reinterpret_cast<const void *>(kg_rc_row<UB>);
+    case 1: return reinterpret_cast<const void *>(kg_rc_col<UB, false>);
+    default: return reinterpret_cast<const void *>(kg_rc_col<UB, true>);
+  }
+}
+
+const void *rc_func(int ub, int pass) {
+  switch (ub) {
+    case 1: return rc_func_ub<1>(pass);
+    case 2: return rc_func_ub<2>(pass);
+    case 4: return rc_func_ub<4>(pass);
+    case 8: return rc_func_ub<8>(pass);
+    default: return rc_func_ub<16>(pass);
+  }
+}
+
+const void *lds_func(int ub, int upl) {
+  switch (ub) {
+    case 1: return lds_func_ub<1>(upl);
+    case 2: return lds_func_ub<2>(upl);
+    case 4: return lds_func_ub<4>(upl);
+    case 8: return lds_func_ub<8>(upl);
+    default: return lds_func_ub<16>(upl);
+  }
+}
+
+}  // namespace
+
+struct Batch {
+  std::vector<std::vector<Group>> levels;
+  void *params = nullptr;    // device copy of all item arrays and prefix sums
+  uint32_t *flags = nullptr; // visited bits of every global-bit stage
+  uint64_t flag_words = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = true;
+  int64_t stages = 0;
+  double bytes = 0;  // algorithmic bytes of one run (2 x every matrix)
+  std::vector<std::string> routes;
+};
+
+namespace {
+
+bool enqueue(Batch &b, hipStream_t st) {
+  if (b.flag_words) LM_CHECK_HIP(launch_fill_u32(b.flags, b.flag_words, 0, st));
+  unsigned char *base = static_cast<unsigned char *>(b.params);
+  for (auto &level : b.levels) {
+    for (Group &g : level) {
+      const void *items = base + g.items_off;
+      const uint64_t *prefix = reinterpret_cast<const uint64_t *>(base + g.prefix_off);
+      uint32_t n = uint32_t(g.prefix.size() - 1);
+      uint64_t total = g.prefix.back();
+      void *args[] = {&items, &prefix, &n, &total};
+      LM_CHECK_HIP(hipLaunchKernel(g.func, dim3(g.grid), dim3(g.block), args, g.lds, st));
+    }
+  }
+  return false;
+}
+
+}  // namespace
+
+}  // namespace lm
+
+using namespace lm;
+
+struct hip_marshal_batch {
+  Batch b;
+};
+
+extern "C" {
+
+hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64_t count,
+                                            int use_graph) {
+  if (!descs || count < 1) {
+    fprintf(stderr, "[libmarshal] hip_marshal_batch_create: empty batch\n");
+    return nullptr;
+  }
+  auto *hb = new hip_marshal_batch();
+  Batch &b = hb->b;
+  b.use_graph = use_graph != 0;
+  // 1. routes
+  std::vector<std::vector<StageRec>> per(count);
+  b.routes.resize(count);
+  for (int64_t i = 0; i < count; i++) {
+    const hip_marshal_desc &d = descs[i];
+    size_t es = size_t(d.elem_bytes);
+    unsigned char *buf = static_cast<unsigned char *>(d.ptr);
+    bool ok = (es == 1 || es == 2 || es == 4 || es == 8 || es == 16) && d.ptr &&
+              reinterpret_cast<uintptr_t>(d.ptr) % std::min<size_t>(es, 16) == 0 &&
+              d.rows >= 1 && d.cols >= 1;
+    if (ok && d.op != HIP_MARSHAL_OP_TRANSPOSE) ok = d.tile >= 1 && d.rows % d.tile == 0;
+    if (!ok) {
+      fprintf(stderr, "[libmarshal] hip_marshal_batch_create: invalid descriptor %lld\n",
+              (long long)i);
+      hip_marshal_batch_destroy(hb);
+      return nullptr;
+    }
+    uint64_t h = uint64_t(d.rows), w = uint64_t(d.cols), t = uint64_t(d.tile);
+    std::string route;
+    switch (d.op) {
+      case HIP_MARSHAL_OP_TRANSPOSE:
+        ok = plan_transpose_stages(per[i], buf, es, h, w, route);
+        break;
+      case HIP_MARSHAL_OP_AOS_ASTA:  // 010(h/t, t, w)
+        ok = push_010(per[i], buf, es, h / t, t, w, M010::Auto, 1, 1, nullptr);
+        route = "aos_asta";
+        break;
+      case HIP_MARSHAL_OP_ASTA_AOS:  // 010(h/t, w, t)
+        ok = push_010(per[i], buf, es, h / t, w, t, M010::Auto, 1, 1, nullptr);
+        route = "asta_aos";
+        break;
+      case HIP_MARSHAL_OP_SOA_ASTA:  // 100(w, h/t, t)
+        push_0100(per[i], buf, es, 1, w, h / t, t);
+        route = "soa_asta";
+        break;
+      case HIP_MARSHAL_OP_ASTA_SOA:  // 100(h/t, w, t)
+        push_0100(per[i], buf, es, 1, h / t, w, t);
+        route = "asta_soa";
+        break;
+      default:
+        ok = false;
+    }
+    if (!ok) {
+      fprintf(stderr, "[libmarshal] hip_marshal_batch_create: cannot plan descriptor %lld\n",
+              (long long)i);
+      hip_marshal_batch_destroy(hb);
+      return nullptr;
+    }
+    b.routes[i] = route;
+    b.stages += int64_t(per[i].size());
+    b.bytes += 2.0 * double(h) * double(w) * double(es);
+  }
+  // 2. flag space for every global-bit stage
+  size_t maxlev = 0;
+  for (auto &v : per) maxlev = std::max(maxlev, v.size());
+  std::vector<std::vector<uint64_t>> flag_off(count);
+  for (int64_t i = 0; i < count; i++) {
+    for (const StageRec &r : per[i]) {
+      flag_off[i].push_back(b.flag_words);
+      if (r.kind == StageKind::PT100G) b.flag_words += (r.pt.flag_words + 63) / 64 * 64;
+    }
+  }
+  if (b.flag_words && hipMalloc(&b.flags, b.flag_words * 4) != hipSuccess) {
+    fprintf(stderr, "[libmarshal] hip_marshal_batch_create: flag allocation failed\n");
+    hip_marshal_batch_destroy(hb);
+    return nullptr;
+  }
+  // 3. groups per level
+  const DeviceInfo &di = device_info();
+  for (size_t lv = 0; lv < maxlev; lv++) {
+    std::map<GroupKey, Group> groups;
+    for (int64_t i = 0; i < count; i++) {
+      if (lv >= per[i].size()) continue;
+      const StageRec &r = per[i][lv];
+      switch (r.kind) {
+        case StageKind::BS010: {
+          GroupKey k{0, int(r.es), r.bs.G, r.bs.upt};
+          Group &g = groups[k];
+          g.key = k;
+          g.lds = std::max(g.lds, r.bs.lds_bytes);
+          add_item(g, make_bs(r), r.bs.nchunks);
+          break;
+        }
+        case StageKind::PT010: {
+          GroupKey k{1, int(r.es), 0, 0};
+          Group &g = groups[k];
+          g.key = k;
+          g.lds = std::max<size_t>(g.lds, pt010_flag_words(r.d1, r.d2, r.R, r.P) * 4);
+          add_item(g, make_pt010(r), r.d0);
+          break;
+        }
+        case StageKind::PT100G: {
+          bool i64 = r.d1 * r.d2 >= (1ull << 32);
+          GroupKey k{2, r.pt.ub, r.pt.upl, i64 ? 1 : 0};
+          Group &g = groups[k];
+          g.key = k;
+          Pt100Params p = make_pt100(r, b.flags + flag_off[i][lv]);
+          uint64_t gpw = 64u >> p.vw_log2;
+          add_item(g, p, (p.walkers + gpw - 1) / gpw);
+          break;
+        }
+        case StageKind::RC_ROW:
+        case StageKind::RC_COL1:
+        case StageKind::RC_COL2: {
+          int pass = r.kind == StageKind::RC_ROW ? 0 : r.kind == StageKind::RC_COL1 ? 1 : 2;
+          GroupKey k{4 + pass, r.rc.ub, 0, 0};
+          Group &g = groups[k];
+          g.key = k;
+          RcParams p = make_rc(r.buf, r.d1, r.d2, r.rc);
+          if (pass == 0) {
+            g.lds = std::max(g.lds, r.rc.row_lds);
+            add_item(g, p, (r.d1 + p.rows - 1) / p.rows);
+          } else {
+            g.lds = std::max(g.lds, r.rc.col_lds);
+            add_item(g, p, uint64_t((p.ncolblk + 7) / 8) * 8);
+          }
+          break;
+        }
+        case StageKind::PT0100L: {
+          GroupKey k{3, r.pt.ub, r.pt.upl, 0};
+          Group &g = groups[k];
+          g.key = k;
+          g.lds = std::max(g.lds, lds0100_bytes(r.d1, r.d2, r.pt));
+          add_item(g, make_pt100(r, nullptr), r.d0);
+          break;
+        }
+      }
+    }
+    std::vector<Group> lvl;
+    for (auto &kv : groups) {
+      Group g = kv.second;
+      if (g.prefix.size() < 2) continue;
+      uint64_t total = g.prefix.back();
+      switch (g.key.kind) {
+        case 0:
+          g.func = bs_func_es(g.key.a, g.key.b, g.key.c);
+          g.grid = uint32_t(std::min<uint64_t>(total, uint64_t(di.cus) * tuning("bs_blocks_per_cu")));
+          break;
+        case 1:
+          g.func = pt010_func(g.key.a);
+          g.grid = uint32_t(std::min<uint64_t>(total, uint64_t(di.cus) * tuning("pt010_blocks_per_cu")));
+          break;
+        case 2:
+          g.func = pt100_func(g.key.a, g.key.b, g.key.c != 0);
+          g.grid = uint32_t(std::min<uint64_t>((total + 3) / 4,
+                                               uint64_t(di.cus) * tuning("pt100_blocks_per_cu")));
+          break;
+        case 3:
+          g.func = lds_func(g.key.a, g.key.b);
+          g.grid = uint32_t(std::min<uint64_t>(total, uint64_t(di.cus) * 8));
+          break;
+        case 4:
+          g.block = 1024;
+          g.func = rc_func(g.key.a, 0);
+          g.grid = uint32_t(std::min<uint64_t>(total, uint64_t(di.cus) * tuning("rc_row_blocks_per_cu")));
+          break;
+        default:
+          g.block = 1024;
+          g.func = rc_func(g.key.a, g.key.kind - 4);
+          g.grid = uint32_t(std::min<uint64_t>(total, uint64_t(di.cus) * tuning("rc_col_blocks_per_cu")));
+          g.grid = (g.grid + 7) / 8 * 8;
+          break;
+      }
+      if (g.lds > 65536 &&
+          hipFuncSetAttribute(g.func, hipFuncAttributeMaxDynamicSharedMemorySize, int(g.lds)) !=
+              hipSuccess) {
+        fprintf(stderr, "[libmarshal] hip_marshal_batch_create: LDS attribute failed\n");
+        hip_marshal_batch_destroy(hb);
+        return nullptr;
+      }
+      lvl.push_back(std::move(g));
+    }
+    b.levels.push_back(std::move(lvl));
+  }
+  // 4. one device buffer for every work list
+  size_t total_bytes = 0;
+  for (auto &lvl : b.levels)
+    for (Group &g : lvl) {
+      g.items_off = total_bytes;
+      total_bytes += (g.items.size() + 255) / 256 * 256;
+      g.prefix_off = total_bytes;
+      total_bytes += (g.prefix.size() * 8 + 255) / 256 * 256;
+    }
+  if (total_bytes) {
+    if (hipMalloc(&b.params, total_bytes) != hipSuccess) {
+      fprintf(stderr, "[libmarshal] hip_marshal_batch_create: parameter allocation failed\n");
+      hip_marshal_batch_destroy(hb);
+      return nullptr;
+    }
+    std::vector<unsigned char> host(total_bytes, 0);
+    for (auto &lvl : b.levels)
+      for (Group &g : lvl) {
+        std::memcpy(host.data() + g.items_off, g.items.data(), g.items.size());
+        std::memcpy(host.data() + g.prefix_off, g.prefix.data(), g.prefix.size() * 8);
+      }
+    if (hipMemcpy(b.params, host.data(), total_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      fprintf(stderr, "[libmarshal] hip_marshal_batch_create: parameter upload failed\n");
+      hip_marshal_batch_destroy(hb);
+      return nullptr;
+    }
+  }
+  return hb;
+}
+
+bool hip_marshal_batch_run(hip_marshal_batch *hb, hipStream_t st, uint64_t *elapsed_ns) {
+  if (!hb) return true;
+  Batch &b = hb->b;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (elapsed_ns) {
+    LM_CHECK_HIP(hipEventCreate(&e0));
+    LM_CHECK_HIP(hipEventCreate(&e1));
+    LM_CHECK_HIP(hipEventRecord(e0, st));
+  }
+  if (b.use_graph) {
+    if (!b.exec) {
+      // Captured once, on a private stream (the caller's may be the null stream).
+      hipStream_t cs = nullptr;
+      LM_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+      bool err = e != hipSuccess || enqueue(b, cs);
+      hipGraph_t g = nullptr;
+      if (e == hipSuccess) e = hipStreamEndCapture(cs, &g);
+      hipStreamDestroy(cs);
+      if (err || e != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        fprintf(stderr, "[libmarshal] hip_marshal_batch_run: capture failed\n");
+        return true;
+      }
+      b.graph = g;
+      LM_CHECK_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+    }
+    LM_CHECK_HIP(hipGraphLaunch(b.exec, st));
+  } else if (enqueue(b, st)) {
+    return true;
+  }
+  if (elapsed_ns) {
+    float ms = 0;
+    LM_CHECK_HIP(hipEventRecord(e1, st));
+    LM_CHECK_HIP(hipEventSynchronize(e1));
+    LM_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *elapsed_ns += uint64_t(double(ms) * 1e6 + 0.5);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
+  return false;
+}
+
+int64_t hip_marshal_batch_info(const hip_marshal_batch *hb, const char *key) {
+  if (!hb || !key) return -1;
+  const Batch &b = hb->b;
+  if (!strcmp(key, "levels")) return int64_t(b.levels.size());
+  if (!strcmp(key, "stages")) return b.stages;
+  if (!strcmp(key, "flag_bytes")) return int64_t(b.flag_words * 4);
+  if (!strcmp(key, "bytes")) return int64_t(b.bytes);
+  if (!strcmp(key, "kernels")) {
+    int64_t k = b.flag_words ? 1 : 0;
+    for (auto &l : b.levels) k += int64_t(l.size());
+    return k;
+  }
+  return -1;
+}
+
+const char *hip_marshal_batch_route(const hip_marshal_batch *hb, int64_t i) {
+  if (!hb || i < 0 || i >= int64_t(hb->b.routes.size())) return "";
+  return hb->b.routes[size_t(i)].c_str();
+}
+
+void hip_marshal_batch_destroy(hip_marshal_batch *hb) {
+  if (!hb) return;
+  Batch &b = hb->b;
+  if (b.exec) hipGraphExecDestroy(b.exec);
+  if (b.graph) hipGraphDestroy(b.graph);
+  if (b.params) hipFree(b.params);
+  if (b.flags) hipFree(b.flags);
+  delete hb;
+}
+
+}  // extern "C"

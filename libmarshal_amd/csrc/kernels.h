@@ -3,6 +3,9 @@
 
 #include "common.h"
 
+#include <string>
+#include <vector>
+
 namespace lm {
 
 // ---- 010, LDS-staged ------------------------------------------------------------------
@@ -69,6 +72,35 @@ hipError_t launch_100_pttwac(unsigned char *base, uint32_t *flags, uint64_t inst
                              uint64_t C, uint64_t S, const Pt100Plan &pl, uint32_t max_blocks,
                              hipStream_t st);
 
+// 0100 with the visited bits of one instance in LDS (one workgroup per instance).
+size_t lds0100_bytes(uint64_t R, uint64_t C, const Pt100Plan &pl);
+hipError_t launch_0100_lds(unsigned char *base, uint64_t inst, uint64_t R, uint64_t C,
+                           uint64_t S, const Pt100Plan &pl, uint32_t max_blocks,
+                           hipStream_t st);
+
+// ---- full transpose by row / column shuffles (LDS-staged passes) -------------------------
+struct RcPlan {
+  int ub;             // unit bytes
+  uint32_t m;         // units per element
+  uint32_t rows;      // rows per workgroup iteration of the row pass
+  uint32_t w;         // columns per block of a column pass
+  uint32_t c;         // gcd(M, N)
+  size_t row_lds, col_lds;
+};
+struct RcParams {
+  unsigned char *base;
+  uint32_t M, N, c, a, b, ainv, m, rows, w, ncolblk;
+  FastDiv32 div_c, div_b, div_m, div_upr, div_pitch, div_seg_last;
+  FastDiv64 div_b64, div_M64;
+};
+// False when M or N is < 2 or >= 2^31, or a row or a one-element-wide column of S-byte
+// elements does not fit `lds_budget`.
+bool plan_rc(size_t es, uint64_t M, uint64_t N, uint64_t S, size_t lds_budget, RcPlan *out);
+RcParams make_rc(unsigned char *base, uint64_t M, uint64_t N, const RcPlan &pl);
+// pass: 0 = row shuffle, 1 = column rotation, 2 = column shuffle.
+hipError_t launch_rc(unsigned char *base, uint64_t M, uint64_t N, const RcPlan &pl, int pass,
+                     uint32_t max_blocks, hipStream_t st);
+
 // ---- utilities --------------------------------------------------------------------------
 hipError_t launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st);
 
@@ -81,6 +113,42 @@ hipError_t launch_padding(unsigned char *base, size_t es, uint64_t x, uint64_t y
 // Run-time tuning knobs (launch shapes). Keys: "bs_upt", "bs_blocks_per_cu",
 // "pt010_blocks_per_cu", "pt100_blocks_per_cu".
 int64_t tuning(const char *key);
+
+// ---- stage records (a route as a list of elementary stages) -----------------------------
+enum class M010 { Auto, BS, PTTWAC, Global };
+enum class StageKind : int { BS010, PT010, PT100G, PT0100L, RC_ROW, RC_COL1, RC_COL2 };
+struct StageRec {
+  StageKind kind;
+  size_t es;
+  unsigned char *buf;
+  uint64_t d0, d1, d2, d3;  // 010: (A, a, B); 0100: (instances, R, C, b)
+  int R, P;                 // 010 cycle following: bit spreading / padding
+  Bs010Plan bs;
+  Pt100Plan pt;
+  RcPlan rc;                // RC_*: (M, N) in d1, d2 and the element bytes in d3
+};
+struct Choice {
+  double cost = 1e300;
+  uint64_t a = 1, b = 1;
+  int stages = 0;  // 2, 22, 3, 32, or -1 for the element-granular fallback
+};
+size_t current_lds_budget();
+M010 choose_010(size_t es, uint64_t A, uint64_t a, uint64_t B, int R, int P,
+                uint64_t base_addr);
+void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t inst,
+               uint64_t R, uint64_t C, uint64_t b);
+bool push_010(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t A,
+              uint64_t a, uint64_t B, M010 m, int R, int P, const char **tag);
+bool route_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t A,
+                  uint64_t a, uint64_t B, uint64_t b, int R, int stages, std::string &route);
+bool plan_transpose_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es,
+                           uint64_t M, uint64_t N, std::string &route);
+Choice plan_full(size_t es, uint64_t M, uint64_t N);
+bool run_stages(const std::vector<StageRec> &stages, hipStream_t st);
+// Appends the row / column decomposition of an M x N transpose of S-byte elements; false
+// when it does not fit the LDS budget.
+bool push_rc(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t M,
+             uint64_t N, uint64_t S);
 
 // Device properties used for launch sizing.
 struct DeviceInfo {

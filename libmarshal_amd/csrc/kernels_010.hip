@@ -8,6 +8,7 @@
 //                   bits live in LDS (the reference's REMAP layout), for tiles too large to
 //                   stage.
 #include "kernels.h"
+#include "kernels_dev.h"
 
 #include <algorithm>
 #include <type_traits>
@@ -27,137 +28,23 @@ namespace lm {
 // and advanced by constant deltas; the only divisions are one per lane per chunk.
 // ---------------------------------------------------------------------------------------
 
-template <int ES, int G> struct UnitOf { using T = typename Elem<ES>::T; };
-template <> struct UnitOf<1, 4> { using T = uint32_t; };
-template <> struct UnitOf<2, 2> { using T = uint32_t; };
-
-template <int ES, int G>
-__device__ __forceinline__ typename Elem<ES>::T unit_get(const typename UnitOf<ES, G>::T &v,
-                                                         int k) {
-  if constexpr (G == 1) {
-    return v;
-  } else {
-    return typename Elem<ES>::T(v >> (8 * ES * k));
-  }
-}
-
-template <int ES, int G>
-__device__ __forceinline__ void unit_put(typename UnitOf<ES, G>::T &v, int k,
-                                         typename Elem<ES>::T e) {
-  if constexpr (G == 1) {
-    v = e;
-  } else {
-    v |= uint32_t(e) << (8 * ES * k);
-  }
-}
-
-// Units of chunk `c` owned by this thread in round `r` are u = r*NT*UPT + tid + k*NT.
-template <int ES, int G, int NT, int UPT>
-__device__ __forceinline__ void bs_load_round(const Bs010Params &p, uint64_t c, uint32_t r,
-                                              typename UnitOf<ES, G>::T (&v)[UPT]) {
-  using U = typename UnitOf<ES, G>::T;
-  const uint64_t t0 = c * p.g;
-  const uint32_t tiles = uint32_t(std::min<uint64_t>(p.g, p.A - t0));
-  const uint32_t nU = tiles * p.n / G;
-  const U *gu = reinterpret_cast<const U *>(p.base + t0 * uint64_t(p.n) * ES);
-#pragma unroll
-  for (int k = 0; k < UPT; k++) {
-    uint32_t u = r * (NT * UPT) + threadIdx.x + k * NT;
-    if (u < nU) v[k] = __builtin_nontemporal_load(gu + u);
-  }
-}
-
 template <int ES, int G, int NT, int UPT>
 __global__ __launch_bounds__(NT) void k_010_bs(Bs010Params p) {
   using E = typename Elem<ES>::T;
   using U = typename UnitOf<ES, G>::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   E *lds = reinterpret_cast<E *>(lds_raw);
-  const uint32_t tid = threadIdx.x;
-  const uint32_t a = p.a, B = p.B, Bp = p.Bp, n = p.n;
-  constexpr uint32_t RU = NT * UPT;  // units per round
-
   uint64_t c = blockIdx.x;
   if (c >= p.nchunks) return;
   U v[UPT];
   bs_load_round<ES, G, NT, UPT>(p, c, 0, v);
-
   while (true) {
-    const uint64_t t0 = c * p.g;
-    const uint32_t tiles = uint32_t(std::min<uint64_t>(p.g, p.A - t0));
-    const uint32_t C = tiles * n;
-    const uint32_t nU = C / G;
-    unsigned char *gbase = p.base + t0 * uint64_t(n) * ES;
-    U *gu = reinterpret_cast<U *>(gbase);
-    const uint32_t rounds = (nU + RU - 1) / RU;
-
-    // ---- stage in: registers -> padded LDS rows (round 0 was loaded ahead of time)
-    for (uint32_t r = 0; r < rounds; r++) {
-      if (r > 0) bs_load_round<ES, G, NT, UPT>(p, c, r, v);
-      uint32_t f = (r * RU + tid) * G;
-      uint32_t t = p.div_n.div(f);
-      uint32_t rr = f - t * n;
-      uint32_t i = p.div_B.div(rr);
-      uint32_t j = rr - i * B;
-#pragma unroll
-      for (int k = 0; k < UPT; k++) {
-        uint32_t u = r * RU + tid + k * NT;
-        if (u < nU) {
-#pragma unroll
-          for (int e = 0; e < G; e++) {
-            lds[(t * a + i) * Bp + j] = unit_get<ES, G>(v[k], e);
-            if (e < G - 1) {
-              if (++j == B) { j = 0; if (++i == a) { i = 0; ++t; } }
-            }
-          }
-        }
-        j += p.din_j; if (j >= B) { j -= B; ++i; }
-        i += p.din_i; if (i >= a) { i -= a; ++t; }
-        t += p.din_t;
-      }
-    }
-    if constexpr (G > 1) {
-      for (uint32_t f2 = nU * G + tid; f2 < C; f2 += NT) {
-        uint32_t tt = p.div_n.div(f2), rr = f2 - tt * n;
-        uint32_t ii = p.div_B.div(rr), jj = rr - ii * B;
-        lds[(tt * a + ii) * Bp + jj] = reinterpret_cast<E *>(gbase)[f2];
-      }
-    }
+    bs_stage_in<ES, G, NT, UPT>(p, c, v, lds);
     __syncthreads();
-
-    // ---- prefetch the next chunk (a disjoint region) while this one is written back
+    // Prefetch the next chunk (a disjoint region) while this one is written back.
     const uint64_t cn = c + gridDim.x;
     if (cn < p.nchunks) bs_load_round<ES, G, NT, UPT>(p, cn, 0, v);
-
-    // ---- gather out: output position o = t*n + j*a + i reads LDS (t, i, j)
-    {
-      uint32_t o = tid * G;
-      uint32_t t = p.div_n.div(o);
-      uint32_t rr = o - t * n;
-      uint32_t j = p.div_a.div(rr);
-      uint32_t i = rr - j * a;
-      for (uint32_t u = tid; u < nU; u += NT) {
-        U w{};
-#pragma unroll
-        for (int e = 0; e < G; e++) {
-          unit_put<ES, G>(w, e, lds[(t * a + i) * Bp + j]);
-          if (e < G - 1) {
-            if (++i == a) { i = 0; if (++j == B) { j = 0; ++t; } }
-          }
-        }
-        __builtin_nontemporal_store(w, gu + u);
-        i += p.dout_i; if (i >= a) { i -= a; ++j; }
-        j += p.dout_j; if (j >= B) { j -= B; ++t; }
-        t += p.dout_t;
-      }
-      if constexpr (G > 1) {
-        for (uint32_t o2 = nU * G + tid; o2 < C; o2 += NT) {
-          uint32_t tt = p.div_n.div(o2), rr2 = o2 - tt * n;
-          uint32_t jj = p.div_a.div(rr2), ii = rr2 - jj * a;
-          reinterpret_cast<E *>(gbase)[o2] = lds[(tt * a + ii) * Bp + jj];
-        }
-      }
-    }
+    bs_gather_out<ES, G, NT>(p, c, lds);
     __syncthreads();
     if (cn >= p.nchunks) break;
     c = cn;
@@ -182,44 +69,8 @@ __global__ __launch_bounds__(NT) void k_010_bs(Bs010Params p) {
 // ---------------------------------------------------------------------------------------
 template <int ES, int NT>
 __global__ __launch_bounds__(NT) void k_010_pttwac(Pt010Params p) {
-  using E = typename Elem<ES>::T;
   extern __shared__ __align__(16) uint32_t lflags[];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t a = p.a, B = p.B, n = p.n, sh = p.sh;
-  for (uint64_t t = blockIdx.x; t < p.A; t += gridDim.x) {
-    E *in = reinterpret_cast<E *>(p.base) + t * uint64_t(n);
-    for (uint32_t w = tid; w < sh; w += NT) lflags[w] = 0;
-    __syncthreads();
-    const uint32_t last = n - 1;
-    for (uint32_t s = tid; s < last; s += NT) {
-      uint32_t q = p.div_B.div(s);
-      uint32_t nxt = (s - q * B) * a + q;
-      if (nxt == s) continue;
-      uint32_t bi = p.div_sh.div(s);
-      uint32_t word = s - bi * sh, mask = 1u << bi;
-      if (__hip_atomic_load(&lflags[word], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) &
-          mask)
-        continue;
-      E d1 = in[s];
-      wait_vm_loads();
-      if (__hip_atomic_load(&lflags[word], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) &
-          mask)
-        continue;
-      while (true) {
-        E d2 = in[nxt];
-        uint32_t bj = p.div_sh.div(nxt);
-        uint32_t wj = nxt - bj * sh, mj = 1u << bj;
-        uint32_t old = __hip_atomic_fetch_or(&lflags[wj], mj, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (old & mj) break;
-        in[nxt] = d1;
-        d1 = d2;
-        q = p.div_B.div(nxt);
-        nxt = (nxt - q * B) * a + q;
-      }
-    }
-    __syncthreads();
-  }
+  for (uint64_t t = blockIdx.x; t < p.A; t += gridDim.x) pt010_tile<ES, NT>(p, t, lflags);
 }
 
 // ---------------------------------------------------------------------------------------

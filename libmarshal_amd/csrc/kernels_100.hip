@@ -28,104 +28,34 @@
 // a stale cached copy of a slot can only ever hold the slot's original content, which is
 // exactly what a walker that wins that slot needs.
 #include "kernels.h"
+#include "kernels_dev.h"
 
 #include <algorithm>
 
 namespace lm {
 
-template <bool I64>
-__device__ __forceinline__ uint64_t next_slot(uint64_t s, const Pt100Params &p) {
-  if constexpr (I64) {
-    uint64_t q = p.div_C64.div(s);
-    return (s - q * p.C) * p.R + q;
-  } else {
-    uint32_t s32 = uint32_t(s);
-    uint32_t q = p.div_C32.div(s32);
-    return uint64_t((s32 - q * uint32_t(p.C)) * uint32_t(p.R) + q);
-  }
-}
-
-template <int UB, int UPL>
-__device__ __forceinline__ void load_piece(typename Elem<UB>::T (&d)[UPL],
-                                           const unsigned char *src, uint32_t sub,
-                                           uint32_t vw, uint32_t pbytes) {
-  using V = typename Elem<UB>::T;
-#pragma unroll
-  for (int k = 0; k < UPL; k++) {
-    uint32_t off = (sub + k * vw) * UB;
-    if (off < pbytes) d[k] = *reinterpret_cast<const V *>(src + off);
-  }
-}
-
-template <int UB, int UPL>
-__device__ __forceinline__ void store_piece(unsigned char *dst,
-                                            const typename Elem<UB>::T (&d)[UPL],
-                                            uint32_t sub, uint32_t vw, uint32_t pbytes) {
-  using V = typename Elem<UB>::T;
-#pragma unroll
-  for (int k = 0; k < UPL; k++) {
-    uint32_t off = (sub + k * vw) * UB;
-    if (off < pbytes) *reinterpret_cast<V *>(dst + off) = d[k];
-  }
-}
-
 template <int UB, int UPL, bool I64>
 __global__ __launch_bounds__(256) void k_100_pttwac(Pt100Params p) {
-  using V = typename Elem<UB>::T;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t vwl = p.vw_log2;
-  const uint32_t vw = 1u << vwl;
-  const uint32_t sub = lane & (vw - 1u);
-  const uint32_t leader = lane & ~(vw - 1u);
-  const bool is_leader = (sub == 0);
-  const uint64_t gpw = 64u >> vwl;  // walkers per wave
+  const uint64_t gpw = 64u >> p.vw_log2;  // walkers per wave
   const uint64_t wave = uint64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const uint64_t nwaves = uint64_t(gridDim.x) * (blockDim.x >> 6);
-  const uint64_t RC = p.RC, S = p.S;
+  for (uint64_t wb = wave * gpw; wb < p.walkers; wb += nwaves * gpw)
+    pt100_walkers<UB, UPL, I64>(p, wb);
+}
 
-  for (uint64_t wb = wave * gpw; wb < p.walkers; wb += nwaves * gpw) {
-    const uint64_t w = wb + (lane >> vwl);
-    bool walking = w < p.walkers;
-    uint64_t x = 0, s = 0, c = 0, nxt = 0;
-    if (walking) {
-      uint64_t sc = p.div_nsub.div(w);
-      c = w - sc * p.nsub;
-      x = p.div_RC.div(sc);
-      s = sc - x * RC;
-      nxt = next_slot<I64>(s, p);
-      if (nxt == s) walking = false;
-    }
-    unsigned char *ibase = p.base + x * RC * S + c * p.sub_bytes;
-    const uint64_t fbase = x * RC;
-    const uint32_t pbytes = uint32_t(std::min<uint64_t>(p.sub_bytes, S - c * p.sub_bytes));
-
-    // Skip start points whose slot is already final without touching their data.
-    uint32_t f = 0;
-    if (walking && is_leader) f = flag_load(&p.flags[w >> 5]);
-    uint64_t bal = __ballot(walking && is_leader && ((f >> (w & 31)) & 1u));
-    if ((bal >> leader) & 1u) walking = false;
-
-    V d1[UPL], d2[UPL];
-    if (walking) load_piece<UB, UPL>(d1, ibase + s * S, sub, vw, pbytes);
-    wait_vm_loads();
-    if (walking && is_leader) f = flag_load(&p.flags[w >> 5]);
-    bal = __ballot(walking && is_leader && ((f >> (w & 31)) & 1u));
-    if ((bal >> leader) & 1u) walking = false;
-
-    while (walking) {
-      load_piece<UB, UPL>(d2, ibase + nxt * S, sub, vw, pbytes);
-      const uint64_t fi = (fbase + nxt) * p.nsub + c;
-      const uint32_t mask = 1u << (fi & 31);
-      uint32_t old = 0;
-      if (is_leader) old = flag_or(&p.flags[fi >> 5], mask);
-      uint64_t lost = __ballot(is_leader && (old & mask));
-      if ((lost >> leader) & 1u) break;
-      store_piece<UB, UPL>(ibase + nxt * S, d1, sub, vw, pbytes);
-#pragma unroll
-      for (int k = 0; k < UPL; k++) d1[k] = d2[k];
-      nxt = next_slot<I64>(nxt, p);
-    }
-  }
+// ---------------------------------------------------------------------------------------
+// 0100 with one instance per workgroup and the instance's visited bits in LDS.
+//
+// When an instance's grid is small (R*C*nsub bits fit the LDS budget) and there are many
+// instances, claims are workgroup-scope LDS atomics instead of device-scope atomics on a
+// small, heavily shared region of global memory, and the instance's data stays in the L2 of
+// the one XCD that works on it. Same claim protocol as k_100_pttwac; walkers are groups of
+// VW lanes, start points (slot, piece) are strided over the workgroup's walkers.
+// ---------------------------------------------------------------------------------------
+template <int UB, int UPL, int NT>
+__global__ __launch_bounds__(NT) void k_0100_lds(Pt100Params p, uint64_t inst) {
+  extern __shared__ __align__(16) uint32_t lflags[];
+  for (uint64_t x = blockIdx.x; x < inst; x += gridDim.x) pt0100_instance<UB, UPL, NT>(p, x, lflags);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint64_t words, uint32_t value) {
@@ -154,17 +84,21 @@ void plan_100(size_t es, uint64_t inst, uint64_t R, uint64_t C, uint64_t b, Pt10
   uint64_t nsub = 1, per = nU;
   int upl = 1;
   uint32_t vwl = 0;
-  if (nU <= 64) {
+  uint64_t vmax = uint64_t(tuning("pt100_vw_max"));
+  if (vmax < 1 || vmax > 64 || (vmax & (vmax - 1))) vmax = 64;
+  uint32_t vmaxl = 0;
+  while ((1ull << vmaxl) < vmax) vmaxl++;
+  if (nU <= vmax) {
     uint64_t vw = next_pow2(nU);
     while ((1ull << vwl) < vw) vwl++;
   } else {
-    vwl = 6;
-    const uint64_t cap = 64 * 8;
+    vwl = vmaxl;
+    const uint64_t cap = vmax * 8;
     if (nU > cap) {
       nsub = (nU + cap - 1) / cap;
       per = (nU + nsub - 1) / nsub;
     }
-    upl = int(next_pow2((per + 63) / 64));
+    upl = int(next_pow2((per + vmax - 1) / vmax));
   }
   out->ub = ub;
   out->upl = upl;
@@ -198,6 +132,79 @@ hipError_t launch_100_ub(const Pt100Params &p, int upl, uint32_t blocks, hipStre
   return hipErrorInvalidValue;
 }
 }  // namespace
+
+namespace {
+template <int UB, int UPL>
+hipError_t launch_lds_t(const Pt100Params &p, uint64_t inst, size_t lds, uint32_t blocks,
+                        int nt, hipStream_t st) {
+  if (nt == 1024) {
+    auto k = k_0100_lds<UB, UPL, 1024>;
+    if (lds > 65536) {
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(1024), lds, st, p, inst);
+  } else {
+    auto k = k_0100_lds<UB, UPL, 256>;
+    if (lds > 65536) {
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, st, p, inst);
+  }
+  return hipGetLastError();
+}
+
+template <int UB>
+hipError_t launch_lds_ub(const Pt100Params &p, uint64_t inst, int upl, size_t lds,
+                         uint32_t blocks, int nt, hipStream_t st) {
+  switch (upl) {
+    case 1: return launch_lds_t<UB, 1>(p, inst, lds, blocks, nt, st);
+    case 2: return launch_lds_t<UB, 2>(p, inst, lds, blocks, nt, st);
+    case 4: return launch_lds_t<UB, 4>(p, inst, lds, blocks, nt, st);
+    case 8: return launch_lds_t<UB, 8>(p, inst, lds, blocks, nt, st);
+  }
+  return hipErrorInvalidValue;
+}
+}  // namespace
+
+size_t lds0100_bytes(uint64_t R, uint64_t C, const Pt100Plan &pl) {
+  return size_t((R * C * pl.nsub + 31) / 32 * 4);
+}
+
+hipError_t launch_0100_lds(unsigned char *base, uint64_t inst, uint64_t R, uint64_t C,
+                           uint64_t S, const Pt100Plan &pl, uint32_t max_blocks,
+                           hipStream_t st) {
+  Pt100Params p{};
+  p.base = base;
+  p.flags = nullptr;
+  p.R = R;
+  p.C = C;
+  p.RC = R * C;
+  p.S = S;
+  p.nsub = pl.nsub;
+  p.sub_bytes = pl.sub_bytes;
+  p.walkers = p.RC * pl.nsub;
+  p.vw_log2 = pl.vw_log2;
+  p.div_C64 = FastDiv64(C);
+  p.div_RC = FastDiv64(p.RC);
+  p.div_nsub = FastDiv64(pl.nsub);
+  p.div_C32 = FastDiv32(uint32_t(C));
+  size_t lds = lds0100_bytes(R, C, pl);
+  // Few instances: make each workgroup wide so that the machine still has enough walkers.
+  int nt = inst < uint64_t(device_info().cus) * 2 ? 1024 : 256;
+  uint32_t blocks = uint32_t(std::min<uint64_t>(inst, max_blocks));
+  switch (pl.ub) {
+    case 1: return launch_lds_ub<1>(p, inst, pl.upl, lds, blocks, nt, st);
+    case 2: return launch_lds_ub<2>(p, inst, pl.upl, lds, blocks, nt, st);
+    case 4: return launch_lds_ub<4>(p, inst, pl.upl, lds, blocks, nt, st);
+    case 8: return launch_lds_ub<8>(p, inst, pl.upl, lds, blocks, nt, st);
+    case 16: return launch_lds_ub<16>(p, inst, pl.upl, lds, blocks, nt, st);
+  }
+  return hipErrorInvalidValue;
+}
 
 hipError_t launch_100_pttwac(unsigned char *base, uint32_t *flags, uint64_t inst, uint64_t R,
                              uint64_t C, uint64_t S, const Pt100Plan &pl, uint32_t max_blocks,

@@ -13,7 +13,8 @@ def header_symbols():
     for h in ("hip_marshal.h", "marshal.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"\b(?:bool|void|size_t|int64_t|const char\s*\*)\s*\**\s*(\w+)\s*\(", text):
+        for m in re.finditer(r"^(?:const\s+)?\w+\s*\*?\s*((?:hip|gpu)_\w+)\s*\(", text,
+                             flags=re.M):
             syms.add(m.group(1))
     return syms
 

@@ -294,3 +294,52 @@ def test_config2_full_size_round_trip(hip, orc):
     assert torch.equal(d[dst].view(torch.int32), orig[sample].view(torch.int32))
     assert hip.asta_soa(d, n, w, t) is False
     assert torch.equal(d.view(torch.int32), orig.view(torch.int32))
+
+
+@pytest.mark.parametrize("es", ES)
+def test_0100_lds_and_global_bit_forms_agree(hip, orc, es):
+    """The LDS-bit (one workgroup per instance) and global-bit forms of 0100 give the same
+    bytes, including instances whose super-elements are split into pieces."""
+    cases = ((100, 100, 72, 18), (40, 7, 9, 1), (3, 33, 17, 700), (64, 16, 64, 1),
+             (5, 2, 3, 4096))
+    try:
+        for (A, a, B, b) in cases:
+            x = orc.tagged(A * a * B * b, es, seed=A * a + B * b + es)
+            want = orc.op_0100(x, A, a, B, b, es=es)
+            for min_inst in (1, 1 << 20):
+                hip.set_tuning("lds0100_min_inst", min_inst)
+                for vw in (64, 8, 1):
+                    hip.set_tuning("pt100_vw_max", vw)
+                    say("0100 forms", es, (A, a, B, b), min_inst, vw)
+                    check(hip.transpose_0100, x, want, A, a, B, b, elem_bytes=es)
+    finally:
+        hip.set_tuning("lds0100_min_inst", 32)
+        hip.set_tuning("pt100_vw_max", 8)
+
+
+@pytest.mark.parametrize("es", ES)
+def test_row_column_decomposition_route(hip, orc, es):
+    """stages = 5: the LDS-staged row/column decomposition, on coprime and non-coprime shapes,
+    primes included (no factorisation needed)."""
+    shapes = [(2, 3), (3, 2), (17, 19), (12, 18), (64, 48), (1000, 999), (997, 1009),
+              (4096, 3001), (256, 4096), (5, 1024), (1024, 5), (2, 6000), (6000, 2)]
+    for (M, N) in shapes:
+        if M * N * es > 200 << 20 or max(M, N) * es > 160 * 1024:
+            continue
+        say("rc", es, M, N)
+        x = orc.tagged(M * N, es, seed=M * 7 + N)
+        want = orc.op_transpose(x, M, N, es=es)
+        t = dev(x)
+        assert hip.transpose(t, 1, M, 1, N, 1, 5, elem_bytes=es) is False
+        assert hip.last_route() == "RC-"
+        assert np.array_equal(host(t), want)
+
+
+@pytest.mark.parametrize("es", [1, 4, 16])
+def test_soa_small_tiles_take_row_column_route(hip, orc, es):
+    """SoA->ASTA with a small tile is a transpose of b*es-byte super-elements; it is routed
+    through the row/column decomposition and must match the oracle's 100."""
+    for (h, w, t) in ((999, 37, 1), (1002, 13, 3), (4096, 7, 2), (35, 4096, 5)):
+        say("soa small tile", es, h, w, t)
+        x = orc.tagged(h * w, es, seed=h + w + t)
+        check(hip.soa_asta, x, orc.op_100(x, w, h // t, t, es=es), h, w, t, elem_bytes=es)
