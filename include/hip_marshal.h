@@ -172,6 +172,45 @@ const char *hip_marshal_batch_route(const hip_marshal_batch *batch, int64_t inde
 void hip_marshal_batch_destroy(hip_marshal_batch *batch);
 
 /* ---------------------------------------------------------------------------------------
+ * Serving: a request loop with continuous batching. No counterpart in the reference.
+ * hip_marshal_server_create starts a scheduler thread that owns `streams` HIP streams.
+ * Submitted requests convert device-resident buffers in place; they are admitted in arrival
+ * order while the bytes of admitted, unfinished requests (buffer + visited bits) fit
+ * hbm_budget_bytes, packed into dynamic batches of at most max_batch requests and launched
+ * on an idle stream, so requests arriving while others run are admitted as budget and
+ * streams free up and run concurrently with them. A request larger than the budget is
+ * rejected (-1). The caller keeps each buffer alive and untouched until its request is done
+ * and must not submit overlapping buffers concurrently. Results are bit-identical to the
+ * plain entry points run on each buffer alone.
+ * ------------------------------------------------------------------------------------- */
+typedef struct hip_marshal_server hip_marshal_server;
+typedef struct hip_marshal_server_stats {
+  int64_t completed, failed, batches, max_batch;
+  double p50_ms, p99_ms, mean_ms, max_ms; /* arrival -> completion */
+  double bytes;                           /* algorithmic bytes moved (2 x each buffer) */
+  double span_s;                          /* first arrival -> last completion */
+} hip_marshal_server_stats;
+hip_marshal_server *hip_marshal_server_create(size_t hbm_budget_bytes, int streams,
+                                              int max_batch);
+/* Submission with the arrival time on hip_marshal_server_now()'s clock (0 = now). Returns
+ * the request id, or -1 for an invalid request. */
+int64_t hip_marshal_server_submit(hip_marshal_server *server, const hip_marshal_desc *request);
+int64_t hip_marshal_server_submit_at(hip_marshal_server *server,
+                                     const hip_marshal_desc *request, double arrival_s);
+double hip_marshal_server_now(void);
+/* 0 queued, 1 running, 2 done, -1 failed or unknown. */
+int hip_marshal_server_status(hip_marshal_server *server, int64_t id);
+/* false once the request is done; true on failure or timeout. */
+bool hip_marshal_server_wait(hip_marshal_server *server, int64_t id, double timeout_s);
+bool hip_marshal_server_drain(hip_marshal_server *server, double timeout_s);
+bool hip_marshal_server_request_times(hip_marshal_server *server, int64_t id, double *arrival,
+                                      double *launch, double *done);
+void hip_marshal_server_get_stats(hip_marshal_server *server, hip_marshal_server_stats *out);
+void hip_marshal_server_reset_stats(hip_marshal_server *server);
+/* Waits for every submitted request, then releases the streams and buffers. */
+void hip_marshal_server_destroy(hip_marshal_server *server);
+
+/* ---------------------------------------------------------------------------------------
  * Library configuration and introspection (no counterpart in the reference, where these
  * are compile-time macros: MAX_MEM src/cl_marshal.cc:45-59, LIBMARSHAL_OCL_PROFILE).
  * ------------------------------------------------------------------------------------- */

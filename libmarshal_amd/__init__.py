@@ -27,6 +27,18 @@ class Desc(ctypes.Structure):
                 ("rows", ctypes.c_int64), ("cols", ctypes.c_int64), ("tile", ctypes.c_int64)]
 
 
+class ServerStats(ctypes.Structure):
+    """hip_marshal_server_stats of include/hip_marshal.h."""
+    _fields_ = [("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
+                ("batches", ctypes.c_int64), ("max_batch", ctypes.c_int64),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("bytes", ctypes.c_double), ("span_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def lib():
     """Loads the HIP library (raises if it has not been built)."""
     global _lib
@@ -80,6 +92,19 @@ def lib():
         "hip_marshal_batch_info": (i64, [vp, ctypes.c_char_p]),
         "hip_marshal_batch_route": (ctypes.c_char_p, [vp, i64]),
         "hip_marshal_batch_destroy": (None, [vp]),
+        "hip_marshal_server_create": (vp, [sz, i32, i32]),
+        "hip_marshal_server_submit": (i64, [vp, ctypes.POINTER(Desc)]),
+        "hip_marshal_server_submit_at": (i64, [vp, ctypes.POINTER(Desc), ctypes.c_double]),
+        "hip_marshal_server_now": (ctypes.c_double, []),
+        "hip_marshal_server_status": (i32, [vp, i64]),
+        "hip_marshal_server_wait": (b, [vp, i64, ctypes.c_double]),
+        "hip_marshal_server_drain": (b, [vp, ctypes.c_double]),
+        "hip_marshal_server_request_times": (b, [vp, i64, ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_double)]),
+        "hip_marshal_server_get_stats": (None, [vp, ctypes.POINTER(ServerStats)]),
+        "hip_marshal_server_reset_stats": (None, [vp]),
+        "hip_marshal_server_destroy": (None, [vp]),
         "hip_marshal_get_tuning": (i64, [ctypes.c_char_p]),
         "gpu_aos_asta_bs_float": (b, [vp, i32, i32, i32, vp]),
         "gpu_aos_asta_bs_double": (b, [vp, i32, i32, i32, vp]),
@@ -107,6 +132,11 @@ EXPORTED = [
     "hip_marshal_flag_bytes_0100", "hip_marshal_version", "hip_marshal_set_tuning",
     "hip_marshal_get_tuning", "hip_marshal_plan_transpose", "hip_marshal_batch_create", "hip_marshal_batch_run",
     "hip_marshal_batch_info", "hip_marshal_batch_route", "hip_marshal_batch_destroy",
+    "hip_marshal_server_create", "hip_marshal_server_submit", "hip_marshal_server_submit_at",
+    "hip_marshal_server_now", "hip_marshal_server_status", "hip_marshal_server_wait",
+    "hip_marshal_server_drain", "hip_marshal_server_request_times",
+    "hip_marshal_server_get_stats", "hip_marshal_server_reset_stats",
+    "hip_marshal_server_destroy",
     "gpu_aos_asta_bs_float", "gpu_aos_asta_bs_double", "gpu_aos_asta_pttwac",
     "gpu_soa_asta_pttwac",
 ]
@@ -288,6 +318,63 @@ class Batch:
     def close(self):
         if self._h:
             lib().hip_marshal_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop (hip_marshal_server_*)."""
+
+    def __init__(self, hbm_budget_bytes, streams=4, max_batch=256):
+        self._h = lib().hip_marshal_server_create(hbm_budget_bytes, streams, max_batch)
+        if not self._h:
+            raise ValueError("hip_marshal_server_create failed")
+
+    @staticmethod
+    def now():
+        return lib().hip_marshal_server_now()
+
+    def submit(self, tensor, op, elem_bytes, rows, cols, tile=0, arrival=0.0, ptr=None):
+        """Submits a conversion of `tensor` (or of the raw device pointer `ptr`); returns its id."""
+        p = ptr if ptr is not None else _ptr(tensor).value
+        d = Desc(p, op, elem_bytes, rows, cols, tile)
+        rid = lib().hip_marshal_server_submit_at(self._h, ctypes.byref(d), arrival)
+        if rid < 0:
+            raise ValueError("request rejected")
+        return rid
+
+    def status(self, rid):
+        return lib().hip_marshal_server_status(self._h, rid)
+
+    def wait(self, rid, timeout=60.0):
+        return lib().hip_marshal_server_wait(self._h, rid, timeout)
+
+    def drain(self, timeout=600.0):
+        return lib().hip_marshal_server_drain(self._h, timeout)
+
+    def times(self, rid):
+        a, l, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        if lib().hip_marshal_server_request_times(self._h, rid, ctypes.byref(a), ctypes.byref(l),
+                                                  ctypes.byref(d)):
+            raise KeyError(rid)
+        return a.value, l.value, d.value
+
+    def stats(self):
+        st = ServerStats()
+        lib().hip_marshal_server_get_stats(self._h, ctypes.byref(st))
+        return st.as_dict()
+
+    def reset_stats(self):
+        lib().hip_marshal_server_reset_stats(self._h)
+
+    def close(self):
+        if self._h:
+            lib().hip_marshal_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

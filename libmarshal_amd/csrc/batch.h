@@ -1,0 +1,49 @@
+// batch.h - host-side planning of grouped work lists (shared by the batch and the server).
+#pragma once
+
+#include "kernels.h"
+#include "../../include/hip_marshal.h"
+
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace lm {
+
+// Kernel form + template instantiation of a group.
+struct GroupKey {
+  int kind, a, b, c;
+  bool operator<(const GroupKey &o) const {
+    return std::tie(kind, a, b, c) < std::tie(o.kind, o.a, o.b, o.c);
+  }
+};
+
+// One persistent kernel launch: the work list of every stage of one form in one level.
+struct Group {
+  GroupKey key;
+  std::vector<unsigned char> items;  // packed parameter blocks
+  size_t item_size = 0;
+  std::vector<uint64_t> prefix{0};   // prefix sum of work units per item
+  size_t lds = 0;
+  size_t items_off = 0, prefix_off = 0;  // offsets in the parameter buffer
+  const void *func = nullptr;
+  uint32_t grid = 0;
+  uint32_t block = 256;
+};
+
+struct BatchPlan {
+  std::vector<std::vector<Group>> levels;
+  uint64_t flag_words = 0;  // visited bits of every global-bit stage
+  size_t param_bytes = 0;   // size of the parameter buffer
+  int64_t stages = 0;
+  double bytes = 0;         // algorithmic bytes of one run (2 x every matrix)
+  std::vector<std::string> routes;
+};
+
+bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, const char *who);
+// Host image of the parameter buffer, with the flag offsets turned into pointers into `flags`.
+void build_image(const BatchPlan &b, const uint32_t *flags, std::vector<unsigned char> &host);
+// Clears the flags and launches every group, level by level, on `st`.
+bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_t st);
+
+}  // namespace lm

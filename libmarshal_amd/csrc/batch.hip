@@ -12,8 +12,8 @@
 //
 // No counterpart in the reference, which launches one kernel per stage per matrix and waits
 // after each (src/cl_marshal.cc:561-776, include/cl_profile.h:23-24).
+#include "batch.h"
 #include "kernels_dev.h"
-#include "../../include/hip_marshal.h"
 
 #include <algorithm>
 #include <cstring>
@@ -145,27 +145,6 @@ __global__ __launch_bounds__(1024) void kg_rc_col(const RcParams *items, const u
 // ---- host side -----------------------------------------------------------------------------
 
 namespace {
-
-// Kernel form + template instantiation of a group.
-struct GroupKey {
-  int kind, a, b, c;
-  bool operator<(const GroupKey &o) const {
-    return std::tie(kind, a, b, c) < std::tie(o.kind, o.a, o.b, o.c);
-  }
-};
-
-struct Group {
-  GroupKey key;
-  std::vector<unsigned char> items;  // packed parameter blocks
-  size_t item_size = 0;
-  std::vector<uint64_t> prefix{0};
-  size_t lds = 0;
-  // device copies (offsets into the batch's parameter buffer)
-  size_t items_off = 0, prefix_off = 0;
-  const void *func = nullptr;
-  uint32_t grid = 0;
-  uint32_t block = 256;
-};
 
 template <typename P>
 void add_item(Group &g, const P &p, uint64_t units) {
@@ -327,26 +306,11 @@ const void *lds_func(int ub, int upl) {
 
 }  // namespace
 
-struct Batch {
-  std::vector<std::vector<Group>> levels;
-  void *params = nullptr;    // device copy of all item arrays and prefix sums
-  uint32_t *flags = nullptr; // visited bits of every global-bit stage
-  uint64_t flag_words = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  bool use_graph = true;
-  int64_t stages = 0;
-  double bytes = 0;  // algorithmic bytes of one run (2 x every matrix)
-  std::vector<std::string> routes;
-};
-
-namespace {
-
-bool enqueue(Batch &b, hipStream_t st) {
-  if (b.flag_words) LM_CHECK_HIP(launch_fill_u32(b.flags, b.flag_words, 0, st));
-  unsigned char *base = static_cast<unsigned char *>(b.params);
+bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_t st) {
+  if (b.flag_words) LM_CHECK_HIP(launch_fill_u32(flags, b.flag_words, 0, st));
+  unsigned char *base = static_cast<unsigned char *>(params);
   for (auto &level : b.levels) {
-    for (Group &g : level) {
+    for (const Group &g : level) {
       const void *items = base + g.items_off;
       const uint64_t *prefix = reinterpret_cast<const uint64_t *>(base + g.prefix_off);
       uint32_t n = uint32_t(g.prefix.size() - 1);
@@ -358,27 +322,25 @@ bool enqueue(Batch &b, hipStream_t st) {
   return false;
 }
 
-}  // namespace
+void build_image(const BatchPlan &b, const uint32_t *flags, std::vector<unsigned char> &host) {
+  host.assign(b.param_bytes, 0);
+  for (auto &lvl : b.levels)
+    for (const Group &g : lvl) {
+      std::memcpy(host.data() + g.items_off, g.items.data(), g.items.size());
+      std::memcpy(host.data() + g.prefix_off, g.prefix.data(), g.prefix.size() * 8);
+      if (g.key.kind != 2) continue;  // global visited bits: offsets become pointers
+      for (size_t off = 0; off < g.items.size(); off += sizeof(Pt100Params)) {
+        Pt100Params p;
+        std::memcpy(&p, host.data() + g.items_off + off, sizeof(p));
+        p.flags = const_cast<uint32_t *>(flags) + reinterpret_cast<uintptr_t>(p.flags);
+        std::memcpy(host.data() + g.items_off + off, &p, sizeof(p));
+      }
+    }
+}
 
-}  // namespace lm
-
-using namespace lm;
-
-struct hip_marshal_batch {
-  Batch b;
-};
-
-extern "C" {
-
-hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64_t count,
-                                            int use_graph) {
-  if (!descs || count < 1) {
-    fprintf(stderr, "[libmarshal] hip_marshal_batch_create: empty batch\n");
-    return nullptr;
-  }
-  auto *hb = new hip_marshal_batch();
-  Batch &b = hb->b;
-  b.use_graph = use_graph != 0;
+// Plans every descriptor and groups the stages; flags of global-bit stages are recorded as
+// word offsets into the batch's flag space (see build_image).
+bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, const char *who) {
   // 1. routes
   std::vector<std::vector<StageRec>> per(count);
   b.routes.resize(count);
@@ -391,10 +353,8 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
               d.rows >= 1 && d.cols >= 1;
     if (ok && d.op != HIP_MARSHAL_OP_TRANSPOSE) ok = d.tile >= 1 && d.rows % d.tile == 0;
     if (!ok) {
-      fprintf(stderr, "[libmarshal] hip_marshal_batch_create: invalid descriptor %lld\n",
-              (long long)i);
-      hip_marshal_batch_destroy(hb);
-      return nullptr;
+      fprintf(stderr, "[libmarshal] %s: invalid descriptor %lld\n", who, (long long)i);
+      return false;
     }
     uint64_t h = uint64_t(d.rows), w = uint64_t(d.cols), t = uint64_t(d.tile);
     std::string route;
@@ -422,10 +382,8 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
         ok = false;
     }
     if (!ok) {
-      fprintf(stderr, "[libmarshal] hip_marshal_batch_create: cannot plan descriptor %lld\n",
-              (long long)i);
-      hip_marshal_batch_destroy(hb);
-      return nullptr;
+      fprintf(stderr, "[libmarshal] %s: cannot plan descriptor %lld\n", who, (long long)i);
+      return false;
     }
     b.routes[i] = route;
     b.stages += int64_t(per[i].size());
@@ -440,11 +398,6 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
       flag_off[i].push_back(b.flag_words);
       if (r.kind == StageKind::PT100G) b.flag_words += (r.pt.flag_words + 63) / 64 * 64;
     }
-  }
-  if (b.flag_words && hipMalloc(&b.flags, b.flag_words * 4) != hipSuccess) {
-    fprintf(stderr, "[libmarshal] hip_marshal_batch_create: flag allocation failed\n");
-    hip_marshal_batch_destroy(hb);
-    return nullptr;
   }
   // 3. groups per level
   const DeviceInfo &di = device_info();
@@ -475,7 +428,7 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
           GroupKey k{2, r.pt.ub, r.pt.upl, i64 ? 1 : 0};
           Group &g = groups[k];
           g.key = k;
-          Pt100Params p = make_pt100(r, b.flags + flag_off[i][lv]);
+          Pt100Params p = make_pt100(r, reinterpret_cast<uint32_t *>(uintptr_t(flag_off[i][lv])));
           uint64_t gpw = 64u >> p.vw_log2;
           add_item(g, p, (p.walkers + gpw - 1) / gpw);
           break;
@@ -545,60 +498,86 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
       if (g.lds > 65536 &&
           hipFuncSetAttribute(g.func, hipFuncAttributeMaxDynamicSharedMemorySize, int(g.lds)) !=
               hipSuccess) {
-        fprintf(stderr, "[libmarshal] hip_marshal_batch_create: LDS attribute failed\n");
-        hip_marshal_batch_destroy(hb);
-        return nullptr;
+        fprintf(stderr, "[libmarshal] %s: LDS attribute failed\n", who);
+        return false;
       }
       lvl.push_back(std::move(g));
     }
     b.levels.push_back(std::move(lvl));
   }
-  // 4. one device buffer for every work list
-  size_t total_bytes = 0;
+  // 4. layout of every work list in one parameter buffer
+  b.param_bytes = 0;
   for (auto &lvl : b.levels)
     for (Group &g : lvl) {
-      g.items_off = total_bytes;
-      total_bytes += (g.items.size() + 255) / 256 * 256;
-      g.prefix_off = total_bytes;
-      total_bytes += (g.prefix.size() * 8 + 255) / 256 * 256;
+      g.items_off = b.param_bytes;
+      b.param_bytes += (g.items.size() + 255) / 256 * 256;
+      g.prefix_off = b.param_bytes;
+      b.param_bytes += (g.prefix.size() * 8 + 255) / 256 * 256;
     }
-  if (total_bytes) {
-    if (hipMalloc(&b.params, total_bytes) != hipSuccess) {
-      fprintf(stderr, "[libmarshal] hip_marshal_batch_create: parameter allocation failed\n");
-      hip_marshal_batch_destroy(hb);
-      return nullptr;
-    }
-    std::vector<unsigned char> host(total_bytes, 0);
-    for (auto &lvl : b.levels)
-      for (Group &g : lvl) {
-        std::memcpy(host.data() + g.items_off, g.items.data(), g.items.size());
-        std::memcpy(host.data() + g.prefix_off, g.prefix.data(), g.prefix.size() * 8);
-      }
-    if (hipMemcpy(b.params, host.data(), total_bytes, hipMemcpyHostToDevice) != hipSuccess) {
-      fprintf(stderr, "[libmarshal] hip_marshal_batch_create: parameter upload failed\n");
-      hip_marshal_batch_destroy(hb);
-      return nullptr;
-    }
+  return true;
+}
+
+}  // namespace lm
+
+using namespace lm;
+
+struct hip_marshal_batch {
+  BatchPlan b;
+  void *params = nullptr;     // device copy of all item arrays and prefix sums
+  uint32_t *flags = nullptr;  // visited bits of every global-bit stage
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = true;
+};
+
+extern "C" {
+
+hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64_t count,
+                                            int use_graph) {
+  const char *who = "hip_marshal_batch_create";
+  if (!descs || count < 1) {
+    fprintf(stderr, "[libmarshal] %s: empty batch\n", who);
+    return nullptr;
+  }
+  auto *hb = new hip_marshal_batch();
+  hb->use_graph = use_graph != 0;
+  BatchPlan &b = hb->b;
+  if (!plan_batch(descs, count, b, who)) {
+    hip_marshal_batch_destroy(hb);
+    return nullptr;
+  }
+  bool ok = true;
+  if (b.flag_words) ok = hipMalloc(&hb->flags, b.flag_words * 4) == hipSuccess;
+  if (ok && b.param_bytes) ok = hipMalloc(&hb->params, b.param_bytes) == hipSuccess;
+  if (ok && b.param_bytes) {
+    std::vector<unsigned char> host;
+    build_image(b, hb->flags, host);
+    ok = hipMemcpy(hb->params, host.data(), b.param_bytes, hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (!ok) {
+    fprintf(stderr, "[libmarshal] %s: device allocation or upload failed\n", who);
+    hip_marshal_batch_destroy(hb);
+    return nullptr;
   }
   return hb;
 }
 
 bool hip_marshal_batch_run(hip_marshal_batch *hb, hipStream_t st, uint64_t *elapsed_ns) {
   if (!hb) return true;
-  Batch &b = hb->b;
+  BatchPlan &b = hb->b;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (elapsed_ns) {
     LM_CHECK_HIP(hipEventCreate(&e0));
     LM_CHECK_HIP(hipEventCreate(&e1));
     LM_CHECK_HIP(hipEventRecord(e0, st));
   }
-  if (b.use_graph) {
-    if (!b.exec) {
+  if (hb->use_graph) {
+    if (!hb->exec) {
       // Captured once, on a private stream (the caller's may be the null stream).
       hipStream_t cs = nullptr;
       LM_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
       hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-      bool err = e != hipSuccess || enqueue(b, cs);
+      bool err = e != hipSuccess || enqueue_batch(b, hb->params, hb->flags, cs);
       hipGraph_t g = nullptr;
       if (e == hipSuccess) e = hipStreamEndCapture(cs, &g);
       hipStreamDestroy(cs);
@@ -607,11 +586,11 @@ bool hip_marshal_batch_run(hip_marshal_batch *hb, hipStream_t st, uint64_t *elap
         fprintf(stderr, "[libmarshal] hip_marshal_batch_run: capture failed\n");
         return true;
       }
-      b.graph = g;
-      LM_CHECK_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+      hb->graph = g;
+      LM_CHECK_HIP(hipGraphInstantiate(&hb->exec, hb->graph, nullptr, nullptr, 0));
     }
-    LM_CHECK_HIP(hipGraphLaunch(b.exec, st));
-  } else if (enqueue(b, st)) {
+    LM_CHECK_HIP(hipGraphLaunch(hb->exec, st));
+  } else if (enqueue_batch(b, hb->params, hb->flags, st)) {
     return true;
   }
   if (elapsed_ns) {
@@ -628,7 +607,7 @@ bool hip_marshal_batch_run(hip_marshal_batch *hb, hipStream_t st, uint64_t *elap
 
 int64_t hip_marshal_batch_info(const hip_marshal_batch *hb, const char *key) {
   if (!hb || !key) return -1;
-  const Batch &b = hb->b;
+  const BatchPlan &b = hb->b;
   if (!strcmp(key, "levels")) return int64_t(b.levels.size());
   if (!strcmp(key, "stages")) return b.stages;
   if (!strcmp(key, "flag_bytes")) return int64_t(b.flag_words * 4);
@@ -648,11 +627,10 @@ const char *hip_marshal_batch_route(const hip_marshal_batch *hb, int64_t i) {
 
 void hip_marshal_batch_destroy(hip_marshal_batch *hb) {
   if (!hb) return;
-  Batch &b = hb->b;
-  if (b.exec) hipGraphExecDestroy(b.exec);
-  if (b.graph) hipGraphDestroy(b.graph);
-  if (b.params) hipFree(b.params);
-  if (b.flags) hipFree(b.flags);
+  if (hb->exec) hipGraphExecDestroy(hb->exec);
+  if (hb->graph) hipGraphDestroy(hb->graph);
+  if (hb->params) hipFree(hb->params);
+  if (hb->flags) hipFree(hb->flags);
   delete hb;
 }
 

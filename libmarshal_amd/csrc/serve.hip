@@ -1,0 +1,376 @@
+// serve.hip - a request loop with continuous batching for in-place layout conversions.
+//
+// Clients submit conversions of device-resident buffers; a scheduler thread admits queued
+// requests in arrival order while the bytes of admitted, unfinished requests stay within the
+// HBM budget (conversions are in place, so a request's footprint is its buffer plus visited
+// bits, never a second copy), packs everything admitted into one dynamic batch, and launches
+// it as grouped persistent kernels (the batched form's work lists) on whichever of its
+// streams is idle. Requests that arrive while batches are on the GPU are admitted as soon as
+// budget and a stream free up and run concurrently with them. Completion is observed with an
+// event per batch; latency is arrival -> completion.
+//
+// No counterpart in the reference (a single-call library, src/cl_marshal.cc).
+#include "batch.h"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace lm {
+
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Req {
+  hip_marshal_desc d;
+  size_t bytes = 0;
+  double t_arrive = 0, t_launch = 0, t_done = 0;
+  int state = 0;  // 0 queued, 1 running, 2 done, -1 failed
+};
+
+struct Slot {
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;
+  void *params = nullptr;
+  size_t params_cap = 0;
+  unsigned char *host = nullptr;  // pinned staging for the parameter upload
+  size_t host_cap = 0;
+  uint32_t *flags = nullptr;
+  size_t flags_cap = 0;
+  bool busy = false;
+  std::vector<int64_t> reqs;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct Server {
+  size_t budget = 0;
+  int max_batch = 256;
+  int device = 0;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<int64_t> queue;
+  std::unordered_map<int64_t, Req> reqs;
+  int64_t next_id = 0;
+  size_t inflight = 0;
+  std::vector<Slot> slots;
+  std::thread th;
+  bool stop = false;
+  // statistics
+  std::vector<double> lat;
+  double bytes_done = 0;
+  int64_t batches = 0, failed = 0, max_in_batch = 0;
+  double t_first = -1, t_last = 0;
+};
+
+namespace {
+
+size_t footprint(const hip_marshal_desc &d) {
+  size_t data = size_t(d.rows) * size_t(d.cols) * size_t(d.elem_bytes);
+  return data + data / 32 + 4096;  // buffer + an upper bound on its visited bits
+}
+
+bool grow(void **p, size_t *cap, size_t want, bool pinned) {
+  if (*cap >= want) return true;
+  size_t n = std::max<size_t>(want + want / 2, size_t(1) << 20);
+  if (*p) {
+    if (pinned) hipHostFree(*p);
+    else hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+  }
+  hipError_t e = pinned ? hipHostMalloc(p, n, hipHostMallocDefault) : hipMalloc(p, n);
+  if (e != hipSuccess) return false;
+  *cap = n;
+  return true;
+}
+
+// Plans and launches one dynamic batch on an idle slot. Called without the lock held.
+bool launch_batch(Server &s, Slot &sl, const std::vector<hip_marshal_desc> &descs) {
+  BatchPlan plan;
+  if (!plan_batch(descs.data(), int64_t(descs.size()), plan, "hip_marshal_server")) return false;
+  if (!grow(reinterpret_cast<void **>(&sl.flags), &sl.flags_cap, plan.flag_words * 4 + 4, false) ||
+      !grow(&sl.params, &sl.params_cap, plan.param_bytes + 256, false) ||
+      !grow(reinterpret_cast<void **>(&sl.host), &sl.host_cap, plan.param_bytes + 256, true))
+    return false;
+  std::vector<unsigned char> img;
+  build_image(plan, sl.flags, img);
+  if (!img.empty()) {
+    std::memcpy(sl.host, img.data(), img.size());
+    if (hipMemcpyAsync(sl.params, sl.host, img.size(), hipMemcpyHostToDevice, sl.st) !=
+        hipSuccess)
+      return false;
+  }
+  if (enqueue_batch(plan, sl.params, sl.flags, sl.st)) return false;
+  return hipEventRecord(sl.ev, sl.st) == hipSuccess;
+}
+
+void scheduler(Server *sp) {
+  Server &s = *sp;
+  hipSetDevice(s.device);
+  std::unique_lock<std::mutex> lk(s.mu);
+  while (true) {
+    bool progressed = false;
+    // 1. retire finished batches
+    for (Slot &sl : s.slots) {
+      if (!sl.busy) continue;
+      hipError_t q = hipEventQuery(sl.ev);
+      if (q == hipErrorNotReady) continue;
+      double t = now_s();
+      for (int64_t id : sl.reqs) {
+        Req &r = s.reqs[id];
+        r.t_done = t;
+        r.state = q == hipSuccess ? 2 : -1;
+        if (q == hipSuccess) {
+          s.lat.push_back(t - r.t_arrive);
+          s.bytes_done += 2.0 * double(r.d.rows) * double(r.d.cols) * double(r.d.elem_bytes);
+        } else {
+          s.failed++;
+        }
+        s.t_last = std::max(s.t_last, t);
+      }
+      s.inflight -= sl.bytes;
+      sl.bytes = 0;
+      sl.reqs.clear();
+      sl.busy = false;
+      progressed = true;
+    }
+    if (progressed) s.cv.notify_all();
+    // 2. admit in arrival order and launch on an idle stream
+    Slot *free_slot = nullptr;
+    for (Slot &sl : s.slots)
+      if (!sl.busy) { free_slot = &sl; break; }
+    if (free_slot && !s.queue.empty()) {
+      std::vector<int64_t> ids;
+      std::vector<hip_marshal_desc> descs;
+      size_t add = 0;
+      while (!s.queue.empty() && int(ids.size()) < s.max_batch) {
+        Req &r = s.reqs[s.queue.front()];
+        if (s.inflight + add + r.bytes > s.budget) break;  // wait for budget to free up
+        add += r.bytes;
+        ids.push_back(s.queue.front());
+        descs.push_back(r.d);
+        s.queue.pop_front();
+      }
+      if (!ids.empty()) {
+        double t = now_s();
+        for (int64_t id : ids) {
+          s.reqs[id].state = 1;
+          s.reqs[id].t_launch = t;
+        }
+        s.inflight += add;
+        free_slot->busy = true;
+        free_slot->bytes = add;
+        free_slot->reqs = ids;
+        s.batches++;
+        s.max_in_batch = std::max<int64_t>(s.max_in_batch, int64_t(ids.size()));
+        lk.unlock();
+        bool ok = launch_batch(s, *free_slot, descs);
+        lk.lock();
+        if (!ok) {  // mark failed; the slot is retired once its stream drains
+          hipStreamSynchronize(free_slot->st);
+          for (int64_t id : ids) {
+            s.reqs[id].state = -1;
+            s.failed++;
+          }
+          s.inflight -= add;
+          free_slot->busy = false;
+          free_slot->bytes = 0;
+          free_slot->reqs.clear();
+          s.cv.notify_all();
+        }
+        continue;  // look for more work right away
+      }
+    }
+    bool any_busy = false;
+    for (Slot &sl : s.slots) any_busy |= sl.busy;
+    if (s.stop && s.queue.empty() && !any_busy) break;
+    // 3. idle: wake on a submission, or poll running batches every 20 us
+    s.cv.wait_for(lk, std::chrono::microseconds(any_busy ? 20 : 1000));
+  }
+}
+
+}  // namespace
+}  // namespace lm
+
+using namespace lm;
+
+struct hip_marshal_server {
+  Server s;
+};
+
+extern "C" {
+
+hip_marshal_server *hip_marshal_server_create(size_t hbm_budget_bytes, int streams,
+                                              int max_batch) {
+  if (hbm_budget_bytes == 0 || streams < 1 || streams > 16 || max_batch < 1) {
+    fprintf(stderr, "[libmarshal] hip_marshal_server_create: bad arguments\n");
+    return nullptr;
+  }
+  auto *h = new hip_marshal_server();
+  Server &s = h->s;
+  s.budget = hbm_budget_bytes;
+  s.max_batch = max_batch;
+  hipGetDevice(&s.device);
+  s.slots.resize(size_t(streams));
+  for (Slot &sl : s.slots) {
+    if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+      fprintf(stderr, "[libmarshal] hip_marshal_server_create: stream creation failed\n");
+      delete h;
+      return nullptr;
+    }
+  }
+  s.th = std::thread(scheduler, &s);
+  return h;
+}
+
+double hip_marshal_server_now(void) { return now_s(); }
+
+int64_t hip_marshal_server_submit_at(hip_marshal_server *h, const hip_marshal_desc *d,
+                                     double arrival) {
+  if (!h || !d) return -1;
+  Server &s = h->s;
+  size_t es = size_t(d->elem_bytes);
+  bool ok = (es == 1 || es == 2 || es == 4 || es == 8 || es == 16) && d->ptr && d->rows >= 1 &&
+            d->cols >= 1 && d->op >= 0 && d->op <= 4 &&
+            reinterpret_cast<uintptr_t>(d->ptr) % std::min<size_t>(es, 16) == 0;
+  if (ok && d->op != HIP_MARSHAL_OP_TRANSPOSE) ok = d->tile >= 1 && d->rows % d->tile == 0;
+  size_t fp = ok ? footprint(*d) : 0;
+  if (!ok || fp > s.budget) {
+    fprintf(stderr, "[libmarshal] hip_marshal_server_submit: invalid request or larger than "
+                    "the HBM budget\n");
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(s.mu);
+  int64_t id = s.next_id++;
+  Req &r = s.reqs[id];
+  r.d = *d;
+  r.bytes = fp;
+  r.t_arrive = arrival > 0 ? arrival : now_s();
+  if (s.t_first < 0 || r.t_arrive < s.t_first) s.t_first = r.t_arrive;
+  s.queue.push_back(id);
+  s.cv.notify_all();
+  return id;
+}
+
+int64_t hip_marshal_server_submit(hip_marshal_server *h, const hip_marshal_desc *d) {
+  return hip_marshal_server_submit_at(h, d, 0.0);
+}
+
+int hip_marshal_server_status(hip_marshal_server *h, int64_t id) {
+  if (!h) return -1;
+  std::lock_guard<std::mutex> lk(h->s.mu);
+  auto it = h->s.reqs.find(id);
+  return it == h->s.reqs.end() ? -1 : it->second.state;
+}
+
+bool hip_marshal_server_wait(hip_marshal_server *h, int64_t id, double timeout_s) {
+  if (!h) return true;
+  Server &s = h->s;
+  std::unique_lock<std::mutex> lk(s.mu);
+  auto done = [&] {
+    auto it = s.reqs.find(id);
+    return it == s.reqs.end() || it->second.state == 2 || it->second.state == -1;
+  };
+  if (!s.cv.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) return true;
+  auto it = s.reqs.find(id);
+  return it == s.reqs.end() || it->second.state != 2;
+}
+
+bool hip_marshal_server_drain(hip_marshal_server *h, double timeout_s) {
+  if (!h) return true;
+  Server &s = h->s;
+  std::unique_lock<std::mutex> lk(s.mu);
+  auto idle = [&] {
+    if (!s.queue.empty()) return false;
+    for (Slot &sl : s.slots)
+      if (sl.busy) return false;
+    return true;
+  };
+  return !s.cv.wait_for(lk, std::chrono::duration<double>(timeout_s), idle);
+}
+
+bool hip_marshal_server_request_times(hip_marshal_server *h, int64_t id, double *arrival,
+                                      double *launch, double *done) {
+  if (!h) return true;
+  std::lock_guard<std::mutex> lk(h->s.mu);
+  auto it = h->s.reqs.find(id);
+  if (it == h->s.reqs.end()) return true;
+  if (arrival) *arrival = it->second.t_arrive;
+  if (launch) *launch = it->second.t_launch;
+  if (done) *done = it->second.t_done;
+  return false;
+}
+
+void hip_marshal_server_get_stats(hip_marshal_server *h, hip_marshal_server_stats *out) {
+  if (!h || !out) return;
+  Server &s = h->s;
+  std::lock_guard<std::mutex> lk(s.mu);
+  std::memset(out, 0, sizeof(*out));
+  out->completed = int64_t(s.lat.size());
+  out->failed = s.failed;
+  out->batches = s.batches;
+  out->max_batch = s.max_in_batch;
+  out->bytes = s.bytes_done;
+  out->span_s = s.t_first >= 0 && s.t_last > s.t_first ? s.t_last - s.t_first : 0;
+  if (!s.lat.empty()) {
+    std::vector<double> v = s.lat;
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double p) { return v[std::min(v.size() - 1, size_t(p * (v.size() - 1) + 0.5))]; };
+    double sum = 0;
+    for (double x : v) sum += x;
+    out->p50_ms = pct(0.50) * 1e3;
+    out->p99_ms = pct(0.99) * 1e3;
+    out->mean_ms = sum / double(v.size()) * 1e3;
+    out->max_ms = v.back() * 1e3;
+  }
+}
+
+void hip_marshal_server_reset_stats(hip_marshal_server *h) {
+  if (!h) return;
+  Server &s = h->s;
+  std::lock_guard<std::mutex> lk(s.mu);
+  s.lat.clear();
+  s.bytes_done = 0;
+  s.batches = s.failed = s.max_in_batch = 0;
+  s.t_first = -1;
+  s.t_last = 0;
+  for (auto it = s.reqs.begin(); it != s.reqs.end();) {
+    if (it->second.state == 2 || it->second.state == -1) it = s.reqs.erase(it);
+    else ++it;
+  }
+}
+
+void hip_marshal_server_destroy(hip_marshal_server *h) {
+  if (!h) return;
+  Server &s = h->s;
+  {
+    std::lock_guard<std::mutex> lk(s.mu);
+    s.stop = true;
+    s.cv.notify_all();
+  }
+  if (s.th.joinable()) s.th.join();
+  for (Slot &sl : s.slots) {
+    if (sl.st) hipStreamSynchronize(sl.st);
+    if (sl.flags) hipFree(sl.flags);
+    if (sl.params) hipFree(sl.params);
+    if (sl.host) hipHostFree(sl.host);
+    if (sl.ev) hipEventDestroy(sl.ev);
+    if (sl.st) hipStreamDestroy(sl.st);
+  }
+  delete h;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+"""Serving loop: continuous batching with admission by HBM budget.
+
+Each request's result must be bit-identical to the oracle and to the plain entry point run on
+it alone, whether requests arrive together, trickle in while others run, or queue behind a
+tight budget.
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def small_requests(orc, count, seed):
+    import libmarshal_amd as lm
+    from libmarshal_amd import workloads as wl
+    reqs = wl.request_stream(rate=2000.0, duration=count / 2000.0, seed=seed,
+                             min_bytes=4 << 10, max_bytes=2 << 20)
+    out = []
+    for i, (ta, op, es, rows, cols, tile) in enumerate(reqs[:count]):
+        x = orc.tagged(rows * cols, es, seed=seed * 7919 + i)
+        if op == lm.OP_TRANSPOSE:
+            want = orc.op_transpose(x, rows, cols, es=es)
+        elif op == lm.OP_AOS_ASTA:
+            want = orc.op_010(x, rows // tile, tile, cols, es=es)
+        else:
+            want = orc.op_010(x, rows // tile, cols, tile, es=es)
+        out.append((ta, op, es, rows, cols, tile, x, want))
+    return out
+
+
+def run_and_check(hip, reqs, server, realtime):
+    import torch
+    bufs = [torch.from_numpy(r[6].copy()).cuda() for r in reqs]
+    torch.cuda.synchronize()
+    t0 = server.now() + 0.02
+    ids = []
+    for (ta, op, es, rows, cols, tile, x, want), b in zip(reqs, bufs):
+        if realtime:
+            while server.now() < t0 + ta:
+                time.sleep(0.0001)
+        ids.append(server.submit(b, op, es, rows, cols, tile,
+                                 arrival=t0 + ta if realtime else 0.0))
+    assert server.drain(120.0) is False
+    for rid, b, r in zip(ids, bufs, reqs):
+        assert server.status(rid) == 2
+        assert np.array_equal(b.cpu().numpy(), r[7]), (rid, r[1:6])
+    return ids
+
+
+@pytest.mark.parametrize("realtime", [False, True])
+def test_server_results_match_alone(hip, orc, realtime):
+    reqs = small_requests(orc, 400, seed=1 if realtime else 2)
+    server = hip.Server(64 << 30, streams=4, max_batch=64)
+    run_and_check(hip, reqs, server, realtime)
+    st = server.stats()
+    say("stats", st)
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["batches"] >= 1 and st["p99_ms"] >= st["p50_ms"] > 0
+    server.close()
+    # alone: plain calls give the same bytes
+    import torch
+    for (ta, op, es, rows, cols, tile, x, want) in reqs[:60]:
+        d = torch.from_numpy(x.copy()).cuda()
+        if op == hip.OP_TRANSPOSE:
+            assert hip.transpose_full(d, rows, cols, elem_bytes=es) is False
+        elif op == hip.OP_AOS_ASTA:
+            assert hip.aos_asta(d, rows, cols, tile, elem_bytes=es) is False
+        else:
+            assert hip.asta_aos(d, rows, cols, tile, elem_bytes=es) is False
+        assert np.array_equal(d.cpu().numpy(), want)
+
+
+def test_server_tight_budget_queues_but_stays_exact(hip, orc):
+    """A budget of a few MB forces requests to wait for earlier ones to finish."""
+    reqs = small_requests(orc, 150, seed=3)
+    biggest = max(r[3] * r[4] * r[2] for r in reqs)
+    server = hip.Server(3 * biggest, streams=3, max_batch=8)
+    run_and_check(hip, reqs, server, realtime=False)
+    st = server.stats()
+    say("tight", st)
+    assert st["completed"] == len(reqs) and st["batches"] > len(reqs) // 8
+    server.close()
+
+
+def test_server_rejects_oversized_and_invalid(hip):
+    import torch
+    d = torch.zeros(1 << 20, dtype=torch.uint8, device="cuda")
+    server = hip.Server(1 << 16, streams=1, max_batch=4)
+    with pytest.raises(ValueError):
+        server.submit(d, hip.OP_TRANSPOSE, 4, 512, 512)  # 1 MiB > 64 KiB budget
+    with pytest.raises(ValueError):
+        server.submit(d, hip.OP_AOS_ASTA, 4, 100, 3, 64)  # 100 rows not a multiple of 64
+    server.close()
