@@ -75,6 +75,85 @@ def cpu_baseline(n_structs, w, t, seconds):
                       f"{reps} repetitions, oracle/oracle.c orc_par_010_bs on {cores} threads"}
 
 
+def extra_batch(lm, reps=3):
+    """Mixed batch: 4096 matrices in one graph launch vs per-matrix plain calls."""
+    import torch
+    from libmarshal_amd import workloads as wl
+    items = wl.batch_mix(4096)
+    bufs = [torch.empty(r * c * es, dtype=torch.uint8, device="cuda") for (_, es, r, c, _) in items]
+    total = sum(r * c * es for (_, es, r, c, _) in items)
+    batch = lm.Batch([(b, op, es, r, c, t) for b, (op, es, r, c, t) in zip(bufs, items)])
+    stream = torch.cuda.current_stream()
+    sb = time_calls(batch.run, reps, stream)
+
+    def plain():
+        for b, (op, es, r, c, t) in zip(bufs, items):
+            if op == wl.OP_TRANSPOSE:
+                e = lm.transpose_full(b, r, c, elem_bytes=es)
+            elif op == wl.OP_AOS_ASTA:
+                e = lm.aos_asta(b, r, c, t, elem_bytes=es)
+            else:
+                e = lm.soa_asta(b, r, c, t, elem_bytes=es)
+            if e:
+                return True
+        return False
+    sp = time_calls(plain, 1, stream)
+    out = {"matrices": len(items), "gbytes": round(total / 1e9, 2),
+           "kernels_per_launch": batch.info("kernels"),
+           "batched_GBs": round(2 * total / sb / 1e9, 1), "batched_ms": round(sb * 1e3, 2),
+           "plain_GBs": round(2 * total / sp / 1e9, 1), "plain_ms": round(sp * 1e3, 2),
+           "speedup": round(sp / sb, 2)}
+    batch.close()
+    del bufs
+    torch.cuda.empty_cache()
+    return out
+
+
+def extra_serve(lm, rate=2000.0, duration=1.0):
+    """Poisson request stream through the continuous-batching server vs sequential calls."""
+    import numpy as np
+    import torch
+    from libmarshal_amd import workloads as wl
+    trace = wl.request_stream(rate, duration, seed=7)
+    arena = torch.empty(64 << 30, dtype=torch.uint8, device="cuda")
+    server = lm.Server(256 << 30, streams=4, max_batch=256)
+    wl.serve_trace(lm, server, arena, trace[:100], realtime=False)  # grow per-stream buffers
+    server.drain()
+    server.reset_stats()
+    wl.serve_trace(lm, server, arena, trace, realtime=True)
+    server.drain(600)
+    st = server.stats()
+    server.close()
+    lat, nb, span = wl.plain_trace(lm, arena, trace, realtime=True)
+    lat = np.sort(np.array(lat))
+    out = {"requests": len(trace), "rate_per_s": rate,
+           "payload_gbytes": round(sum(wl.request_bytes(r) for r in trace) / 1e9, 1),
+           "served_p50_ms": round(st["p50_ms"], 2), "served_p99_ms": round(st["p99_ms"], 2),
+           "served_GBs": round(st["bytes"] / st["span_s"] / 1e9, 1),
+           "batches": st["batches"], "largest_batch": st["max_batch"],
+           "sequential_p50_ms": round(1e3 * lat[len(lat) // 2], 2),
+           "sequential_p99_ms": round(1e3 * lat[int(0.99 * (len(lat) - 1))], 2),
+           "sequential_GBs": round(nb / span / 1e9, 1)}
+    del arena
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_config1():
+    """The reference's 3-stage route restated on the host: 1800 x 7200 int32, a=18, b=72."""
+    import numpy as np
+    import oracle
+    M, N = 1800, 7200
+    x = oracle.tagged(M * N, 4, seed=3).view(np.uint32)
+    cores = oracle.num_cpus()
+    t0 = time.perf_counter()
+    oracle.par_transpose_3(x, M // 18, 18, N // 72, 72, nthreads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": round(2 * x.nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": cores,
+            "kind": "port", "sample": "1800 x 7200 int32, a=18 b=72, stages 100 -> 010 -> 0100 "
+                                      "(oracle/oracle.c orc_par_transpose_3)"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -147,7 +226,6 @@ def main():
         extra["soa_asta_GBs"] = round(2 * nbytes / s / 1e9, 1)
         if not torch.equal(d.view(torch.int32), ref.view(torch.int32)):
             raise RuntimeError("cycle-following round trips are not bit-exact")
-        del ref
         # Full 1800 x 7200 int32 transpose, a = 18, b = 72, 3-stage route (paper shape).
         M, N = 1800, 7200
         x = torch.randint(-2**31, 2**31 - 1, (M * N,), dtype=torch.int32, device="cuda")
@@ -156,6 +234,13 @@ def main():
         s = time_calls(lambda: lm.transpose_full(x, M, N), reps, stream)
         extra["transpose_1800x7200_i32_planner_GBs"] = round(2 * M * N * 4 / s / 1e9, 1)
         extra["planner_route"] = lm.last_route()
+        del x
+        torch.cuda.empty_cache()
+        extra["cpu_transpose_1800x7200_i32_3stage"] = cpu_config1()
+        extra["batch_4096_mixed"] = extra_batch(lm)
+        extra["serve_poisson_2000"] = extra_serve(lm)
+    del ref, d
+    torch.cuda.empty_cache()
 
     out = {
         "metric": METRIC,

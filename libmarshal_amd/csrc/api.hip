@@ -14,6 +14,8 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <map>
+#include <tuple>
 #include <vector>
 
 namespace lm {
@@ -414,7 +416,12 @@ bool route_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es, uin
     default:
       break;
   }
-  // Element-granular fallback: 100 over the whole matrix (src/cl_marshal.cc:769-775).
+  // Element-granular fallback: 100 over the whole matrix (src/cl_marshal.cc:769-775). When
+  // the rows and columns fit LDS, the row / column decomposition replaces it.
+  if (push_rc(out, buf, es, A * a, B * b, es)) {
+    route += "RC-";
+    return true;
+  }
   route += "Fallback-";
   push_0100(out, buf, es, 1, A * a, B * b, 1);
   return true;
@@ -476,11 +483,14 @@ static bool stage_010(unsigned char *buf, size_t es, uint64_t A, uint64_t a, uin
 
 // ---------------------------------------------------------------------------------------
 // Planner: tile factors and route for an M x N transpose.
-// Each stage is priced as bytes / modelled bandwidth: LDS-staged 010 at the measured
-// ~5.5 TB/s (less for 1- and 2-byte elements), cycle following over super-elements of S
-// bytes at a bandwidth that grows with S (short moves waste most of each line) and with the
-// number of independent start points. Candidates: routes 2, 22, 3, 32 over all divisor
-// pairs, and the element-granular fallback.
+// Every candidate route is expanded into the stage list it would run (the same records a
+// call executes), and each stage is priced as 2 x bytes / the bandwidth its kernel form
+// reaches on MI355X (measured): LDS-staged 010 ~5.5 TB/s; cycle following over S-byte
+// super-elements is bound by device-scope atomic claims (one per move, ~5*10^9/s), i.e. by
+// ~10 GB/s per byte of S, up to ~3 TB/s; LDS-bit forms ~0.7-1.5 TB/s; the row pass of the
+// row/column decomposition ~3 TB/s and each of its column passes ~0.55 TB/s.
+// Candidates: routes 2, 22, 3, 32 over divisor pairs, the row/column decomposition and the
+// element-granular fallback. Choices are cached per (M, N, element size).
 // ---------------------------------------------------------------------------------------
 namespace {
 std::vector<uint64_t> divisors(uint64_t n) {
@@ -494,57 +504,97 @@ std::vector<uint64_t> divisors(uint64_t n) {
   return lo;
 }
 
-// Seconds per byte of matrix for each stage form (relative units).
-double t_bs(size_t es) { return 1.0 / (es >= 4 ? 5.5 : es == 2 ? 4.5 : 2.8); }
-double t_pt(uint64_t S) {
-  double bw = 0.25 * std::pow(double(S), 0.45);  // ~0.5 TB/s at 4 B, ~3 TB/s at 256 B
-  return 1.0 / std::min(bw, 3.0);
+// At most `cap` divisors, spread over the whole range (always keeping 1 and n).
+std::vector<uint64_t> some_divisors(uint64_t n, size_t cap) {
+  std::vector<uint64_t> d = divisors(n);
+  if (d.size() <= cap) return d;
+  std::vector<uint64_t> out;
+  for (size_t i = 0; i < cap; i++) out.push_back(d[i * (d.size() - 1) / (cap - 1)]);
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+  return out;
 }
-double t_rc_row(size_t es) { return 1.0 / (es >= 4 ? 4.0 : es == 2 ? 3.0 : 1.5); }
-double t_rc_col(size_t es) { return 1.0 / (es >= 4 ? 2.5 : es == 2 ? 2.0 : 1.0); }
-double t_010(size_t es, uint64_t A, uint64_t a, uint64_t B) {
-  if (a <= 1 || B <= 1) return 0.0;
-  Bs010Plan pl{};
-  if (plan_010_bs(es, A, a, B, lds_budget(), 0, 16, &pl)) return t_bs(es);
-  if (a * B < (1ull << 31) && pt010_flag_words(a, B, 1, 1) * 4 <= lds_budget())
-    return 2.0 * t_pt(es);
-  return 1.5 * t_pt(es);
+
+double stage_bw_tbs(const StageRec &r) {
+  size_t es = r.es;
+  switch (r.kind) {
+    case StageKind::BS010: return es >= 4 ? 5.5 : es == 2 ? 4.5 : 2.8;
+    case StageKind::PT010: return 1.5;
+    case StageKind::PT0100L: return std::min(3.0, 0.01 * double(r.d3 * es));
+    case StageKind::PT100G: {
+      double S = double(r.d3 * es);
+      return std::min(3.0, 0.12 + 0.011 * S);
+    }
+    case StageKind::RC_ROW: return es >= 4 ? 3.0 : es == 2 ? 2.0 : 1.2;
+    case StageKind::RC_COL1:
+    case StageKind::RC_COL2: return es >= 4 ? 0.55 : es == 2 ? 0.4 : 0.3;
+  }
+  return 0.1;
 }
+
+double stage_bytes(const StageRec &r) {
+  switch (r.kind) {
+    case StageKind::BS010:
+    case StageKind::PT010: return double(r.d0) * double(r.d1) * double(r.d2) * double(r.es);
+    case StageKind::PT100G:
+    case StageKind::PT0100L:
+      return double(r.d0) * double(r.d1) * double(r.d2) * double(r.d3) * double(r.es);
+    default: return double(r.d1) * double(r.d2) * double(r.es);
+  }
+}
+
+// Modelled time (relative units) of a stage list.
+double route_cost(const std::vector<StageRec> &v) {
+  double t = 0;
+  for (const StageRec &r : v) t += stage_bytes(r) / stage_bw_tbs(r);
+  return t;
+}
+
+std::mutex g_plan_mu;
+std::map<std::tuple<uint64_t, uint64_t, size_t, size_t>, Choice> g_plan_cache;
 }  // namespace
 
 Choice plan_full(size_t es, uint64_t M, uint64_t N) {
   Choice best;
-  best.cost = t_pt(es);  // fallback
   best.a = M;
   best.b = N;
   best.stages = -1;
   if (M <= 1 || N <= 1) return best;
-  RcPlan rp{};
-  if (plan_rc(es, M, N, es, lds_budget(), &rp)) {  // row / column decomposition
-    double c = t_rc_row(es) + t_rc_col(es) * (rp.c > 1 ? 2.0 : 1.0);
-    if (c < best.cost) best = {c, M, N, 5};
+  auto key = std::make_tuple(M, N, es, lds_budget());
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plan_cache.find(key);
+    if (it != g_plan_cache.end()) return it->second;
   }
-  std::vector<uint64_t> dm = divisors(M), dn = divisors(N);
+  std::string route;
+  auto consider = [&](uint64_t A, uint64_t a, uint64_t B, uint64_t b, int stages) {
+    std::vector<StageRec> v;
+    if (!route_stages(v, nullptr, es, A, a, B, b, 1, stages, route)) return;
+    if (route.find("Fallback") != std::string::npos && stages != -1) return;
+    if (route.find("RC-") == std::string::npos && stages == 5) return;
+    double c = route_cost(v);
+    if (c < best.cost) {
+      best.cost = c;
+      best.stages = stages;
+      best.a = (stages == 22) ? M : a;
+      best.b = (stages == 2) ? N : b;
+    }
+  };
+  consider(1, M, 1, N, -1);  // element-granular fallback
+  consider(1, M, 1, N, 5);   // row / column decomposition
+  std::vector<uint64_t> dm = some_divisors(M, 24), dn = some_divisors(N, 24);
   for (uint64_t a : dm) {
-    uint64_t A = M / a;
-    if (a > 1 && a < M) {  // route 2: 010(A, a, N) + 100(A, N, a)
-      double c = t_010(es, A, a, N) + t_pt(a * es);
-      if (c < best.cost) best = {c, a, N, 2};
-    }
+    if (a > 1 && a < M) consider(M / a, a, 1, N, 2);
     for (uint64_t b : dn) {
-      uint64_t B = N / b;
-      if (a == 1 && b > 1 && b < N) {  // route 22 with a = M: 100(M, B, b) + 010(B, M, b)
-        double c = t_pt(b * es) + t_010(es, B, M, b);
-        if (c < best.cost) best = {c, M, b, 22};
-      }
+      if (a == 1 && b > 1 && b < N) consider(1, M, N / b, b, 22);
       if (a > 1 && b > 1 && a < M && b < N) {
-        double c3 = t_pt(b * es) + t_010(es, B * A, a, b) + t_pt(a * es);
-        if (c3 < best.cost) best = {c3, a, b, 3};
-        double c32 = t_pt(b * es) + t_010(es, A * B, a, b) + t_pt(a * es);
-        if (c32 < best.cost) best = {c32, a, b, 32};
+        consider(M / a, a, N / b, b, 3);
+        consider(M / a, a, N / b, b, 32);
       }
     }
   }
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  if (g_plan_cache.size() > 65536) g_plan_cache.clear();
+  g_plan_cache[key] = best;
   return best;
 }
 
