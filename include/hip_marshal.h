@@ -221,14 +221,19 @@ size_t hip_marshal_get_lds_budget(void);
 /* 1 = time calls with events and synchronize (default), 0 = asynchronous. */
 void hip_marshal_set_profiling(int on);
 /* Launch-shape knobs: "bs_upt" (4, 8, 16 units per lane per round of the LDS-staged 010),
- * "bs_blocks_per_cu", "pt010_blocks_per_cu", "pt100_blocks_per_cu" (grid caps). Replaces the
- * reference's hard-coded work-group tables (src/cl_marshal.cc:165-231, 388-465). The setter
- * returns true on an unknown key or a value outside [1, 2^30]; the getter returns 0 for an
- * unknown key. */
+ * "bs_blocks_per_cu", "pt010_blocks_per_cu", "pt100_blocks_per_cu", "rc_row_blocks_per_cu",
+ * "rc_col_blocks_per_cu" (grid caps), "pt100_vw_max" (lanes per cycle-following walker),
+ * "lds0100_min_inst", "rc_max_super_bytes" (route thresholds), and for the stripe route
+ * "st_super_bytes" (target super-element bytes), "st_scratch_mb" (scratch cap),
+ * "st_tile_kb" (LDS tile of the out-of-place transpose), "st_pt_vw" and "st_rotate" (0/1).
+ * Replaces the reference's hard-coded work-group tables (src/cl_marshal.cc:165-231,
+ * 388-465). The setter returns true on an unknown key or a value outside [min, 2^30] (min is
+ * 0 for "st_rotate", 1 otherwise); the getter returns 0 for an unknown key. */
 bool hip_marshal_set_tuning(const char *key, int64_t value);
 int64_t hip_marshal_get_tuning(const char *key);
 /* The planner's choice for an M x N transpose of elem_bytes elements: tile factors a | M,
- * b | N and the stage sequence (2, 22, 3, 32; -1 = element-granular fallback). Host-only
+ * b | N and the stage sequence (2, 22, 3, 32; 5 = row/column shuffles, 6/7 = stripes of
+ * rows/columns through bounded scratch; -1 = element-granular fallback). Host-only
  * (no device work). Replaces the tile heuristic of the reference's test
  * (test/cl_unittest.cc:396-495, "Alg. 5" and "Alg. 3"). */
 bool hip_marshal_plan_transpose(int64_t M, int64_t N, size_t elem_bytes, int64_t *a,

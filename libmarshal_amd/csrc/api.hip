@@ -67,6 +67,7 @@ namespace {
 struct Knob {
   const char *key;
   std::atomic<int64_t> value;
+  int64_t min = 1;  // smallest accepted value (0 for on/off switches)
 };
 Knob g_knobs[] = {
     {"bs_upt", {16}},
@@ -78,6 +79,11 @@ Knob g_knobs[] = {
     {"rc_row_blocks_per_cu", {2}},
     {"rc_max_super_bytes", {64}},
     {"rc_col_blocks_per_cu", {1}},
+    {"st_scratch_mb", {256}},
+    {"st_super_bytes", {2048}},
+    {"st_tile_kb", {32}},
+    {"st_pt_vw", {8}},
+    {"st_rotate", {1}, 0},
 };
 Knob *find_knob(const char *key) {
   if (!key) return nullptr;
@@ -361,6 +367,23 @@ bool push_rc(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t
   return true;
 }
 
+bool push_st(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t M,
+             uint64_t N, int variant) {
+  if (M <= 1 || N <= 1) return true;
+  StageRec r{};
+  if (!plan_st(es, M, N, variant, uint64_t(tuning("st_super_bytes")),
+               uint64_t(tuning("st_scratch_mb")) << 20, tuning("st_pt_vw"),
+               tuning("st_rotate") != 0, &r.stp))
+    return false;
+  r.kind = StageKind::ST;
+  r.es = es;
+  r.buf = buf;
+  r.d1 = M;
+  r.d2 = N;
+  out.push_back(r);
+  return true;
+}
+
 // Composer: the stage sequences of cl_transpose (src/cl_marshal.cc:561-776).
 bool route_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t A,
                   uint64_t a, uint64_t B, uint64_t b, int R, int stages, std::string &route) {
@@ -403,6 +426,13 @@ bool route_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es, uin
     case 5:  // row / column decomposition (no counterpart in the reference)
       if (push_rc(out, buf, es, A * a, B * b, es)) {
         route += "RC-";
+        return true;
+      }
+      break;
+    case 6:  // stripes of rows / of columns through bounded scratch (no counterpart)
+    case 7:
+      if (push_st(out, buf, es, A * a, B * b, stages - 6)) {
+        route += stages == 6 ? "ST_R-" : "ST_C-";
         return true;
       }
       break;
@@ -451,6 +481,14 @@ bool run_stages(const std::vector<StageRec> &stages, hipStream_t st) {
                                grid_cap(uint32_t(tuning(pass ? "rc_col_blocks_per_cu"
                                                              : "rc_row_blocks_per_cu"))),
                                st));
+        break;
+      }
+      case StageKind::ST: {
+        Lease lease(pool_here(), st);
+        unsigned char *scr = reinterpret_cast<unsigned char *>(lease.get(r.stp.scratch_bytes));
+        if (!scr) return true;
+        LM_CHECK_HIP(run_st(r.buf, r.stp, scr, size_t(tuning("st_tile_kb")) << 10,
+                            grid_cap(uint32_t(tuning("pt100_blocks_per_cu"))), st));
         break;
       }
       case StageKind::PT100G: {
@@ -527,6 +565,17 @@ double stage_bw_tbs(const StageRec &r) {
     case StageKind::RC_ROW: return es >= 4 ? 3.0 : es == 2 ? 2.0 : 1.2;
     case StageKind::RC_COL1:
     case StageKind::RC_COL2: return es >= 4 ? 0.55 : es == 2 ? 0.4 : 0.3;
+    case StageKind::ST: {  // copy + out-of-place transpose + 100 (+ padding and a 2-D copy)
+      const StPlan &p = r.stp;
+      StageRec pt{};
+      pt.kind = StageKind::PT100G;
+      pt.es = es;
+      pt.d3 = p.h;
+      double t = 1.0 / 5.5 + 1.0 / (es >= 4 ? 4.5 : es == 2 ? 3.5 : 2.5);
+      if (p.q > 1) t += 1.0 / stage_bw_tbs(pt);
+      if (p.r) t += 1.0 / (es >= 4 ? 3.0 : es == 2 ? 2.0 : 1.0);
+      return 1.0 / t;
+    }
   }
   return 0.1;
 }
@@ -538,7 +587,7 @@ double stage_bytes(const StageRec &r) {
     case StageKind::PT100G:
     case StageKind::PT0100L:
       return double(r.d0) * double(r.d1) * double(r.d2) * double(r.d3) * double(r.es);
-    default: return double(r.d1) * double(r.d2) * double(r.es);
+    default: return double(r.d1) * double(r.d2) * double(r.es);  // RC_*, ST
   }
 }
 
@@ -550,16 +599,16 @@ double route_cost(const std::vector<StageRec> &v) {
 }
 
 std::mutex g_plan_mu;
-std::map<std::tuple<uint64_t, uint64_t, size_t, size_t>, Choice> g_plan_cache;
+std::map<std::tuple<uint64_t, uint64_t, size_t, size_t, bool>, Choice> g_plan_cache;
 }  // namespace
 
-Choice plan_full(size_t es, uint64_t M, uint64_t N) {
+Choice plan_full(size_t es, uint64_t M, uint64_t N, bool allow_st) {
   Choice best;
   best.a = M;
   best.b = N;
   best.stages = -1;
   if (M <= 1 || N <= 1) return best;
-  auto key = std::make_tuple(M, N, es, lds_budget());
+  auto key = std::make_tuple(M, N, es, lds_budget(), allow_st);
   {
     std::lock_guard<std::mutex> lk(g_plan_mu);
     auto it = g_plan_cache.find(key);
@@ -571,6 +620,7 @@ Choice plan_full(size_t es, uint64_t M, uint64_t N) {
     if (!route_stages(v, nullptr, es, A, a, B, b, 1, stages, route)) return;
     if (route.find("Fallback") != std::string::npos && stages != -1) return;
     if (route.find("RC-") == std::string::npos && stages == 5) return;
+    if (route.find("ST_") == std::string::npos && (stages == 6 || stages == 7)) return;
     double c = route_cost(v);
     if (c < best.cost) {
       best.cost = c;
@@ -581,6 +631,10 @@ Choice plan_full(size_t es, uint64_t M, uint64_t N) {
   };
   consider(1, M, 1, N, -1);  // element-granular fallback
   consider(1, M, 1, N, 5);   // row / column decomposition
+  if (allow_st) {            // stripes through bounded scratch
+    consider(1, M, 1, N, 6);
+    consider(1, M, 1, N, 7);
+  }
   std::vector<uint64_t> dm = some_divisors(M, 24), dn = some_divisors(N, 24);
   for (uint64_t a : dm) {
     if (a > 1 && a < M) consider(M / a, a, 1, N, 2);
@@ -600,13 +654,14 @@ Choice plan_full(size_t es, uint64_t M, uint64_t N) {
 
 // Stage list of an M x N transpose as chosen by the planner (stages = 0).
 bool plan_transpose_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es,
-                           uint64_t M, uint64_t N, std::string &route) {
-  Choice c = plan_full(es, M, N);
+                           uint64_t M, uint64_t N, std::string &route, bool allow_st) {
+  Choice c = plan_full(es, M, N, allow_st);
   if (c.stages == 2) return route_stages(out, buf, es, M / c.a, c.a, 1, N, 1, 2, route);
   if (c.stages == 22) return route_stages(out, buf, es, 1, M, N / c.b, c.b, 1, 22, route);
   if (c.stages == 3 || c.stages == 32)
     return route_stages(out, buf, es, M / c.a, c.a, N / c.b, c.b, 1, c.stages, route);
-  if (c.stages == 5) return route_stages(out, buf, es, 1, M, 1, N, 1, 5, route);
+  if (c.stages >= 5 && c.stages <= 7)
+    return route_stages(out, buf, es, 1, M, 1, N, 1, c.stages, route);
   return route_stages(out, buf, es, 1, M, 1, N, 1, -1, route);
 }
 
@@ -635,7 +690,7 @@ bool hip_marshal_plan_transpose(int64_t M, int64_t N, size_t es, int64_t *a, int
 
 bool hip_marshal_set_tuning(const char *key, int64_t value) {
   Knob *k = find_knob(key);
-  if (!k || value < 1 || value > (int64_t(1) << 30)) {
+  if (!k || value < k->min || value > (int64_t(1) << 30)) {
     fprintf(stderr, "[libmarshal] hip_marshal_set_tuning: bad key or value\n");
     return true;
   }
@@ -816,7 +871,7 @@ static bool pad_common(const char *who, hipStream_t st, void *src, size_t es, in
   }
   Timer tm(st, unpad ? "Unpadding" : "Padding", 2.0 * double(x) * y * es, ns);
   Lease lease(pool_here(), st);
-  uint32_t *flags = lease.get(pad_flag_words(n) * 4);
+  uint32_t *flags = lease.get(pad_flag_words(n, es) * 4);
   if (!flags) return tm.finish(true);
   hipError_t e = launch_padding(static_cast<unsigned char *>(src), es, x, y, pad, unpad, flags,
                                 st);

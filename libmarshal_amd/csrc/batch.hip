@@ -360,7 +360,7 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
     std::string route;
     switch (d.op) {
       case HIP_MARSHAL_OP_TRANSPOSE:
-        ok = plan_transpose_stages(per[i], buf, es, h, w, route);
+        ok = plan_transpose_stages(per[i], buf, es, h, w, route, false);
         break;
       case HIP_MARSHAL_OP_AOS_ASTA:  // 010(h/t, t, w)
         ok = push_010(per[i], buf, es, h / t, t, w, M010::Auto, 1, 1, nullptr);
@@ -458,6 +458,9 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
           add_item(g, make_pt100(r, nullptr), r.d0);
           break;
         }
+        case StageKind::ST:  // never planned for a batch (allow_st = false)
+          fprintf(stderr, "[libmarshal] %s: stripe route in a batch\n", who);
+          return false;
       }
     }
     std::vector<Group> lvl;

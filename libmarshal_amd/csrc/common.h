@@ -92,4 +92,52 @@ __device__ __forceinline__ uint32_t flag_or(uint32_t *p, uint32_t mask) {
   return __hip_atomic_fetch_or(p, mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// 16 bytes starting at byte offset s of `base` (any alignment): the one or two aligned 16-byte
+// chunks covering them are loaded and shifted into place with v_alignbyte. When s is not a
+// multiple of 16 the load covers bytes up to (s & ~15) + 32, which the caller keeps in bounds.
+// Split form: window_raw issues the loads, window_join shifts (call it after other loads have
+// been issued so their latencies overlap).
+__device__ __forceinline__ void window_raw(const unsigned char *base, uint64_t s, U128 &lo,
+                                           U128 &hi) {
+  const uint64_t a = s & ~uint64_t(15);
+  lo = *reinterpret_cast<const U128 *>(base + a);
+  if (s & 15u) hi = *reinterpret_cast<const U128 *>(base + a + 16);
+}
+__device__ __forceinline__ U128 window_join(uint64_t s, const U128 &lo, const U128 &hi) {
+  const uint32_t o = uint32_t(s & 15u);
+  if (o == 0) return lo;
+  const uint32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+  const uint32_t q = o >> 2, b = o & 3u;
+  uint32_t v[5];
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+    v[i] = q == 0 ? w[i] : q == 1 ? w[i + 1] : q == 2 ? w[i + 2] : w[i + 3];
+  U128 r;
+  r.x = __builtin_amdgcn_alignbyte(v[1], v[0], b);
+  r.y = __builtin_amdgcn_alignbyte(v[2], v[1], b);
+  r.z = __builtin_amdgcn_alignbyte(v[3], v[2], b);
+  r.w = __builtin_amdgcn_alignbyte(v[4], v[3], b);
+  return r;
+}
+
+__device__ __forceinline__ U128 load_window16(const unsigned char *base, uint64_t s) {
+  const uint64_t a = s & ~uint64_t(15);
+  const uint32_t o = uint32_t(s & 15u);
+  const U128 lo = *reinterpret_cast<const U128 *>(base + a);
+  if (o == 0) return lo;
+  const U128 hi = *reinterpret_cast<const U128 *>(base + a + 16);
+  const uint32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+  const uint32_t q = o >> 2, b = o & 3u;
+  uint32_t v[5];
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+    v[i] = q == 0 ? w[i] : q == 1 ? w[i + 1] : q == 2 ? w[i + 2] : w[i + 3];
+  U128 r;
+  r.x = __builtin_amdgcn_alignbyte(v[1], v[0], b);
+  r.y = __builtin_amdgcn_alignbyte(v[2], v[1], b);
+  r.z = __builtin_amdgcn_alignbyte(v[3], v[2], b);
+  r.w = __builtin_amdgcn_alignbyte(v[4], v[3], b);
+  return r;
+}
+
 }  // namespace lm

@@ -56,6 +56,7 @@ struct Pt100Params {
   uint64_t sub_bytes;     // bytes per piece (multiple of the unit; the last may be shorter)
   uint64_t walkers;       // instances * RC * nsub
   uint32_t vw_log2;       // lanes per walker = 1 << vw_log2 (<= 64)
+  uint32_t rot_r, rot_c;  // rotations folded into the map (0 for a plain 100; see next_slot)
   FastDiv64 div_C64, div_RC, div_nsub;
   FastDiv32 div_C32;
 };
@@ -67,10 +68,15 @@ struct Pt100Plan {
   uint64_t nsub, sub_bytes;
   uint64_t flag_words;    // 32-bit words of visited bits
 };
-void plan_100(size_t es, uint64_t inst, uint64_t R, uint64_t C, uint64_t b, Pt100Plan *out);
+// vw_max > 0 overrides the "pt100_vw_max" knob (lanes per walker; pieces of up to 8 units of
+// 16 bytes per lane).
+void plan_100(size_t es, uint64_t inst, uint64_t R, uint64_t C, uint64_t b, Pt100Plan *out,
+              int64_t vw_max = 0);
+// rot_r / rot_c (< R / < C): the slot in row i, column j moves to row (j + rot_c) mod C,
+// column (i + rot_r) mod R of the transposed grid (a plain 100 has both 0).
 hipError_t launch_100_pttwac(unsigned char *base, uint32_t *flags, uint64_t inst, uint64_t R,
                              uint64_t C, uint64_t S, const Pt100Plan &pl, uint32_t max_blocks,
-                             hipStream_t st);
+                             hipStream_t st, uint32_t rot_r = 0, uint32_t rot_c = 0);
 
 // 0100 with the visited bits of one instance in LDS (one workgroup per instance).
 size_t lds0100_bytes(uint64_t R, uint64_t C, const Pt100Plan &pl);
@@ -103,10 +109,43 @@ hipError_t launch_rc(unsigned char *base, uint64_t M, uint64_t N, const RcPlan &
 
 // ---- utilities --------------------------------------------------------------------------
 hipError_t launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st);
+// Byte copy (16-byte vectors when both ends and the size allow) and a pitched 2-D copy.
+hipError_t launch_copy(unsigned char *dst, const unsigned char *src, uint64_t bytes,
+                       hipStream_t st);
+hipError_t launch_copy2d(unsigned char *dst, uint64_t dpitch, const unsigned char *src,
+                         uint64_t spitch, uint64_t width, uint64_t rows, hipStream_t st);
+// Out-of-place transpose of `blocks` R x C matrices (src and dst must not overlap), tiles of
+// at most `tile_bytes` staged in LDS. src_end (0: just the blocks) is how many bytes from
+// src may be read: 16-byte window loads of misaligned rows read up to 31 bytes past a row.
+hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, size_t es,
+                                uint64_t blocks, uint64_t R, uint64_t C, uint64_t src_stride,
+                                uint64_t dst_stride, size_t tile_bytes, hipStream_t st,
+                                uint64_t src_end = 0);
+
+// ---- stripe route (full transpose through bounded scratch) -------------------------------
+struct StPlan {
+  int variant;             // 0: stripes of h rows (M = q*h + r); 1: of h columns (N = q*h + r)
+  size_t es;
+  uint64_t M, N, h, q, r;
+  uint64_t group;          // stripes per scratch fill
+  bool rotate;             // stripes written one slot down, rotation folded into the 100
+  Pt100Plan pt;            // the 100 over h-element super-elements (when q > 1)
+  size_t stripe_off, b_off, f_off, scratch_bytes;
+  uint64_t flag_words;
+};
+// False when no stripe height keeps one stripe within `scratch_cap` bytes or the striped
+// dimension holds fewer than two stripes.
+// rotate: each stripe is transposed into its neighbour's slot (no copy pass) and the 100
+// folds the one-slot rotation into its map; otherwise stripes go through scratch in groups.
+bool plan_st(size_t es, uint64_t M, uint64_t N, int variant, uint64_t target_bytes,
+             uint64_t scratch_cap, int64_t pt_vw, bool rotate, StPlan *out);
+// `scratch` must hold plan.scratch_bytes and not overlap `buf`.
+hipError_t run_st(unsigned char *buf, const StPlan &p, unsigned char *scratch, size_t tile_bytes,
+                  uint32_t pt_blocks, hipStream_t st);
 
 // ---- padding / unpadding ----------------------------------------------------------------
 // flags must hold pad_flag_words(...) words; they are initialised by the launcher.
-uint64_t pad_flag_words(uint64_t total_elems);
+uint64_t pad_flag_words(uint64_t total_elems, size_t es);
 hipError_t launch_padding(unsigned char *base, size_t es, uint64_t x, uint64_t y, uint64_t pad,
                           bool unpad, uint32_t *flags, hipStream_t st);
 
@@ -116,7 +155,7 @@ int64_t tuning(const char *key);
 
 // ---- stage records (a route as a list of elementary stages) -----------------------------
 enum class M010 { Auto, BS, PTTWAC, Global };
-enum class StageKind : int { BS010, PT010, PT100G, PT0100L, RC_ROW, RC_COL1, RC_COL2 };
+enum class StageKind : int { BS010, PT010, PT100G, PT0100L, RC_ROW, RC_COL1, RC_COL2, ST };
 struct StageRec {
   StageKind kind;
   size_t es;
@@ -126,11 +165,12 @@ struct StageRec {
   Bs010Plan bs;
   Pt100Plan pt;
   RcPlan rc;                // RC_*: (M, N) in d1, d2 and the element bytes in d3
+  StPlan stp;               // ST: the whole stripe route (M, N in d1, d2)
 };
 struct Choice {
   double cost = 1e300;
   uint64_t a = 1, b = 1;
-  int stages = 0;  // 2, 22, 3, 32, or -1 for the element-granular fallback
+  int stages = 0;  // 2, 22, 3, 32, 5 (rows/columns), 6/7 (stripes), -1 (element-granular)
 };
 size_t current_lds_budget();
 M010 choose_010(size_t es, uint64_t A, uint64_t a, uint64_t B, int R, int P,
@@ -141,9 +181,11 @@ bool push_010(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_
               uint64_t a, uint64_t B, M010 m, int R, int P, const char **tag);
 bool route_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t A,
                   uint64_t a, uint64_t B, uint64_t b, int R, int stages, std::string &route);
+// allow_st = false leaves out the stripe route (it needs scratch beyond the visited bits and
+// has no group-kernel form for batches).
 bool plan_transpose_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es,
-                           uint64_t M, uint64_t N, std::string &route);
-Choice plan_full(size_t es, uint64_t M, uint64_t N);
+                           uint64_t M, uint64_t N, std::string &route, bool allow_st = true);
+Choice plan_full(size_t es, uint64_t M, uint64_t N, bool allow_st = true);
 bool run_stages(const std::vector<StageRec> &stages, hipStream_t st);
 // Appends the row / column decomposition of an M x N transpose of S-byte elements; false
 // when it does not fit the LDS budget.

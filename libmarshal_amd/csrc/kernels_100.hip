@@ -77,14 +77,15 @@ static uint64_t next_pow2(uint64_t v) {
   return r;
 }
 
-void plan_100(size_t es, uint64_t inst, uint64_t R, uint64_t C, uint64_t b, Pt100Plan *out) {
+void plan_100(size_t es, uint64_t inst, uint64_t R, uint64_t C, uint64_t b, Pt100Plan *out,
+              int64_t vw_max) {
   uint64_t S = b * es;
   int ub = (S % 16 == 0) ? 16 : (S % 8 == 0) ? 8 : (S % 4 == 0) ? 4 : (S % 2 == 0) ? 2 : 1;
   uint64_t nU = S / ub;
   uint64_t nsub = 1, per = nU;
   int upl = 1;
   uint32_t vwl = 0;
-  uint64_t vmax = uint64_t(tuning("pt100_vw_max"));
+  uint64_t vmax = uint64_t(vw_max > 0 ? vw_max : tuning("pt100_vw_max"));
   if (vmax < 1 || vmax > 64 || (vmax & (vmax - 1))) vmax = 64;
   uint32_t vmaxl = 0;
   while ((1ull << vmaxl) < vmax) vmaxl++;
@@ -208,8 +209,10 @@ hipError_t launch_0100_lds(unsigned char *base, uint64_t inst, uint64_t R, uint6
 
 hipError_t launch_100_pttwac(unsigned char *base, uint32_t *flags, uint64_t inst, uint64_t R,
                              uint64_t C, uint64_t S, const Pt100Plan &pl, uint32_t max_blocks,
-                             hipStream_t st) {
+                             hipStream_t st, uint32_t rot_r, uint32_t rot_c) {
   Pt100Params p{};
+  p.rot_r = uint32_t(rot_r % (R ? R : 1));
+  p.rot_c = uint32_t(rot_c % (C ? C : 1));
   p.base = base;
   p.flags = flags;
   p.R = R;

@@ -34,16 +34,36 @@ __device__ __forceinline__ void unit_put(typename UnitOf<ES, G>::T &v, int k,
 }
 
 
-// Destination of super-element slot s in an R x C grid: (s mod C)*R + floor(s/C).
+// Destination of super-element slot s in an R x C grid: (s mod C)*R + floor(s/C), with the
+// optional rotations: ((s mod C + rot_c) mod C)*R + (floor(s/C) + rot_r) mod R. Any such map
+// is a permutation of the slots, so cycle following applies unchanged.
 template <bool I64>
 __device__ __forceinline__ uint64_t next_slot(uint64_t s, const Pt100Params &p) {
   if constexpr (I64) {
     uint64_t q = p.div_C64.div(s);
-    return (s - q * p.C) * p.R + q;
+    uint64_t col = s - q * p.C;
+    if (p.rot_r) {
+      q += p.rot_r;
+      if (q >= p.R) q -= p.R;
+    }
+    if (p.rot_c) {
+      col += p.rot_c;
+      if (col >= p.C) col -= p.C;
+    }
+    return col * p.R + q;
   } else {
     uint32_t s32 = uint32_t(s);
     uint32_t q = p.div_C32.div(s32);
-    return uint64_t((s32 - q * uint32_t(p.C)) * uint32_t(p.R) + q);
+    uint32_t col = s32 - q * uint32_t(p.C);
+    if (p.rot_r) {
+      q += p.rot_r;
+      if (q >= uint32_t(p.R)) q -= uint32_t(p.R);
+    }
+    if (p.rot_c) {
+      col += p.rot_c;
+      if (col >= uint32_t(p.C)) col -= uint32_t(p.C);
+    }
+    return uint64_t(col * uint32_t(p.R) + q);
   }
 }
 

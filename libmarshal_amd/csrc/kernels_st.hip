@@ -1,0 +1,436 @@
+// kernels_st.hip - stripe route: full M x N transposes whose rows and columns are too long
+// for LDS, with a bounded scratch buffer.
+//
+// Row-stripe form (M = q*h + r): the top q*h rows are q stripes of h x N. Each stripe is
+// copied to scratch and transposed out of place back into its own place ([q][h][N] ->
+// [q][N][h]); one cycle-following 100 over the q x N grid of h-element super-elements then
+// gives [N][q][h] = the transpose of the top part. When r > 0 the bottom r x N rows are
+// first transposed into scratch, the top's N rows of q*h elements are padded in place to M,
+// and the saved N x r block fills the pad columns.
+//
+// Column-stripe form (N = q*h + r): the mirror image. When r > 0 the last r columns are
+// saved to scratch and the rows unpadded from N to q*h; 100 over the M x q grid of h-element
+// super-elements gives [q][M][h]; each M x h block is transposed out of place through
+// scratch into h x M; the saved M x r block, transposed, fills the last r rows.
+//
+// Every element is moved a bounded number of times by streaming kernels (copy, tiled
+// out-of-place transpose, padding) and once by cycle following over large super-elements,
+// so prime or otherwise awkward dimensions need no element-granular cycle following. The
+// scratch buffer holds a group of stripes (a few hundred MB), not a second copy of the
+// matrix. Not a reference algorithm: the reference transposes such matrices only by
+// element-granular cycle following (src/cl_marshal.cc:769-775); its padding kernels
+// (src/cl/cl_aos_asta.cl:813-938) are the padding/unpadding used here.
+#include "kernels.h"
+
+#include <algorithm>
+
+namespace lm {
+
+// ---- tiled out-of-place transpose of `blocks` R x C matrices ----------------------------
+// A tile of TR x TC source elements is staged transposed in LDS: row c of the LDS tile holds
+// source column c (TR elements), so a destination row segment is one contiguous LDS row and
+// is read back with 16-byte LDS reads. LDS rows have a pitch that is a multiple of 128 bytes
+// and byte offsets inside a row are XOR-swizzled with a per-row mask taken from c >> SW: the
+// 16-byte chunk index with bits 0-2, the 4-byte word inside the chunk with bits 3-4 (the
+// 8-byte half for 8-byte elements). SW is 0 when lanes load single elements (consecutive c)
+// and log2(16 / ES) when each lane loads 16 bytes (c steps by 16 / ES): either way the lanes
+// of one LDS write hit distinct banks, and 16-byte reads along a row stay conflict-free.
+// Source rows are read as 16-byte windows at any byte alignment (two aligned loads and a
+// byte shift), so rows of an odd number of bytes still load one vector per lane.
+struct OopParams {
+  const unsigned char *src;         // 16-byte aligned base; the first block starts at src_off
+  unsigned char *dst;
+  uint64_t src_off;
+  uint64_t src_stride, dst_stride;  // bytes between consecutive blocks
+  uint64_t src_pitch, dst_pitch;    // bytes between rows (source rows of C, destination of R)
+  uint64_t src_end;                 // bytes readable from src (bounds the window over-read)
+  uint64_t R, C;                    // each block R x C (source) -> C x R (destination)
+  uint32_t tr_log2, tc_log2;        // tile TR x TC elements, both powers of two
+  uint32_t PB;                      // LDS row pitch in bytes (multiple of 128)
+  uint64_t ntr, ntc, per_block, tiles, per_xcd;
+};
+
+template <int ES, int SW>
+__device__ __forceinline__ uint32_t oop_mask(uint32_t c) {
+  const uint32_t cs = c >> SW;
+  const uint32_t hi = (cs & 7u) << 4;
+  if (ES <= 4) return hi | (((cs >> 3) & 3u) << 2);
+  if (ES == 8) return hi | (((cs >> 3) & 1u) << 3);
+  return hi;
+}
+
+template <int ES, bool VIN, bool VOUT, int NT>
+__global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
+  using E = typename Elem<ES>::T;
+  constexpr uint32_t EPV = 16 / ES;  // elements per 16-byte vector
+  constexpr uint32_t EPV_LOG2 = ES == 1 ? 4 : ES == 2 ? 3 : ES == 4 ? 2 : ES == 8 ? 1 : 0;
+  constexpr int SW = VIN ? int(EPV_LOG2) : 0;
+  extern __shared__ __align__(16) unsigned char lds[];
+  // Consecutive tiles go to the same XCD (workgroups are dealt round-robin over 8 XCDs), so
+  // partial lines shared by neighbouring tiles meet in one L2.
+  const uint64_t t = uint64_t(blockIdx.x & 7u) * p.per_xcd + (blockIdx.x >> 3);
+  if (t >= p.tiles) return;
+  const uint64_t g = t / p.per_block;
+  const uint64_t rem = t - g * p.per_block;
+  const uint64_t tr = rem / p.ntc, tc = rem - tr * p.ntc;
+  const uint32_t TR = 1u << p.tr_log2, TC = 1u << p.tc_log2;
+  const uint64_t r0 = tr << p.tr_log2, c0 = tc << p.tc_log2;
+  const uint32_t nr = uint32_t(std::min<uint64_t>(TR, p.R - r0));
+  const uint32_t nc = uint32_t(std::min<uint64_t>(TC, p.C - c0));
+  const uint64_t sblk = p.src_off + g * p.src_stride;  // offset of this block's row 0 in src
+  unsigned char *dst = p.dst + g * p.dst_stride;
+  const uint32_t PB = p.PB;
+
+  // Load: source row r, column c -> LDS row c, element r.
+  if (VIN) {  // one 16-byte window per lane (TC >= EPV)
+    const uint32_t vl = p.tc_log2 - EPV_LOG2;
+    const uint32_t vpr = 1u << vl;
+    for (uint32_t i = threadIdx.x; i < (TR << vl); i += NT) {
+      const uint32_t r = i >> vl, v = i & (vpr - 1u);
+      if (r < nr && v * EPV < nc) {
+        const uint64_t sb = sblk + (r0 + r) * p.src_pitch + (c0 + v * EPV) * ES;
+        union {
+          U128 x;
+          E e[EPV];
+        } u;
+        const uint64_t need = (sb & 15u) ? (sb & ~uint64_t(15)) + 32 : sb + 16;
+        if (need <= p.src_end) {
+          u.x = load_window16(p.src, sb);
+        } else {  // window would run past the readable end: whole elements only
+#pragma unroll
+          for (uint32_t k = 0; k < EPV; k++)
+            if (v * EPV + k < nc) u.e[k] = *reinterpret_cast<const E *>(p.src + sb + k * ES);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < EPV; k++) {
+          const uint32_t c = v * EPV + k;  // c < TC; columns >= nc are never stored
+          *reinterpret_cast<E *>(lds + c * PB + ((r * ES) ^ oop_mask<ES, SW>(c))) = u.e[k];
+        }
+      }
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < (TR << p.tc_log2); i += NT) {
+      const uint32_t r = i >> p.tc_log2, c = i & (TC - 1u);
+      if (r < nr && c < nc)
+        *reinterpret_cast<E *>(lds + c * PB + ((r * ES) ^ oop_mask<ES, SW>(c))) =
+            *reinterpret_cast<const E *>(p.src + sblk + (r0 + r) * p.src_pitch +
+                                         (c0 + c) * ES);
+    }
+  }
+  __syncthreads();
+  // Store: destination row c0 + c holds source column c0 + c; its segment starts at r0.
+  if (VOUT) {  // destination rows and tile rows are 16-byte aligned
+    const uint32_t vl = p.tr_log2 - EPV_LOG2;
+    const uint32_t vpr = 1u << vl;
+    for (uint32_t i = threadIdx.x; i < (TC << vl); i += NT) {
+      const uint32_t c = i >> vl, j = i & (vpr - 1u);
+      if (c < nc && j * EPV < nr) {
+        const uint32_t m = oop_mask<ES, SW>(c);
+        const U128 y = *reinterpret_cast<const U128 *>(lds + c * PB + ((j << 4) ^ (m & ~15u)));
+        // Undo the word swizzle inside the chunk: logical word w sits at w ^ s.
+        const uint32_t s = (m >> 2) & 3u;
+        U128 x;
+        x.x = (s == 0) ? y.x : (s == 1) ? y.y : (s == 2) ? y.z : y.w;
+        x.y = (s == 0) ? y.y : (s == 1) ? y.x : (s == 2) ? y.w : y.z;
+        x.z = (s == 0) ? y.z : (s == 1) ? y.w : (s == 2) ? y.x : y.y;
+        x.w = (s == 0) ? y.w : (s == 1) ? y.z : (s == 2) ? y.y : y.x;
+        *reinterpret_cast<U128 *>(dst + (c0 + c) * p.dst_pitch + r0 * ES + (j << 4)) = x;
+      }
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < (TC << p.tr_log2); i += NT) {
+      const uint32_t c = i >> p.tr_log2, r = i & (TR - 1u);
+      if (c < nc && r < nr)
+        *reinterpret_cast<E *>(dst + (c0 + c) * p.dst_pitch + (r0 + r) * ES) =
+            *reinterpret_cast<const E *>(lds + c * PB + ((r * ES) ^ oop_mask<ES, SW>(c)));
+    }
+  }
+}
+
+// ---- byte copies ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_copy16(U128 *dst, const U128 *src, uint64_t n) {
+  uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+  const uint64_t stride = uint64_t(gridDim.x) * 256;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(256) void k_copy1(unsigned char *dst, const unsigned char *src,
+                                               uint64_t n) {
+  uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+  const uint64_t stride = uint64_t(gridDim.x) * 256;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// rows x width bytes between pitched layouts.
+__global__ __launch_bounds__(256) void k_copy2d(unsigned char *dst, uint64_t dpitch,
+                                                const unsigned char *src, uint64_t spitch,
+                                                uint64_t width, uint64_t rows,
+                                                FastDiv64 div_w) {
+  const uint64_t n = width * rows;
+  uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+  const uint64_t stride = uint64_t(gridDim.x) * 256;
+  for (; i < n; i += stride) {
+    const uint64_t r = div_w.div(i), c = i - r * width;
+    dst[r * dpitch + c] = src[r * spitch + c];
+  }
+}
+
+hipError_t launch_copy(unsigned char *dst, const unsigned char *src, uint64_t bytes,
+                       hipStream_t st) {
+  if (bytes == 0) return hipSuccess;
+  const uint32_t cap = uint32_t(device_info().cus) * 16;
+  if (((uintptr_t(dst) | uintptr_t(src) | bytes) & 15u) == 0) {
+    const uint64_t n = bytes / 16;
+    const uint32_t blocks = uint32_t(std::min<uint64_t>((n + 255) / 256, cap));
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, st, reinterpret_cast<U128 *>(dst),
+                       reinterpret_cast<const U128 *>(src), n);
+  } else {
+    const uint32_t blocks = uint32_t(std::min<uint64_t>((bytes + 255) / 256, cap));
+    hipLaunchKernelGGL(k_copy1, dim3(blocks), dim3(256), 0, st, dst, src, bytes);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_copy2d(unsigned char *dst, uint64_t dpitch, const unsigned char *src,
+                         uint64_t spitch, uint64_t width, uint64_t rows, hipStream_t st) {
+  if (width == 0 || rows == 0) return hipSuccess;
+  const uint64_t n = width * rows;
+  const uint32_t blocks =
+      uint32_t(std::min<uint64_t>((n + 255) / 256, uint64_t(device_info().cus) * 16));
+  hipLaunchKernelGGL(k_copy2d, dim3(blocks), dim3(256), 0, st, dst, dpitch, src, spitch, width,
+                     rows, FastDiv64(width));
+  return hipGetLastError();
+}
+
+// ---- host: out-of-place transpose launcher ----------------------------------------------
+namespace {
+uint32_t log2_floor(uint64_t v) {
+  uint32_t l = 0;
+  while ((uint64_t(2) << l) <= v) l++;
+  return l;
+}
+uint32_t log2_ceil(uint64_t v) {
+  uint32_t l = 0;
+  while ((uint64_t(1) << l) < v) l++;
+  return l;
+}
+
+constexpr int kOopThreads = 512;
+
+template <int ES, bool VIN, bool VOUT>
+hipError_t launch_oop_t(const OopParams &p, size_t lds, hipStream_t st) {
+  auto k = k_oop_transpose<ES, VIN, VOUT, kOopThreads>;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k, dim3(uint32_t(p.per_xcd * 8)), dim3(kOopThreads), lds, st, p);
+  return hipGetLastError();
+}
+
+template <int ES>
+hipError_t launch_oop_es(const OopParams &p, bool vin, bool vout, size_t lds, hipStream_t st) {
+  if (vin && vout) return launch_oop_t<ES, true, true>(p, lds, st);
+  if (vin) return launch_oop_t<ES, true, false>(p, lds, st);
+  if (vout) return launch_oop_t<ES, false, true>(p, lds, st);
+  return launch_oop_t<ES, false, false>(p, lds, st);
+}
+}  // namespace
+
+hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, size_t es,
+                                uint64_t blocks, uint64_t R, uint64_t C, uint64_t src_stride,
+                                uint64_t dst_stride, size_t tile_bytes, hipStream_t st,
+                                uint64_t src_end) {
+  if (blocks == 0 || R == 0 || C == 0) return hipSuccess;
+  // Tile: cover the shorter side whole when a tile of it stays within `tile_bytes` with at
+  // least 64 bytes along the other side; otherwise a square-ish tile.
+  const uint64_t cap_el = std::max<uint64_t>(tile_bytes / es, 256);
+  const uint64_t min_other = std::max<uint64_t>(1, 64 / es);
+  uint32_t trl, tcl;
+  if (R <= C && (uint64_t(1) << log2_ceil(R)) * min_other <= cap_el) {
+    trl = log2_ceil(R);
+    tcl = log2_floor(cap_el >> trl);
+  } else if (C < R && (uint64_t(1) << log2_ceil(C)) * min_other <= cap_el) {
+    tcl = log2_ceil(C);
+    trl = log2_floor(cap_el >> tcl);
+  } else {
+    const uint32_t l = log2_floor(cap_el);
+    tcl = (l + 1) / 2;
+    trl = l - tcl;
+  }
+  // Never larger than the matrix side rounded up to a power of two.
+  trl = std::min(trl, log2_ceil(R));
+  tcl = std::min(tcl, log2_ceil(C));
+  // LDS rows: TR elements padded to a multiple of 128 bytes; TC rows within the tile budget.
+  const uint32_t PB = uint32_t((((uint64_t(1) << trl) * es) + 127) / 128 * 128);
+  while (tcl > 0 && (uint64_t(1) << tcl) * PB > std::max<size_t>(tile_bytes, PB)) tcl--;
+  OopParams p{};
+  const uintptr_t sa = reinterpret_cast<uintptr_t>(src);
+  p.src = reinterpret_cast<const unsigned char *>(sa & ~uintptr_t(15));
+  p.src_off = sa & 15u;
+  p.dst = dst;
+  p.src_stride = src_stride;
+  p.dst_stride = dst_stride;
+  p.src_pitch = C * es;
+  p.dst_pitch = R * es;
+  p.src_end = p.src_off + (src_end ? src_end : (blocks - 1) * src_stride + R * C * es);
+  p.R = R;
+  p.C = C;
+  p.tr_log2 = trl;
+  p.tc_log2 = tcl;
+  p.PB = PB;
+  const uint32_t TC = 1u << tcl;
+  p.ntr = (R + (1ull << trl) - 1) >> trl;
+  p.ntc = (C + (1ull << tcl) - 1) >> tcl;
+  p.per_block = p.ntr * p.ntc;
+  p.tiles = p.per_block * blocks;
+  p.per_xcd = (p.tiles + 7) / 8;
+  if (p.per_xcd * 8 >= (1ull << 32)) return hipErrorInvalidValue;
+  const size_t lds = size_t(TC) * PB;
+  if (lds > device_info().lds_per_block) return hipErrorInvalidValue;
+  const uint32_t EPV = uint32_t(16 / es);
+  auto al = [](uint64_t v) { return (v & 15u) == 0; };
+  const bool vin = TC >= EPV;
+  const bool vout = al(reinterpret_cast<uintptr_t>(dst)) && al(p.dst_pitch) && al(dst_stride) &&
+                    (1u << trl) >= EPV;
+  switch (es) {
+    case 1: return launch_oop_es<1>(p, vin, vout, lds, st);
+    case 2: return launch_oop_es<2>(p, vin, vout, lds, st);
+    case 4: return launch_oop_es<4>(p, vin, vout, lds, st);
+    case 8: return launch_oop_es<8>(p, vin, vout, lds, st);
+    case 16: return launch_oop_es<16>(p, vin, vout, lds, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+// ---- host: planning ---------------------------------------------------------------------
+namespace {
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Best stripe height for dimension D (the other one is L): a divisor of D near the target
+// super-element size when one exists, else the target itself with a ragged remainder.
+// Returns false when no height keeps a stripe within the scratch limit.
+bool pick_height(size_t es, uint64_t D, uint64_t L, uint64_t target_bytes, uint64_t max_stripe,
+                 uint64_t *h_out) {
+  uint64_t target = std::max<uint64_t>(1, target_bytes / es);
+  target = std::min<uint64_t>(target, 1024);
+  while (target > 1 && target * L * es > max_stripe) target >>= 1;
+  if (target * L * es > max_stripe) return false;
+  if (D < 2 * target) return false;  // fewer than two stripes: not worth a scratch copy
+  // A divisor in [target/2, 2*target] (within the scratch limit) avoids the ragged step.
+  uint64_t best = 0;
+  for (uint64_t h = target / 2 == 0 ? 1 : target / 2; h <= 2 * target && h <= 1024; h++) {
+    if (D % h || h * L * es > max_stripe || (h * es) % 16) continue;
+    if (!best || (h > best && h <= target) || (best > target && h < best)) best = h;
+  }
+  *h_out = best ? best : target;
+  return true;
+}
+}  // namespace
+
+bool plan_st(size_t es, uint64_t M, uint64_t N, int variant, uint64_t target_bytes,
+             uint64_t scratch_cap, int64_t pt_vw, bool rotate, StPlan *out) {
+  if (M < 2 || N < 2) return false;
+  const uint64_t D = variant == 0 ? M : N;   // striped dimension
+  const uint64_t L = variant == 0 ? N : M;   // the other one
+  uint64_t h = 0;
+  const uint64_t max_stripe = std::max<uint64_t>(scratch_cap, 1);
+  if (!pick_height(es, D, L, target_bytes, max_stripe, &h)) return false;
+  StPlan p{};
+  p.variant = variant;
+  p.es = es;
+  p.M = M;
+  p.N = N;
+  p.h = h;
+  p.q = D / h;
+  p.r = D - p.q * h;
+  const uint64_t stripe = h * L * es;
+  p.rotate = rotate;
+  p.group = rotate ? 1 : std::max<uint64_t>(1, std::min<uint64_t>(p.q, scratch_cap / stripe));
+  if (p.q > 1) {
+    if (variant == 0) plan_100(es, 1, p.q, N, h, &p.pt, pt_vw);
+    else plan_100(es, 1, M, p.q, h, &p.pt, pt_vw);
+  }
+  const uint64_t pt_words = p.q > 1 ? p.pt.flag_words : 0;
+  const uint64_t pad_words = p.r ? pad_flag_words(M * N, es) : 0;
+  p.stripe_off = 0;
+  p.b_off = align256(p.group * stripe);
+  p.f_off = p.b_off + align256(p.r * L * es);
+  p.flag_words = std::max(pt_words, pad_words);
+  p.scratch_bytes = p.f_off + align256(p.flag_words * 4);
+  *out = p;
+  return true;
+}
+
+hipError_t run_st(unsigned char *buf, const StPlan &p, unsigned char *scratch, size_t tile_bytes,
+                  uint32_t pt_blocks, hipStream_t st) {
+  const size_t es = p.es;
+  const uint64_t M = p.M, N = p.N, h = p.h, q = p.q, r = p.r;
+  const uint64_t total = M * N * es;
+  unsigned char *S = scratch + p.stripe_off;
+  unsigned char *B = scratch + p.b_off;
+  uint32_t *F = reinterpret_cast<uint32_t *>(scratch + p.f_off);
+  const bool rot = p.rotate && q > 1;
+  hipError_t e = hipSuccess;
+#define ST_TRY(x) do { e = (x); if (e != hipSuccess) return e; } while (0)
+  if (p.variant == 0) {
+    const uint64_t stripe = h * N * es;
+    if (r)
+      ST_TRY(launch_oop_transpose(buf + q * stripe, B, es, 1, r, N, 0, 0, tile_bytes, st,
+                                  total - q * stripe));
+    if (rot) {
+      // Stripe k is transposed into slot k - 1 (stripe 0 into scratch, copied to the last
+      // slot at the end): each launch reads one slot and writes another, so no copy pass.
+      ST_TRY(launch_oop_transpose(buf, S, es, 1, h, N, 0, 0, tile_bytes, st, total));
+      for (uint64_t k = 1; k < q; k++)
+        ST_TRY(launch_oop_transpose(buf + k * stripe, buf + (k - 1) * stripe, es, 1, h, N, 0, 0,
+                                    tile_bytes, st, total - k * stripe));
+      ST_TRY(launch_copy(buf + (q - 1) * stripe, S, stripe, st));
+    } else {
+      for (uint64_t g0 = 0; g0 < q; g0 += p.group) {
+        const uint64_t n = std::min(p.group, q - g0);
+        ST_TRY(launch_copy(S, buf + g0 * stripe, n * stripe, st));
+        ST_TRY(launch_oop_transpose(S, buf + g0 * stripe, es, n, h, N, stripe, stripe,
+                                    tile_bytes, st));
+      }
+    }
+    if (q > 1) {  // slot s holds stripe s + 1 (mod q) when rotated
+      ST_TRY(launch_fill_u32(F, p.pt.flag_words, 0, st));
+      ST_TRY(launch_100_pttwac(buf, F, 1, q, N, h * es, p.pt, pt_blocks, st, rot ? 1 : 0, 0));
+    }
+    if (r) {
+      ST_TRY(launch_padding(buf, es, q * h, N, M, false, F, st));
+      ST_TRY(launch_copy2d(buf + q * h * es, M * es, B, r * es, r * es, N, st));
+    }
+  } else {
+    const uint64_t stripe = h * M * es;
+    if (r) {
+      ST_TRY(launch_copy2d(B, r * es, buf + q * h * es, N * es, r * es, M, st));
+      ST_TRY(launch_padding(buf, es, q * h, M, N, true, F, st));
+    }
+    if (q > 1) {  // rotated: block k lands in slot k + 1 (mod q)
+      ST_TRY(launch_fill_u32(F, p.pt.flag_words, 0, st));
+      ST_TRY(launch_100_pttwac(buf, F, 1, M, q, h * es, p.pt, pt_blocks, st, 0, rot ? 1 : 0));
+    }
+    if (rot) {
+      ST_TRY(launch_copy(S, buf, stripe, st));
+      for (uint64_t k = 0; k + 1 < q; k++)
+        ST_TRY(launch_oop_transpose(buf + (k + 1) * stripe, buf + k * stripe, es, 1, M, h, 0, 0,
+                                    tile_bytes, st, total - (k + 1) * stripe));
+      ST_TRY(launch_oop_transpose(S, buf + (q - 1) * stripe, es, 1, M, h, 0, 0, tile_bytes, st));
+    } else {
+      for (uint64_t g0 = 0; g0 < q; g0 += p.group) {
+        const uint64_t n = std::min(p.group, q - g0);
+        ST_TRY(launch_copy(S, buf + g0 * stripe, n * stripe, st));
+        ST_TRY(launch_oop_transpose(S, buf + g0 * stripe, es, n, M, h, stripe, stripe,
+                                    tile_bytes, st));
+      }
+    }
+    if (r) ST_TRY(launch_oop_transpose(B, buf + q * stripe, es, 1, M, r, 0, 0, tile_bytes, st));
+  }
+#undef ST_TRY
+  return hipSuccess;
+}
+
+}  // namespace lm

@@ -343,3 +343,32 @@ def test_soa_small_tiles_take_row_column_route(hip, orc, es):
         say("soa small tile", es, h, w, t)
         x = orc.tagged(h * w, es, seed=h + w + t)
         check(hip.soa_asta, x, orc.op_100(x, w, h // t, t, es=es), h, w, t, elem_bytes=es)
+
+
+@pytest.mark.parametrize("es", ES)
+def test_stripe_route(hip, orc, es):
+    """stages = 6 / 7: stripes of rows / of columns through bounded scratch, with and without a
+    ragged remainder (prime dimensions included), several scratch fills per call."""
+    shapes = [(768, 131), (131, 768), (1031, 997), (640, 4099), (4099, 640), (4096, 3001)]
+    keep = {k: hip.get_tuning(k) for k in ("st_super_bytes", "st_scratch_mb")}
+    try:
+        for super_bytes, scratch_mb in ((64, 1), (2048, 256)):
+            hip.set_tuning("st_super_bytes", super_bytes)
+            hip.set_tuning("st_scratch_mb", scratch_mb)
+            for (M, N) in shapes:
+                if M * N * es > 100 << 20:
+                    continue
+                x = orc.tagged(M * N, es, seed=M * 3 + N + es)
+                want = orc.op_transpose(x, M, N, es=es)
+                for stages, tag in ((6, "ST_R-"), (7, "ST_C-")):
+                    say("stripes", es, M, N, stages, super_bytes, scratch_mb)
+                    t = dev(x)
+                    assert hip.transpose(t, 1, M, 1, N, 1, stages, elem_bytes=es) is False
+                    route = hip.last_route()
+                    got = host(t)
+                    if route != tag:  # too few stripes for this shape: another route ran
+                        assert route in ("RC-", "Fallback-"), route
+                    assert np.array_equal(got, want), (M, N, stages, route)
+    finally:
+        for k, v in keep.items():
+            hip.set_tuning(k, v)
