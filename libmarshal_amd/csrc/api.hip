@@ -82,7 +82,7 @@ Knob g_knobs[] = {
     {"st_scratch_mb", {256}},
     {"st_super_bytes", {2048}},
     {"st_tile_kb", {32}},
-    {"st_pt_vw", {8}},
+    {"st_pt_vw", {16}},
     {"st_rotate", {1}, 0},
 };
 Knob *find_knob(const char *key) {

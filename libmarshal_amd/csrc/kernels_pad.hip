@@ -6,18 +6,16 @@
 // chains them: group k stores only after group k-1 has loaded, and signals k+1 only after
 // its own wait, so the signal crosses every group in sequence.
 //
-// This version keeps the dynamic ids and the "store only after every earlier group has
-// loaded" rule, but publishes the two facts separately, as in a decoupled look-back: a group
-// marks itself LOADED as soon as its loads have returned, then walks back over earlier
-// groups until it finds one marked COMPLETE (that group and all before it have loaded),
-// marks itself COMPLETE and stores. No group waits on a later one, and a group that got its
-// id earlier is already resident, so every wait ends.
+// This version keeps the dynamic ids but drops the chain: a group marks itself LOADED as
+// soon as its loads have returned and then waits only for the groups that read from its own
+// destination range (one short id range, computed from the row maps; see pad_reader_ids).
+// No group waits on a later one, and a group that got its id earlier is already resident,
+// so every wait ends.
 //
-// Why earlier groups are the only hazard: padding moves every element to an address >= its
+// Why readers are always earlier groups: padding moves every element to an address >= its
 // source and is processed from the tail; unpadding moves every element to an address <= its
-// source and is processed from the front. In both cases a later group's sources lie entirely
-// on its own side of its destination range, so only earlier groups can still need to read
-// the region a group overwrites.
+// source and is processed from the front. In both cases the groups that read a range write
+// on the far side of it, and so come first.
 #include "kernels.h"
 
 #include <algorithm>
@@ -27,9 +25,12 @@ namespace lm {
 constexpr int kPadThreads = 256;
 constexpr int kPadPer = 16;  // elements per thread
 constexpr uint64_t kPadTile = uint64_t(kPadThreads) * kPadPer;
-constexpr uint32_t kLoaded = 1, kComplete = 2;
+constexpr uint32_t kLoaded = 1;
 constexpr int kPadVecPer = 8;  // 16-byte vectors per lane in the 16-byte form
 constexpr uint64_t kPadVecTile = uint64_t(kPadThreads) * kPadVecPer;
+constexpr uint32_t kPadVecTileLog2 = 11;
+static_assert(kPadVecTile == (uint64_t(1) << kPadVecTileLog2), "group size");
+static_assert(kPadTile == 4096, "element-form group size is 2^12");
 
 // Enough state words for either form: the element form's groups of kPadTile elements and the
 // 16-byte form's groups of kPadVecTile vectors.
@@ -49,53 +50,65 @@ struct PadParams {
   bool unpad;
 };
 
-// Wait until every group before k (in processing order) has loaded, then mark k COMPLETE.
-// Group k marks itself LOADED first, so later groups can skip over it. The first wave reads
-// the states of 64 predecessors per step (lane l reads group j - l): the walk ends at the
-// nearest COMPLETE group when every group between it and k is at least LOADED, and moves a
-// whole window back when all 64 are LOADED. States only ever grow (0 -> LOADED ->
-// COMPLETE), so re-reading a window after a short sleep is always safe, and every wait ends
-// because each earlier group already holds its id and is running or done.
-__device__ __forceinline__ void pad_lookback(uint32_t *flags, uint64_t k) {
+// Groups are numbered by a dynamic id in processing order (tail first for padding, front
+// first for unpadding). A group may only store once every group that reads from its
+// destination range has loaded. Those readers form one contiguous id range, computed here
+// from the row maps, and always precede the group: padding moves every element to an address
+// >= its source and processes from the tail; unpadding moves every element to an address <=
+// its source and processes from the front, so a reader of a range is always processed
+// before the range's writer. Waiting only for these few groups, instead of for every earlier
+// group, keeps one slow load from holding up every store behind it.
+//
+// [Dlo, Dhi): destination elements of the group; ids count groups of VT units of U elements
+// (U, VT powers of two), nunits units in all. Returns glo > ghi when nobody reads the range.
+__device__ __forceinline__ void pad_reader_ids(const PadParams &p, uint64_t Dlo, uint64_t Dhi,
+                                               uint32_t u_log2, uint32_t vt_log2,
+                                               uint64_t nunits, int64_t &glo, int64_t &ghi) {
+  glo = 1;
+  ghi = 0;
+  const uint64_t X = p.x, P = p.pad, Y = p.y;
+  if (!p.unpad) {  // sources are compact positions s < Y*X; reader of s writes (s/X)*P + s%X
+    if (Dlo >= Y * X) return;
+    const uint64_t plo = Dlo, phi = std::min<uint64_t>(Dhi, Y * X) - 1;
+    const uint64_t rlo = p.div_x.div(plo), rhi = p.div_x.div(phi);
+    const uint64_t dlo = rlo * P + (plo - rlo * X), dhi = rhi * P + (phi - rhi * X);
+    glo = int64_t((nunits - 1 - (dhi >> u_log2)) >> vt_log2);
+    ghi = int64_t((nunits - 1 - (dlo >> u_log2)) >> vt_log2);
+  } else {  // sources are padded positions with column < X; reader writes row*X + column
+    uint64_t row = p.div_pad.div(Dlo), c = Dlo - row * P;
+    if (c >= X) {
+      row++;
+      c = 0;
+    }
+    const uint64_t row2 = p.div_pad.div(Dhi - 1);
+    uint64_t c2 = Dhi - 1 - row2 * P;
+    if (c2 >= X) c2 = X - 1;
+    if (row > row2 || (row == row2 && c > c2)) return;
+    glo = int64_t(((row * X + c) >> u_log2) >> vt_log2);
+    ghi = int64_t(((row2 * X + c2) >> u_log2) >> vt_log2);
+  }
+}
+
+// Marks group k LOADED (its loads have returned), then waits until groups glo..ghi (other
+// than k) are LOADED. They hold smaller ids, so they are running or done and every wait ends.
+__device__ __forceinline__ void pad_wait_readers(uint32_t *flags, uint64_t k, int64_t glo,
+                                                 int64_t ghi) {
   if (threadIdx.x < 64) {
     uint32_t *st = flags + 1;
     const uint32_t lane = threadIdx.x;
-    if (k == 0) {
-      if (lane == 0)
-        __hip_atomic_store(&st[0], kComplete, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      if (lane == 0)
-        __hip_atomic_store(&st[k], kLoaded, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const int64_t top0 = int64_t(k) - 1;  // newest predecessor
-      int64_t top = top0;
-      uint64_t below = 0;
+    if (lane == 0)
+      __hip_atomic_store(&st[k], kLoaded, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (ghi > int64_t(k) - 1) ghi = int64_t(k) - 1;
+    if (glo < 0) glo = 0;
+    for (int64_t b = glo; b <= ghi; b += 64) {
+      const int64_t j = b + int64_t(lane);
       while (true) {
-        const int64_t j = top - int64_t(lane);
-        const uint32_t s = j >= 0 ? __hip_atomic_load(&st[j], __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT)
-                                  : kComplete;  // nothing precedes group 0
-        const uint64_t done = __ballot(s == kComplete);
-        const uint64_t waiting = __ballot(s != kComplete && s != kLoaded);
-        if (done) {
-          below = (done & (~done + 1)) - 1;  // lanes before the nearest COMPLETE
-          if ((waiting & below) == 0) break;
-        } else if (waiting == 0) {
-          top -= 64;
-          continue;
-        }
+        const uint32_t s = j <= ghi ? __hip_atomic_load(&st[j], __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT)
+                                    : kLoaded;
+        if (__ballot(s != kLoaded) == 0) break;
         __builtin_amdgcn_s_sleep(1);
       }
-      // Every group seen LOADED on the way is now known to have all its predecessors loaded
-      // too: mark the nearest ones COMPLETE on their behalf (a true, idempotent statement) so
-      // later walks stop early instead of crossing every group still in flight.
-      const int64_t jb = top - int64_t(lane);
-      if (((below >> lane) & 1u) && jb >= 0)
-        __hip_atomic_store(&st[jb], kComplete, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const int64_t j0 = top0 - int64_t(lane);
-      if (top != top0 && j0 >= 0)
-        __hip_atomic_store(&st[j0], kComplete, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (lane == 0)
-        __hip_atomic_store(&st[k], kComplete, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
   __syncthreads();
@@ -140,7 +153,9 @@ __global__ __launch_bounds__(kPadThreads) void k_pad(PadParams p) {
   }
   wait_vm_loads();
   __syncthreads();
-  pad_lookback(p.flags, k);
+  int64_t glo, ghi;
+  pad_reader_ids(p, lo, hi, 0, 12, p.total, glo, ghi);  // kPadTile = 2^12 elements
+  pad_wait_readers(p.flags, k, glo, ghi);
 #pragma unroll
   for (int j = 0; j < kPadPer; j++) {
     uint64_t d = lo + threadIdx.x + uint64_t(j) * kPadThreads;
@@ -216,7 +231,11 @@ __global__ __launch_bounds__(kPadThreads) void k_pad_vec(PadParams p) {
     if (win[j]) v[j].v = window_join(wsb[j], wlo[j], whi[j]);
   wait_vm_loads();
   __syncthreads();
-  pad_lookback(p.flags, k);
+  int64_t glo, ghi;
+  constexpr uint32_t kEpvLog2 = ES == 1 ? 4 : ES == 2 ? 3 : ES == 4 ? 2 : ES == 8 ? 1 : 0;
+  pad_reader_ids(p, lo * EPV, std::min<uint64_t>(hi * EPV, p.total), kEpvLog2, kPadVecTileLog2,
+                 nvec, glo, ghi);
+  pad_wait_readers(p.flags, k, glo, ghi);
 #pragma unroll
   for (int j = 0; j < kPadVecPer; j++) {
     const uint64_t vi = lo + threadIdx.x + uint64_t(j) * kPadThreads;
