@@ -33,6 +33,12 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="approximate CPU work for the baseline sample")
     p.add_argument("--no-extra", action="store_true")
+    p.add_argument("--profile-batch", action="store_true",
+                   help="only run the 4096-matrix batch (and the same matrices as plain calls)")
+    p.add_argument("--profile-large", action="store_true",
+                   help="only run the two large prime-dimension transposes (for kernel traces)")
+    p.add_argument("--no-large", action="store_true",
+                   help="skip the 103 GB and 50 GB prime-dimension transposes")
     p.add_argument("--profile", type=int, default=0,
                    help="only run N LDS-staged AoS->ASTA launches and N plain 1 GiB device "
                         "copies (the calibration kernel for HBM byte counters), then exit")
@@ -139,6 +145,51 @@ def extra_serve(lm, rate=2000.0, duration=1.0):
     return out
 
 
+def extra_large(lm, M, N, es):
+    """Full in-place transpose of a large non-square matrix (prime dimensions allowed) and
+    back, each timed with HIP events after one untimed round trip; a million sampled
+    positions are checked after each direction."""
+    import torch
+    n = M * N
+    free, _ = torch.cuda.mem_get_info()
+    if free < n * es + (8 << 30):
+        return {"skipped": f"needs {(n * es) / 1e9:.0f} GB of free device memory"}
+    dt = torch.float64 if es == 8 else torch.uint8
+    x = torch.empty(n, dtype=dt, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(M ^ N)
+    step = 1 << 30
+    for s in range(0, n, step):
+        e = min(n, s + step)
+        if es == 8:
+            x[s:e].copy_(torch.rand(e - s, dtype=dt, device="cuda", generator=g))
+        else:
+            x[s:e].copy_(torch.randint(0, 256, (e - s,), dtype=dt, device="cuda", generator=g))
+    idx = torch.randint(0, n, (1 << 20,), device="cuda", generator=g)
+    ref = x[idx].clone()
+    r, c = idx // N, idx % N
+    stream = torch.cuda.current_stream()
+    for _ in range(1):  # untimed: plans both directions and sizes the scratch pool
+        if lm.transpose_full(x, M, N) or lm.transpose_full(x, N, M):
+            raise RuntimeError("large transpose failed")
+    fwd = time_calls(lambda: lm.transpose_full(x, M, N), 1, stream)
+    route = lm.last_route()
+    ok = torch.equal(x[c * M + r], ref)
+    back = time_calls(lambda: lm.transpose_full(x, N, M), 1, stream)
+    route_back = lm.last_route()
+    ok = ok and torch.equal(x[idx], ref)
+    del x, idx, ref, r, c
+    torch.cuda.empty_cache()
+    if not ok:
+        raise RuntimeError(f"{M}x{N} transpose gave wrong elements")
+    gbs_f = 2 * n * es / fwd / 1e9
+    gbs_b = 2 * n * es / back / 1e9
+    return {"shape": [M, N], "elem_bytes": es, "route": route, "route_back": route_back,
+            "ms": round(fwd * 1e3, 2), "ms_back": round(back * 1e3, 2),
+            "GBs": round(gbs_f, 1), "GBs_back": round(gbs_b, 1),
+            "pct_hbm_peak": round(100 * gbs_f / HBM_PEAK_GBS, 2),
+            "pct_hbm_peak_back": round(100 * gbs_b / HBM_PEAK_GBS, 2)}
+
+
 def cpu_config1():
     """The reference's 3-stage route restated on the host: 1800 x 7200 int32, a=18, b=72."""
     import numpy as np
@@ -165,6 +216,13 @@ def main():
 
     torch.cuda.set_device(0)
     lm.set_profiling(False)  # asynchronous calls: no per-call synchronization
+    if args.profile_batch:
+        print(json.dumps({"profile_batch": extra_batch(lm)}), flush=True)
+        return
+    if args.profile_large:
+        for (M, N, es) in ((98304, 131071, 8), (250007, 199999, 1)):
+            print(json.dumps({"profile_large": extra_large(lm, M, N, es)}), flush=True)
+        return
     n, w, t = 1 << 24, 16, 64
     nbytes = n * w * 4
     stream = torch.cuda.current_stream()
@@ -237,6 +295,12 @@ def main():
         del x
         torch.cuda.empty_cache()
         extra["cpu_transpose_1800x7200_i32_3stage"] = cpu_config1()
+        if not args.no_large:
+            del ref, d
+            torch.cuda.empty_cache()
+            extra["transpose_98304x131071_f64"] = extra_large(lm, 98304, 131071, 8)
+            extra["transpose_250007x199999_u8"] = extra_large(lm, 250007, 199999, 1)
+            d = ref = None
         extra["batch_4096_mixed"] = extra_batch(lm)
         extra["serve_poisson_2000"] = extra_serve(lm)
     del ref, d

@@ -15,6 +15,7 @@
 #include <mutex>
 #include <string>
 #include <map>
+#include <memory>
 #include <tuple>
 #include <vector>
 
@@ -301,11 +302,59 @@ void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64
   out.push_back(r);
 }
 
+// A 010 whose tile is too large for LDS, split along B = B1 * B2:
+//   [A][a][B1][B2] --0100(A, a, B1, B2)--> [A][B1][a][B2] --010(A*B1, a, B2)--> [A][B1][B2][a]
+// The first stage moves B2-element super-elements (coalesced cycle following), the second is
+// LDS-staged. B2 is the largest divisor of B whose a x B2 tile fits LDS, with super-elements
+// of at least 64 bytes. Not a reference route: the reference runs such tiles with the
+// element-granular cycle-following 010 (src/cl_marshal.cc:280-336, 814-818).
+static bool push_010_split(std::vector<StageRec> &out, unsigned char *buf, size_t es,
+                           uint64_t A, uint64_t a, uint64_t B) {
+  Bs010Plan pl{};
+  uint64_t best = 0;
+  for (uint64_t d = 1; d * d <= B; d++) {
+    if (B % d) continue;
+    for (uint64_t B2 : {d, B / d}) {
+      if (B2 <= best || B2 >= B || B2 * es < 64) continue;
+      if (plan_010_bs(es, A * (B / B2), a, B2, lds_budget(), reinterpret_cast<uintptr_t>(buf),
+                      int(tuning("bs_upt")), &pl))
+        best = B2;
+    }
+  }
+  if (!best) return false;
+  push_0100(out, buf, es, A, a, B / best, best);
+  StageRec r{};
+  r.es = es;
+  r.buf = buf;
+  r.d0 = A * (B / best);
+  r.d1 = a;
+  r.d2 = best;
+  r.R = 1;
+  r.P = 1;
+  plan_010_bs(es, r.d0, a, best, lds_budget(), reinterpret_cast<uintptr_t>(buf),
+              int(tuning("bs_upt")), &r.bs);
+  r.kind = StageKind::BS010;
+  out.push_back(r);
+  return true;
+}
+
 bool push_010(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t A,
               uint64_t a, uint64_t B, M010 m, int R, int P, const char **tag) {
   if (tag) *tag = "";
   if (A == 0 || a <= 1 || B <= 1) return true;  // 1 x B and a x 1 tiles are unchanged
-  if (m == M010::Auto) m = choose_010(es, A, a, B, R, P, reinterpret_cast<uintptr_t>(buf));
+  if (m == M010::Auto) {
+    m = choose_010(es, A, a, B, R, P, reinterpret_cast<uintptr_t>(buf));
+    if (m != M010::BS && push_010_split(out, buf, es, A, a, B)) {
+      if (tag) *tag = "010_SPLIT-";
+      return true;
+    }
+    // Otherwise each a x B tile by the row / column decomposition (A instances), when its
+    // rows fit LDS: in place, LDS-staged, no element-granular cycle following.
+    if (m != M010::BS && push_rc(out, buf, es, a, B, es, A)) {
+      if (tag) *tag = "010_RC-";
+      return true;
+    }
+  }
   StageRec r{};
   r.es = es;
   r.buf = buf;
@@ -345,13 +394,14 @@ bool push_010(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_
 }
 
 bool push_rc(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t M,
-             uint64_t N, uint64_t S) {
+             uint64_t N, uint64_t S, uint64_t inst) {
   if (M <= 1 || N <= 1) return true;  // a vector is its own transpose
   RcPlan pl{};
-  if (!plan_rc(es, M, N, S, lds_budget(), &pl)) return false;
+  if (!plan_rc(es, M, N, S, lds_budget(), &pl, inst)) return false;
   StageRec r{};
   r.es = es;
   r.buf = buf;
+  r.d0 = inst;
   r.d1 = M;
   r.d2 = N;
   r.d3 = S;
@@ -457,8 +507,20 @@ bool route_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es, uin
   return true;
 }
 
-// Runs stage records in order on one stream.
-bool run_stages(const std::vector<StageRec> &stages, hipStream_t st) {
+size_t stages_scratch_bytes(const std::vector<StageRec> &stages) {
+  size_t m = 0;
+  for (const StageRec &r : stages) {
+    if (r.kind == StageKind::PT100G) m = std::max<size_t>(m, r.pt.flag_words * 4);
+    if (r.kind == StageKind::ST) m = std::max<size_t>(m, r.stp.scratch_bytes);
+  }
+  return m;
+}
+
+// Runs stage records in order on one stream. Scratch (visited bits, the stripe route's
+// buffer) comes from `scr` when given (at least stages_scratch_bytes, private to the caller),
+// else from the device pool.
+static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
+                            unsigned char *scr) {
   for (const StageRec &r : stages) {
     switch (r.kind) {
       case StageKind::BS010:
@@ -484,16 +546,24 @@ bool run_stages(const std::vector<StageRec> &stages, hipStream_t st) {
         break;
       }
       case StageKind::ST: {
-        Lease lease(pool_here(), st);
-        unsigned char *scr = reinterpret_cast<unsigned char *>(lease.get(r.stp.scratch_bytes));
-        if (!scr) return true;
-        LM_CHECK_HIP(run_st(r.buf, r.stp, scr, size_t(tuning("st_tile_kb")) << 10,
+        std::unique_ptr<Lease> lease;
+        unsigned char *buf = scr;
+        if (!buf) {
+          lease.reset(new Lease(pool_here(), st));
+          buf = reinterpret_cast<unsigned char *>(lease->get(r.stp.scratch_bytes));
+        }
+        if (!buf) return true;
+        LM_CHECK_HIP(run_st(r.buf, r.stp, buf, size_t(tuning("st_tile_kb")) << 10,
                             grid_cap(uint32_t(tuning("pt100_blocks_per_cu"))), st));
         break;
       }
       case StageKind::PT100G: {
-        Lease lease(pool_here(), st);
-        uint32_t *flags = lease.get(r.pt.flag_words * 4);
+        std::unique_ptr<Lease> lease;
+        uint32_t *flags = reinterpret_cast<uint32_t *>(scr);
+        if (!flags) {
+          lease.reset(new Lease(pool_here(), st));
+          flags = lease->get(r.pt.flag_words * 4);
+        }
         if (!flags) return true;
         LM_CHECK_HIP(launch_fill_u32(flags, r.pt.flag_words, 0, st));
         LM_CHECK_HIP(launch_100_pttwac(r.buf, flags, r.d0, r.d1, r.d2, r.d3 * r.es, r.pt,
@@ -503,6 +573,14 @@ bool run_stages(const std::vector<StageRec> &stages, hipStream_t st) {
     }
   }
   return false;
+}
+
+bool run_stages(const std::vector<StageRec> &stages, hipStream_t st) {
+  return run_stages_impl(stages, st, nullptr);
+}
+
+bool run_stages_on(const std::vector<StageRec> &stages, hipStream_t st, unsigned char *scratch) {
+  return run_stages_impl(stages, st, scratch);
 }
 
 static bool stage_0100(unsigned char *buf, size_t es, uint64_t inst, uint64_t R, uint64_t C,
@@ -587,7 +665,8 @@ double stage_bytes(const StageRec &r) {
     case StageKind::PT100G:
     case StageKind::PT0100L:
       return double(r.d0) * double(r.d1) * double(r.d2) * double(r.d3) * double(r.es);
-    default: return double(r.d1) * double(r.d2) * double(r.es);  // RC_*, ST
+    default:  // RC_* (d0 instances), ST
+      return double(std::max<uint64_t>(r.d0, 1)) * double(r.d1) * double(r.d2) * double(r.es);
   }
 }
 

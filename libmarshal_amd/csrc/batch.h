@@ -41,6 +41,9 @@ struct BatchPlan {
 };
 
 bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, const char *who);
+// Stage list of one descriptor; allow_st admits the stripe route (plain execution only).
+bool plan_desc(const hip_marshal_desc &d, std::vector<StageRec> &out, std::string &route,
+               bool allow_st);
 // Host image of the parameter buffer, with the flag offsets turned into pointers into `flags`.
 void build_image(const BatchPlan &b, const uint32_t *flags, std::vector<unsigned char> &host);
 // Clears the flags and launches every group, level by level, on `st`.

@@ -135,10 +135,10 @@ __global__ __launch_bounds__(1024) void kg_rc_col(const RcParams *items, const u
   for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
     uint32_t it = find_item(prefix, n, q);
     const RcParams p = items[it];
-    const uint32_t local = uint32_t(q - prefix[it]);
-    const uint32_t per = (p.ncolblk + 7) / 8;
-    const uint32_t cb = (local & 7) * per + (local >> 3);
-    if (cb < p.ncolblk) rc_col_block<UB, 1024, SHUFFLE>(p, cb, lds);
+    const uint64_t local = q - prefix[it];
+    const uint64_t per = (p.ncb + 7) / 8;
+    const uint64_t cb = (local & 7) * per + (local >> 3);
+    if (cb < p.ncb) rc_col_block<UB, 1024, SHUFFLE>(p, cb, lds);
   }
 }
 
@@ -338,6 +338,40 @@ void build_image(const BatchPlan &b, const uint32_t *flags, std::vector<unsigned
     }
 }
 
+// The stage list of one descriptor (the route its plain entry point would take, except that
+// the stripe route is only allowed when allow_st is set). False for an invalid descriptor.
+bool plan_desc(const hip_marshal_desc &d, std::vector<StageRec> &out, std::string &route,
+               bool allow_st) {
+  size_t es = size_t(d.elem_bytes);
+  unsigned char *buf = static_cast<unsigned char *>(d.ptr);
+  bool ok = (es == 1 || es == 2 || es == 4 || es == 8 || es == 16) && d.ptr &&
+            reinterpret_cast<uintptr_t>(d.ptr) % std::min<size_t>(es, 16) == 0 &&
+            d.rows >= 1 && d.cols >= 1;
+  if (ok && d.op != HIP_MARSHAL_OP_TRANSPOSE) ok = d.tile >= 1 && d.rows % d.tile == 0;
+  if (!ok) return false;
+  uint64_t h = uint64_t(d.rows), w = uint64_t(d.cols), t = uint64_t(d.tile);
+  switch (d.op) {
+    case HIP_MARSHAL_OP_TRANSPOSE:
+      return plan_transpose_stages(out, buf, es, h, w, route, allow_st);
+    case HIP_MARSHAL_OP_AOS_ASTA:  // 010(h/t, t, w)
+      route = "aos_asta";
+      return push_010(out, buf, es, h / t, t, w, M010::Auto, 1, 1, nullptr);
+    case HIP_MARSHAL_OP_ASTA_AOS:  // 010(h/t, w, t)
+      route = "asta_aos";
+      return push_010(out, buf, es, h / t, w, t, M010::Auto, 1, 1, nullptr);
+    case HIP_MARSHAL_OP_SOA_ASTA:  // 100(w, h/t, t)
+      route = "soa_asta";
+      push_0100(out, buf, es, 1, w, h / t, t);
+      return true;
+    case HIP_MARSHAL_OP_ASTA_SOA:  // 100(h/t, w, t)
+      route = "asta_soa";
+      push_0100(out, buf, es, 1, h / t, w, t);
+      return true;
+    default:
+      return false;
+  }
+}
+
 // Plans every descriptor and groups the stages; flags of global-bit stages are recorded as
 // word offsets into the batch's flag space (see build_image).
 bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, const char *who) {
@@ -346,45 +380,13 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
   b.routes.resize(count);
   for (int64_t i = 0; i < count; i++) {
     const hip_marshal_desc &d = descs[i];
-    size_t es = size_t(d.elem_bytes);
-    unsigned char *buf = static_cast<unsigned char *>(d.ptr);
-    bool ok = (es == 1 || es == 2 || es == 4 || es == 8 || es == 16) && d.ptr &&
-              reinterpret_cast<uintptr_t>(d.ptr) % std::min<size_t>(es, 16) == 0 &&
-              d.rows >= 1 && d.cols >= 1;
-    if (ok && d.op != HIP_MARSHAL_OP_TRANSPOSE) ok = d.tile >= 1 && d.rows % d.tile == 0;
-    if (!ok) {
+    std::string route;
+    if (!plan_desc(d, per[i], route, false)) {
       fprintf(stderr, "[libmarshal] %s: invalid descriptor %lld\n", who, (long long)i);
       return false;
     }
-    uint64_t h = uint64_t(d.rows), w = uint64_t(d.cols), t = uint64_t(d.tile);
-    std::string route;
-    switch (d.op) {
-      case HIP_MARSHAL_OP_TRANSPOSE:
-        ok = plan_transpose_stages(per[i], buf, es, h, w, route, false);
-        break;
-      case HIP_MARSHAL_OP_AOS_ASTA:  // 010(h/t, t, w)
-        ok = push_010(per[i], buf, es, h / t, t, w, M010::Auto, 1, 1, nullptr);
-        route = "aos_asta";
-        break;
-      case HIP_MARSHAL_OP_ASTA_AOS:  // 010(h/t, w, t)
-        ok = push_010(per[i], buf, es, h / t, w, t, M010::Auto, 1, 1, nullptr);
-        route = "asta_aos";
-        break;
-      case HIP_MARSHAL_OP_SOA_ASTA:  // 100(w, h/t, t)
-        push_0100(per[i], buf, es, 1, w, h / t, t);
-        route = "soa_asta";
-        break;
-      case HIP_MARSHAL_OP_ASTA_SOA:  // 100(h/t, w, t)
-        push_0100(per[i], buf, es, 1, h / t, w, t);
-        route = "asta_soa";
-        break;
-      default:
-        ok = false;
-    }
-    if (!ok) {
-      fprintf(stderr, "[libmarshal] %s: cannot plan descriptor %lld\n", who, (long long)i);
-      return false;
-    }
+    uint64_t h = uint64_t(d.rows), w = uint64_t(d.cols);
+    size_t es = size_t(d.elem_bytes);
     b.routes[i] = route;
     b.stages += int64_t(per[i].size());
     b.bytes += 2.0 * double(h) * double(w) * double(es);
@@ -443,10 +445,10 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
           RcParams p = make_rc(r.buf, r.d1, r.d2, r.rc);
           if (pass == 0) {
             g.lds = std::max(g.lds, r.rc.row_lds);
-            add_item(g, p, (r.d1 + p.rows - 1) / p.rows);
+            add_item(g, p, (p.inst * r.d1 + p.rows - 1) / p.rows);
           } else {
             g.lds = std::max(g.lds, r.rc.col_lds);
-            add_item(g, p, uint64_t((p.ncolblk + 7) / 8) * 8);
+            add_item(g, p, (p.ncb + 7) / 8 * 8);
           }
           break;
         }

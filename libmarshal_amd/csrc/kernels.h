@@ -91,17 +91,23 @@ struct RcPlan {
   uint32_t rows;      // rows per workgroup iteration of the row pass
   uint32_t w;         // columns per block of a column pass
   uint32_t c;         // gcd(M, N)
+  uint64_t inst;      // independent M x N matrices stored back to back
   size_t row_lds, col_lds;
 };
 struct RcParams {
   unsigned char *base;
+  uint64_t inst, ncb;  // instances; column blocks over all instances (inst * ncolblk)
+  FastDiv32 div_M32;   // local row of a global row index
+  FastDiv64 div_ncolblk;
   uint32_t M, N, c, a, b, ainv, m, rows, w, ncolblk;
   FastDiv32 div_c, div_b, div_m, div_upr, div_pitch, div_seg_last;
   FastDiv64 div_b64, div_M64;
 };
-// False when M or N is < 2 or >= 2^31, or a row or a one-element-wide column of S-byte
-// elements does not fit `lds_budget`.
-bool plan_rc(size_t es, uint64_t M, uint64_t N, uint64_t S, size_t lds_budget, RcPlan *out);
+// False when M or N is < 2 or >= 2^31, inst * M >= 2^31, or a row or a one-element-wide
+// column of S-byte elements does not fit `lds_budget`. `inst` matrices of M x N are
+// transposed independently (each in its own place).
+bool plan_rc(size_t es, uint64_t M, uint64_t N, uint64_t S, size_t lds_budget, RcPlan *out,
+             uint64_t inst = 1);
 RcParams make_rc(unsigned char *base, uint64_t M, uint64_t N, const RcPlan &pl);
 // pass: 0 = row shuffle, 1 = column rotation, 2 = column shuffle.
 hipError_t launch_rc(unsigned char *base, uint64_t M, uint64_t N, const RcPlan &pl, int pass,
@@ -164,7 +170,7 @@ struct StageRec {
   int R, P;                 // 010 cycle following: bit spreading / padding
   Bs010Plan bs;
   Pt100Plan pt;
-  RcPlan rc;                // RC_*: (M, N) in d1, d2 and the element bytes in d3
+  RcPlan rc;                // RC_*: instances in d0, (M, N) in d1, d2, element bytes in d3
   StPlan stp;               // ST: the whole stripe route (M, N in d1, d2)
 };
 struct Choice {
@@ -187,10 +193,13 @@ bool plan_transpose_stages(std::vector<StageRec> &out, unsigned char *buf, size_
                            uint64_t M, uint64_t N, std::string &route, bool allow_st = true);
 Choice plan_full(size_t es, uint64_t M, uint64_t N, bool allow_st = true);
 bool run_stages(const std::vector<StageRec> &stages, hipStream_t st);
+// Same, with caller-owned scratch of at least stages_scratch_bytes(stages) bytes.
+bool run_stages_on(const std::vector<StageRec> &stages, hipStream_t st, unsigned char *scratch);
+size_t stages_scratch_bytes(const std::vector<StageRec> &stages);
 // Appends the row / column decomposition of an M x N transpose of S-byte elements; false
 // when it does not fit the LDS budget.
 bool push_rc(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t M,
-             uint64_t N, uint64_t S);
+             uint64_t N, uint64_t S, uint64_t inst = 1);
 
 // Device properties used for launch sizing.
 struct DeviceInfo {

@@ -398,8 +398,8 @@ __device__ __forceinline__ void rc_row_block(const RcParams &p, uint64_t blk,
                                              typename Elem<UB>::T *lds) {
   using V = typename Elem<UB>::T;
   const uint32_t tid = threadIdx.x;
-  const uint32_t r0 = uint32_t(blk) * p.rows;
-  const uint32_t rows = std::min<uint32_t>(p.rows, p.M - r0);
+  const uint32_t r0 = uint32_t(blk) * p.rows;  // global row (instances back to back)
+  const uint32_t rows = std::min<uint32_t>(p.rows, uint32_t(p.inst * p.M) - r0);
   const uint32_t upr = p.N * p.m;  // units per row
   const uint32_t total = rows * upr;
   V *g = reinterpret_cast<V *>(p.base) + uint64_t(r0) * upr;
@@ -419,7 +419,8 @@ __device__ __forceinline__ void rc_row_block(const RcParams &p, uint64_t blk,
     uint32_t rem = o - rr * upr;
     uint32_t jp = p.div_m.div(rem);
     uint32_t k = rem - jp * p.m;
-    uint32_t r = r0 + rr;
+    uint32_t gr = r0 + rr;
+    uint32_t r = gr - p.div_M32.div(gr) * p.M;  // row within its instance
     uint32_t rc = p.div_c.div(r);
     uint32_t j = rc_row_src(p, r, rc, r - rc * p.c, jp);
     __builtin_nontemporal_store(lds[rr * upr + j * p.m + k], g + o);
@@ -428,18 +429,21 @@ __device__ __forceinline__ void rc_row_block(const RcParams &p, uint64_t blk,
 }
 
 // ---- column pass: a block of w columns over all M rows ------------------------------------
+// cbt counts column blocks over all instances: instance cbt / ncolblk, block cbt % ncolblk.
 template <int UB, int NT, bool SHUFFLE>
-__device__ __forceinline__ void rc_col_block(const RcParams &p, uint32_t cb,
+__device__ __forceinline__ void rc_col_block(const RcParams &p, uint64_t cbt,
                                              typename Elem<UB>::T *lds) {
   using V = typename Elem<UB>::T;
   const uint32_t tid = threadIdx.x;
+  const uint64_t x = p.div_ncolblk.div(cbt);
+  const uint32_t cb = uint32_t(cbt - x * p.ncolblk);
   const uint32_t J = cb * p.w;
   const uint32_t w = std::min<uint32_t>(p.w, p.N - J);
   const uint32_t seg = w * p.m;        // units per row segment
   const uint32_t pitch = p.w * p.m;    // LDS units per row
   const uint32_t upr = p.N * p.m;
   const uint32_t total = p.M * seg;
-  V *g = reinterpret_cast<V *>(p.base) + uint64_t(J) * p.m;
+  V *g = reinterpret_cast<V *>(p.base) + x * uint64_t(p.M) * upr + uint64_t(J) * p.m;
   // stage in: unit v of the block is (row v / seg, unit v % seg)
   {
     FastDiv32 dseg = seg == p.w * p.m ? p.div_pitch : p.div_seg_last;

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(NT) void k_rc_row(RcParams p) {
   using V = typename Elem<UB>::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   V *lds = reinterpret_cast<V *>(lds_raw);
-  const uint64_t nblk = (p.M + p.rows - 1) / p.rows;
+  const uint64_t nblk = (p.inst * p.M + p.rows - 1) / p.rows;
   for (uint64_t b = blockIdx.x; b < nblk; b += gridDim.x) rc_row_block<UB, NT>(p, b, lds);
 }
 
@@ -40,10 +40,10 @@ __global__ __launch_bounds__(NT) void k_rc_col(RcParams p) {
   using V = typename Elem<UB>::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   V *lds = reinterpret_cast<V *>(lds_raw);
-  const uint32_t per = (p.ncolblk + 7) / 8;
-  for (uint32_t q = blockIdx.x; q < per * 8; q += gridDim.x) {
-    uint32_t cb = (q & 7) * per + (q >> 3);
-    if (cb < p.ncolblk) rc_col_block<UB, NT, SHUFFLE>(p, cb, lds);
+  const uint64_t per = (p.ncb + 7) / 8;
+  for (uint64_t q = blockIdx.x; q < per * 8; q += gridDim.x) {
+    uint64_t cb = (q & 7) * per + (q >> 3);
+    if (cb < p.ncb) rc_col_block<UB, NT, SHUFFLE>(p, cb, lds);
   }
 }
 
@@ -60,17 +60,21 @@ static uint64_t inv_mod(uint64_t a, uint64_t m) {
   return uint64_t(t < 0 ? t + int64_t(m) : t);
 }
 
-bool plan_rc(size_t es, uint64_t M, uint64_t N, uint64_t S, size_t lds_budget, RcPlan *out) {
+bool plan_rc(size_t es, uint64_t M, uint64_t N, uint64_t S, size_t lds_budget, RcPlan *out,
+             uint64_t inst) {
   // S-byte elements made of UB-byte units; M, N < 2^31; rows and column blocks fit LDS.
   if (M < 2 || N < 2 || M >= (1ull << 31) || N >= (1ull << 31)) return false;
+  if (inst < 1 || inst * M >= (1ull << 31)) return false;
   int ub = (S % 16 == 0) ? 16 : (S % 8 == 0) ? 8 : (S % 4 == 0) ? 4 : (S % 2 == 0) ? 2 : 1;
   (void)es;
   uint64_t row_bytes = N * S;
   if (row_bytes > lds_budget || M * S > lds_budget) return false;
   // Row pass: as many whole rows as fit ~64 KiB (at least one).
   uint64_t rows = std::max<uint64_t>(1, std::min<uint64_t>(M, (64 * 1024) / row_bytes));
-  // Column pass: segments of >= 16 bytes (whole elements) when the block fits.
-  uint64_t w = std::max<uint64_t>(1, (16 + S - 1) / S);
+  // Column pass: segments of >= 16 bytes (whole elements) when the block fits; a whole
+  // 128-byte line per row when the columns are short.
+  const uint64_t seg_target = M * 128 <= 40 * 1024 ? 128 : 16;
+  uint64_t w = std::max<uint64_t>(1, (seg_target + S - 1) / S);
   while (w > 1 && M * w * S > lds_budget) w--;
   if (w > N) w = N;
   out->ub = ub;
@@ -78,6 +82,7 @@ bool plan_rc(size_t es, uint64_t M, uint64_t N, uint64_t S, size_t lds_budget, R
   out->rows = uint32_t(rows);
   out->w = uint32_t(w);
   out->c = uint32_t(std::gcd(M, N));
+  out->inst = inst;
   out->row_lds = size_t(rows * row_bytes);
   out->col_lds = size_t(M * w * S);
   return true;
@@ -96,6 +101,10 @@ RcParams make_rc(unsigned char *base, uint64_t M, uint64_t N, const RcPlan &pl) 
   p.rows = pl.rows;
   p.w = pl.w;
   p.ncolblk = uint32_t((N + pl.w - 1) / pl.w);
+  p.inst = pl.inst;
+  p.ncb = pl.inst * p.ncolblk;
+  p.div_M32 = FastDiv32(uint32_t(M));
+  p.div_ncolblk = FastDiv64(p.ncolblk);
   p.div_c = FastDiv32(p.c);
   p.div_b = FastDiv32(p.b);
   p.div_b64 = FastDiv64(p.b);
@@ -124,15 +133,15 @@ hipError_t launch_rc_ub(const RcParams &p, const RcPlan &pl, int pass, uint32_t 
     auto k = k_rc_row<UB, NT>;
     hipError_t e = set_lds<NT>(reinterpret_cast<const void *>(k), pl.row_lds);
     if (e != hipSuccess) return e;
-    uint64_t nblk = (p.M + p.rows - 1) / p.rows;
+    uint64_t nblk = (p.inst * p.M + p.rows - 1) / p.rows;
     hipLaunchKernelGGL(k, dim3(uint32_t(std::min<uint64_t>(nblk, max_blocks))), dim3(NT),
                        pl.row_lds, st, p);
   } else {
     auto k = pass == 1 ? k_rc_col<UB, NT, false> : k_rc_col<UB, NT, true>;
     hipError_t e = set_lds<NT>(reinterpret_cast<const void *>(k), pl.col_lds);
     if (e != hipSuccess) return e;
-    uint32_t blocks = uint32_t(std::min<uint64_t>(uint64_t((p.ncolblk + 7) / 8) * 8,
-                                                  (max_blocks + 7) / 8 * 8));
+    uint32_t blocks = uint32_t(std::min<uint64_t>((p.ncb + 7) / 8 * 8,
+                                                  (uint64_t(max_blocks) + 7) / 8 * 8));
     hipLaunchKernelGGL(k, dim3(blocks), dim3(NT), pl.col_lds, st, p);
   }
   return hipGetLastError();

@@ -3,11 +3,13 @@
 // Clients submit conversions of device-resident buffers; a scheduler thread admits queued
 // requests in arrival order while the bytes of admitted, unfinished requests stay within the
 // HBM budget (conversions are in place, so a request's footprint is its buffer plus visited
-// bits, never a second copy), packs everything admitted into one dynamic batch, and launches
-// it as grouped persistent kernels (the batched form's work lists) on whichever of its
-// streams is idle. Requests that arrive while batches are on the GPU are admitted as soon as
-// budget and a stream free up and run concurrently with them. Completion is observed with an
-// event per batch; latency is arrival -> completion.
+// bits, never a second copy), packs the small ones admitted together into one dynamic batch
+// and launches it as grouped persistent kernels (the batched form's work lists) on whichever
+// of its streams is idle; a request of 64 MiB or more runs alone with its plain stage list
+// (per-request kernels, the stripe route included) and the stream's own scratch. Requests
+// that arrive while batches are on the GPU are admitted as soon as budget and a stream free
+// up and run concurrently with them. Completion is observed with an event per batch;
+// latency is arrival -> completion.
 //
 // No counterpart in the reference (a single-call library, src/cl_marshal.cc).
 #include "batch.h"
@@ -20,6 +22,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <string>
 #include <vector>
 
 namespace lm {
@@ -47,6 +50,8 @@ struct Slot {
   size_t host_cap = 0;
   uint32_t *flags = nullptr;
   size_t flags_cap = 0;
+  unsigned char *scr = nullptr;  // scratch of a request run alone (visited bits, stripes)
+  size_t scr_cap = 0;
   bool busy = false;
   std::vector<int64_t> reqs;
   size_t bytes = 0;
@@ -57,6 +62,7 @@ struct Slot {
 struct Server {
   size_t budget = 0;
   int max_batch = 256;
+  size_t solo_bytes = size_t(64) << 20;  // requests this large run alone, on the plain path
   int device = 0;
   std::mutex mu;
   std::condition_variable cv;
@@ -116,6 +122,20 @@ bool launch_batch(Server &s, Slot &sl, const std::vector<hip_marshal_desc> &desc
   return hipEventRecord(sl.ev, sl.st) == hipSuccess;
 }
 
+// A large request runs alone with the stage list its plain entry point would use (stripe
+// route included), on the slot's stream with the slot's own scratch: nothing to amortise,
+// and per-request kernels beat the grouped forms there.
+bool launch_solo(Server &s, Slot &sl, const hip_marshal_desc &d) {
+  (void)s;
+  std::vector<StageRec> list;
+  std::string route;
+  if (!plan_desc(d, list, route, true)) return false;
+  size_t need = stages_scratch_bytes(list);
+  if (need && !grow(reinterpret_cast<void **>(&sl.scr), &sl.scr_cap, need, false)) return false;
+  if (run_stages_on(list, sl.st, sl.scr)) return false;
+  return hipEventRecord(sl.ev, sl.st) == hipSuccess;
+}
+
 void scheduler(Server *sp) {
   Server &s = *sp;
   hipSetDevice(s.device);
@@ -155,9 +175,11 @@ void scheduler(Server *sp) {
       std::vector<int64_t> ids;
       std::vector<hip_marshal_desc> descs;
       size_t add = 0;
+      const bool solo = s.reqs[s.queue.front()].bytes >= s.solo_bytes;
       while (!s.queue.empty() && int(ids.size()) < s.max_batch) {
         Req &r = s.reqs[s.queue.front()];
         if (s.inflight + add + r.bytes > s.budget) break;  // wait for budget to free up
+        if (!ids.empty() && (solo || r.bytes >= s.solo_bytes)) break;  // large ones alone
         add += r.bytes;
         ids.push_back(s.queue.front());
         descs.push_back(r.d);
@@ -176,7 +198,7 @@ void scheduler(Server *sp) {
         s.batches++;
         s.max_in_batch = std::max<int64_t>(s.max_in_batch, int64_t(ids.size()));
         lk.unlock();
-        bool ok = launch_batch(s, *free_slot, descs);
+        bool ok = solo ? launch_solo(s, *free_slot, descs[0]) : launch_batch(s, *free_slot, descs);
         lk.lock();
         if (!ok) {  // mark failed; the slot is retired once its stream drains
           hipStreamSynchronize(free_slot->st);
@@ -365,6 +387,7 @@ void hip_marshal_server_destroy(hip_marshal_server *h) {
   for (Slot &sl : s.slots) {
     if (sl.st) hipStreamSynchronize(sl.st);
     if (sl.flags) hipFree(sl.flags);
+    if (sl.scr) hipFree(sl.scr);
     if (sl.params) hipFree(sl.params);
     if (sl.host) hipHostFree(sl.host);
     if (sl.ev) hipEventDestroy(sl.ev);

@@ -372,3 +372,17 @@ def test_stripe_route(hip, orc, es):
     finally:
         for k, v in keep.items():
             hip.set_tuning(k, v)
+
+
+@pytest.mark.parametrize("es", ES)
+def test_aos_asta_large_tiles_split(hip, orc, es):
+    """AoS->ASTA tiles too large for LDS (t x w elements) take the split route 0100 + 010;
+    both directions must match the oracle's 010, with a ragged and a prime tile width."""
+    for (h, w, t) in ((640, 4096, 64), (128, 3000, 64), (192, 1024, 32), (256, 4099, 64)):
+        if h * w * es > 64 << 20:
+            continue
+        say("aos large tile", es, h, w, t)
+        x = orc.tagged(h * w, es, seed=h * 31 + w + t)
+        check(hip.aos_asta, x, orc.op_010(x, h // t, t, w, es=es), h, w, t, elem_bytes=es)
+        y = orc.op_010(x, h // t, t, w, es=es)
+        check(hip.asta_aos, y, x, h, w, t, elem_bytes=es)
