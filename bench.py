@@ -82,34 +82,53 @@ def cpu_baseline(n_structs, w, t, seconds):
 
 
 def extra_batch(lm, reps=3):
-    """Mixed batch: 4096 matrices in one graph launch vs per-matrix plain calls."""
+    """Mixed batch: 4096 matrices in one graph launch vs the same matrices as per-matrix plain
+    calls (asynchronous, on one stream). Both are run once untimed first (the batch captures
+    its graph, the plain calls fill the planner's cache), then timed with HIP events. The
+    same comparison is repeated per size class on subsets of the same buffers."""
     import torch
     from libmarshal_amd import workloads as wl
     items = wl.batch_mix(4096)
     bufs = [torch.empty(r * c * es, dtype=torch.uint8, device="cuda") for (_, es, r, c, _) in items]
-    total = sum(r * c * es for (_, es, r, c, _) in items)
-    batch = lm.Batch([(b, op, es, r, c, t) for b, (op, es, r, c, t) in zip(bufs, items)])
     stream = torch.cuda.current_stream()
-    sb = time_calls(batch.run, reps, stream)
 
-    def plain():
-        for b, (op, es, r, c, t) in zip(bufs, items):
-            if op == wl.OP_TRANSPOSE:
-                e = lm.transpose_full(b, r, c, elem_bytes=es)
-            elif op == wl.OP_AOS_ASTA:
-                e = lm.aos_asta(b, r, c, t, elem_bytes=es)
-            else:
-                e = lm.soa_asta(b, r, c, t, elem_bytes=es)
-            if e:
-                return True
-        return False
-    sp = time_calls(plain, 1, stream)
-    out = {"matrices": len(items), "gbytes": round(total / 1e9, 2),
-           "kernels_per_launch": batch.info("kernels"),
-           "batched_GBs": round(2 * total / sb / 1e9, 1), "batched_ms": round(sb * 1e3, 2),
-           "plain_GBs": round(2 * total / sp / 1e9, 1), "plain_ms": round(sp * 1e3, 2),
-           "speedup": round(sp / sb, 2)}
-    batch.close()
+    def measure(idx):
+        sel = [items[i] for i in idx]
+        sb_ = [bufs[i] for i in idx]
+        total = sum(r * c * es for (_, es, r, c, _) in sel)
+        batch = lm.Batch([(b, op, es, r, c, t) for b, (op, es, r, c, t) in zip(sb_, sel)])
+
+        def plain():
+            for b, (op, es, r, c, t) in zip(sb_, sel):
+                if op == wl.OP_TRANSPOSE:
+                    e = lm.transpose_full(b, r, c, elem_bytes=es)
+                elif op == wl.OP_AOS_ASTA:
+                    e = lm.aos_asta(b, r, c, t, elem_bytes=es)
+                else:
+                    e = lm.soa_asta(b, r, c, t, elem_bytes=es)
+                if e:
+                    return True
+            return False
+        if batch.run() or plain():
+            raise RuntimeError("batch warm-up failed")
+        sb = time_calls(batch.run, reps, stream)
+        sp = time_calls(plain, 2, stream)
+        out = {"matrices": len(sel), "gbytes": round(total / 1e9, 2),
+               "kernels_per_launch": batch.info("kernels"), "solo": batch.info("solo"),
+               "batched_GBs": round(2 * total / sb / 1e9, 1), "batched_ms": round(sb * 1e3, 2),
+               "plain_GBs": round(2 * total / sp / 1e9, 1), "plain_ms": round(sp * 1e3, 2),
+               "speedup": round(sp / sb, 2)}
+        batch.close()
+        return out
+    out = measure(range(len(items)))
+    classes = {}
+    for name, lo, hi in (("lt_1MB", 0, 1 << 20), ("1_16MB", 1 << 20, 16 << 20),
+                         ("16_128MB", 16 << 20, 128 << 20), ("ge_128MB", 128 << 20, 1 << 62)):
+        idx = [i for i, (_, es, r, c, _) in enumerate(items) if lo <= r * c * es < hi]
+        m = measure(idx)
+        classes[name] = {k: m[k] for k in ("matrices", "gbytes", "batched_GBs", "plain_GBs",
+                                           "speedup")}
+    out["by_size"] = classes
     del bufs
     torch.cuda.empty_cache()
     return out

@@ -143,6 +143,9 @@ bool hip_unpadding_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t 
  * work lists; it returns NULL on an invalid descriptor. hip_marshal_batch_run converts every
  * buffer in place: with use_graph the sequence (bit clearing + one persistent kernel per
  * group and level) is captured into a hipGraph on the first run and replayed afterwards.
+ * Transposes whose best route is the stripe route (rows and columns beyond LDS) have no
+ * grouped form: they run with their plain stage lists, one after another, on a second
+ * stream forked from and joined back to the run's stream (captured into the same graph).
  * Each buffer ends bit-identical to what the plain entry point gives for it alone. Buffers
  * must not overlap; a batch's own visited-bit space is private to it, so different batches
  * may run concurrently on different streams, but one batch must not run twice at once.
@@ -166,7 +169,8 @@ typedef struct hip_marshal_batch hip_marshal_batch;
 hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64_t count,
                                             int use_graph);
 bool hip_marshal_batch_run(hip_marshal_batch *batch, hipStream_t stream, uint64_t *elapsed_ns);
-/* "levels", "stages", "kernels", "flag_bytes", "bytes" (algorithmic bytes of one run). */
+/* "levels", "stages", "kernels", "flag_bytes", "bytes" (algorithmic bytes of one run),
+ * "solo" (matrices run with their plain stage lists on the second stream). */
 int64_t hip_marshal_batch_info(const hip_marshal_batch *batch, const char *key);
 const char *hip_marshal_batch_route(const hip_marshal_batch *batch, int64_t index);
 void hip_marshal_batch_destroy(hip_marshal_batch *batch);

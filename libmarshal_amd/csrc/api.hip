@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -640,9 +641,9 @@ double stage_bw_tbs(const StageRec &r) {
       double S = double(r.d3 * es);
       return std::min(3.0, 0.12 + 0.011 * S);
     }
-    case StageKind::RC_ROW: return es >= 4 ? 3.0 : es == 2 ? 2.0 : 1.2;
+    case StageKind::RC_ROW: return r.rc.ub >= 4 ? 3.0 : r.rc.ub == 2 ? 2.0 : 1.2;
     case StageKind::RC_COL1:
-    case StageKind::RC_COL2: return es >= 4 ? 0.55 : es == 2 ? 0.4 : 0.3;
+    case StageKind::RC_COL2: return r.rc.ub >= 4 ? 0.55 : r.rc.ub == 2 ? 0.4 : 0.3;
     case StageKind::ST: {  // copy + out-of-place transpose + 100 (+ padding and a 2-D copy)
       const StPlan &p = r.stp;
       StageRec pt{};
@@ -665,8 +666,12 @@ double stage_bytes(const StageRec &r) {
     case StageKind::PT100G:
     case StageKind::PT0100L:
       return double(r.d0) * double(r.d1) * double(r.d2) * double(r.d3) * double(r.es);
-    default:  // RC_* (d0 instances), ST
-      return double(std::max<uint64_t>(r.d0, 1)) * double(r.d1) * double(r.d2) * double(r.es);
+    case StageKind::RC_ROW:
+    case StageKind::RC_COL1:
+    case StageKind::RC_COL2:  // d0 instances of d1 x d2 elements of d3 bytes
+      return double(std::max<uint64_t>(r.d0, 1)) * double(r.d1) * double(r.d2) * double(r.d3);
+    default:  // ST: d1 x d2 elements of es bytes
+      return double(r.d1) * double(r.d2) * double(r.es);
   }
 }
 
@@ -675,6 +680,15 @@ double route_cost(const std::vector<StageRec> &v) {
   double t = 0;
   for (const StageRec &r : v) t += stage_bytes(r) / stage_bw_tbs(r);
   return t;
+}
+
+// LIBMARSHAL_PLAN_DEBUG=1 prints every candidate route and its modelled cost per byte.
+bool plan_debug() {
+  static const bool on = [] {
+    const char *e = getenv("LIBMARSHAL_PLAN_DEBUG");
+    return e && *e && *e != '0';
+  }();
+  return on;
 }
 
 std::mutex g_plan_mu;
@@ -701,6 +715,10 @@ Choice plan_full(size_t es, uint64_t M, uint64_t N, bool allow_st) {
     if (route.find("RC-") == std::string::npos && stages == 5) return;
     if (route.find("ST_") == std::string::npos && (stages == 6 || stages == 7)) return;
     double c = route_cost(v);
+    if (plan_debug())
+      fprintf(stderr, "[libmarshal] plan %llux%llu es=%zu: stages %d a=%llu b=%llu %s cost %.4g\n",
+              (unsigned long long)M, (unsigned long long)N, es, stages, (unsigned long long)a,
+              (unsigned long long)b, route.c_str(), c / double(M * N * es));
     if (c < best.cost) {
       best.cost = c;
       best.stages = stages;

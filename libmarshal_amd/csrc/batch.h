@@ -38,15 +38,26 @@ struct BatchPlan {
   int64_t stages = 0;
   double bytes = 0;         // algorithmic bytes of one run (2 x every matrix)
   std::vector<std::string> routes;
+  // Matrices whose best route is the stripe route (no grouped form): their plain stage lists,
+  // run one after another on a second stream, overlapping the grouped kernels.
+  std::vector<std::vector<StageRec>> solo;
+  size_t solo_scratch = 0;  // bytes of scratch the solo lists need (one at a time)
 };
 
-bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, const char *who);
+// allow_solo: matrices best served by the stripe route go to `solo` (the batched form);
+// otherwise every matrix is grouped (the server runs large requests on its own).
+bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, const char *who,
+                bool allow_solo = false);
 // Stage list of one descriptor; allow_st admits the stripe route (plain execution only).
 bool plan_desc(const hip_marshal_desc &d, std::vector<StageRec> &out, std::string &route,
                bool allow_st);
 // Host image of the parameter buffer, with the flag offsets turned into pointers into `flags`.
 void build_image(const BatchPlan &b, const uint32_t *flags, std::vector<unsigned char> &host);
-// Clears the flags and launches every group, level by level, on `st`.
-bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_t st);
+// Clears the flags and launches every group, level by level, on `st`. Solo lists (if any)
+// run on `side` (ordered after the work already on `st`, which waits for them at the end)
+// with `solo_scratch`; fork / join are the events ev_fork / ev_join.
+bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_t st,
+                   hipStream_t side = nullptr, hipEvent_t ev_fork = nullptr,
+                   hipEvent_t ev_join = nullptr, unsigned char *solo_scratch = nullptr);
 
 }  // namespace lm

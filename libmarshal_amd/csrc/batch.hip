@@ -306,7 +306,18 @@ const void *lds_func(int ub, int upl) {
 
 }  // namespace
 
-bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_t st) {
+bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_t st,
+                   hipStream_t side, hipEvent_t ev_fork, hipEvent_t ev_join,
+                   unsigned char *solo_scratch) {
+  const bool fork = !b.solo.empty();
+  if (fork) {
+    if (!side || !ev_fork || !ev_join) return true;
+    LM_CHECK_HIP(hipEventRecord(ev_fork, st));
+    LM_CHECK_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+    for (const auto &list : b.solo)
+      if (run_stages_on(list, side, solo_scratch)) return true;
+    LM_CHECK_HIP(hipEventRecord(ev_join, side));
+  }
   if (b.flag_words) LM_CHECK_HIP(launch_fill_u32(flags, b.flag_words, 0, st));
   unsigned char *base = static_cast<unsigned char *>(params);
   for (auto &level : b.levels) {
@@ -319,6 +330,7 @@ bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_
       LM_CHECK_HIP(hipLaunchKernel(g.func, dim3(g.grid), dim3(g.block), args, g.lds, st));
     }
   }
+  if (fork) LM_CHECK_HIP(hipStreamWaitEvent(st, ev_join, 0));
   return false;
 }
 
@@ -374,13 +386,26 @@ bool plan_desc(const hip_marshal_desc &d, std::vector<StageRec> &out, std::strin
 
 // Plans every descriptor and groups the stages; flags of global-bit stages are recorded as
 // word offsets into the batch's flag space (see build_image).
-bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, const char *who) {
+bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, const char *who,
+                bool allow_solo) {
   // 1. routes
   std::vector<std::vector<StageRec>> per(count);
   b.routes.resize(count);
   for (int64_t i = 0; i < count; i++) {
     const hip_marshal_desc &d = descs[i];
     std::string route;
+    if (allow_solo && d.op == HIP_MARSHAL_OP_TRANSPOSE) {
+      std::vector<StageRec> list;
+      std::string r2;
+      if (plan_desc(d, list, r2, true) && r2.find("ST_") != std::string::npos) {
+        b.solo_scratch = std::max(b.solo_scratch, stages_scratch_bytes(list));
+        b.solo.push_back(std::move(list));
+        b.routes[i] = r2;
+        b.stages += 1;
+        b.bytes += 2.0 * double(d.rows) * double(d.cols) * double(d.elem_bytes);
+        continue;
+      }
+    }
     if (!plan_desc(d, per[i], route, false)) {
       fprintf(stderr, "[libmarshal] %s: invalid descriptor %lld\n", who, (long long)i);
       return false;
@@ -530,6 +555,9 @@ struct hip_marshal_batch {
   BatchPlan b;
   void *params = nullptr;     // device copy of all item arrays and prefix sums
   uint32_t *flags = nullptr;  // visited bits of every global-bit stage
+  unsigned char *solo_scr = nullptr;  // scratch of the stripe-route matrices
+  hipStream_t side = nullptr;         // their stream, forked from and joined to the run's
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;
@@ -547,11 +575,18 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
   auto *hb = new hip_marshal_batch();
   hb->use_graph = use_graph != 0;
   BatchPlan &b = hb->b;
-  if (!plan_batch(descs, count, b, who)) {
+  if (!plan_batch(descs, count, b, who, true)) {
     hip_marshal_batch_destroy(hb);
     return nullptr;
   }
   bool ok = true;
+  if (!b.solo.empty()) {
+    ok = hipStreamCreateWithFlags(&hb->side, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&hb->ev_fork, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&hb->ev_join, hipEventDisableTiming) == hipSuccess;
+    if (ok && b.solo_scratch)
+      ok = hipMalloc(reinterpret_cast<void **>(&hb->solo_scr), b.solo_scratch) == hipSuccess;
+  }
   if (b.flag_words) ok = hipMalloc(&hb->flags, b.flag_words * 4) == hipSuccess;
   if (ok && b.param_bytes) ok = hipMalloc(&hb->params, b.param_bytes) == hipSuccess;
   if (ok && b.param_bytes) {
@@ -582,7 +617,8 @@ bool hip_marshal_batch_run(hip_marshal_batch *hb, hipStream_t st, uint64_t *elap
       hipStream_t cs = nullptr;
       LM_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
       hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-      bool err = e != hipSuccess || enqueue_batch(b, hb->params, hb->flags, cs);
+      bool err = e != hipSuccess || enqueue_batch(b, hb->params, hb->flags, cs, hb->side,
+                                                  hb->ev_fork, hb->ev_join, hb->solo_scr);
       hipGraph_t g = nullptr;
       if (e == hipSuccess) e = hipStreamEndCapture(cs, &g);
       hipStreamDestroy(cs);
@@ -595,7 +631,8 @@ bool hip_marshal_batch_run(hip_marshal_batch *hb, hipStream_t st, uint64_t *elap
       LM_CHECK_HIP(hipGraphInstantiate(&hb->exec, hb->graph, nullptr, nullptr, 0));
     }
     LM_CHECK_HIP(hipGraphLaunch(hb->exec, st));
-  } else if (enqueue_batch(b, hb->params, hb->flags, st)) {
+  } else if (enqueue_batch(b, hb->params, hb->flags, st, hb->side, hb->ev_fork, hb->ev_join,
+                           hb->solo_scr)) {
     return true;
   }
   if (elapsed_ns) {
@@ -617,6 +654,7 @@ int64_t hip_marshal_batch_info(const hip_marshal_batch *hb, const char *key) {
   if (!strcmp(key, "stages")) return b.stages;
   if (!strcmp(key, "flag_bytes")) return int64_t(b.flag_words * 4);
   if (!strcmp(key, "bytes")) return int64_t(b.bytes);
+  if (!strcmp(key, "solo")) return int64_t(b.solo.size());
   if (!strcmp(key, "kernels")) {
     int64_t k = b.flag_words ? 1 : 0;
     for (auto &l : b.levels) k += int64_t(l.size());
@@ -636,6 +674,10 @@ void hip_marshal_batch_destroy(hip_marshal_batch *hb) {
   if (hb->graph) hipGraphDestroy(hb->graph);
   if (hb->params) hipFree(hb->params);
   if (hb->flags) hipFree(hb->flags);
+  if (hb->solo_scr) hipFree(hb->solo_scr);
+  if (hb->ev_fork) hipEventDestroy(hb->ev_fork);
+  if (hb->ev_join) hipEventDestroy(hb->ev_join);
+  if (hb->side) hipStreamDestroy(hb->side);
   delete hb;
 }
 

@@ -141,3 +141,31 @@ def test_batch_large_members(hip, orc):
     torch.cuda.synchronize()
     for i, (b, w) in enumerate(zip(bufs, want)):
         assert np.array_equal(b.cpu().numpy(), w), (i, batch.route(i))
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_with_stripe_route_matrices(hip, orc, use_graph):
+    """Transposes whose best route is the stripe route run on the batch's second stream next
+    to the grouped small ones; every buffer must still match the oracle, on every replay."""
+    import torch
+    big = [(4096, 4099, 16), (2053, 6007, 8)]
+    small = make_mix(orc, 40, 16, 300, seed=21)
+    items = list(small)
+    for k, (M, N, es) in enumerate(big):
+        items.append((orc.tagged(M * N, es, seed=900 + k), hip.OP_TRANSPOSE, es, M, N, 0))
+    want = [expected(orc, it) for it in items]
+    bufs = upload(items)
+    batch = hip.Batch([(b, op, es, r, c, t) for b, (_, op, es, r, c, t) in zip(bufs, items)],
+                      use_graph=use_graph)
+    say("solo", batch.info("solo"), [batch.route(len(small) + k) for k in range(len(big))])
+    assert batch.info("solo") >= 1
+    for replay in range(3):
+        if replay:
+            for b, it in zip(bufs, items):
+                b.copy_(torch.from_numpy(it[0]).cuda())
+            torch.cuda.synchronize()
+        assert batch.run() is False
+        torch.cuda.synchronize()
+        for i, (b, w) in enumerate(zip(bufs, want)):
+            assert np.array_equal(b.cpu().numpy(), w), (replay, i, items[i][1:], batch.route(i))
+    batch.close()
