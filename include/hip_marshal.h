@@ -234,6 +234,17 @@ void hip_marshal_set_profiling(int on);
  * 388-465). The setter returns true on an unknown key or a value outside [min, 2^30] (min is
  * 0 for "st_rotate", 1 otherwise); the getter returns 0 for an unknown key. */
 bool hip_marshal_set_tuning(const char *key, int64_t value);
+/* Measured autotuner for an M x N transpose of elem_bytes elements: the planner's cheapest
+ * candidate routes and the best of each route family are run on `buf` (on `stream`), each
+ * timed and followed by the inverse transpose, so `buf` ends bit-identical to how it started.
+ * The fastest becomes the plan for this shape (hip_transpose_full_es, batches and
+ * hip_marshal_plan_transpose use it from then on); *stages receives its stage code. With the
+ * knob "autotune" set to 1, hip_transpose_full_es does this by itself on the first call for
+ * each shape. Synchronizes the stream; must not be called during graph capture. No
+ * counterpart in the reference (its test picks tiles with a fixed heuristic,
+ * test/cl_unittest.cc:396-495). */
+bool hip_marshal_autotune(hipStream_t stream, void *buf, size_t elem_bytes, int64_t M,
+                          int64_t N, int *stages);
 int64_t hip_marshal_get_tuning(const char *key);
 /* The planner's choice for an M x N transpose of elem_bytes elements: tile factors a | M,
  * b | N and the stage sequence (2, 22, 3, 32; 5 = row/column shuffles, 6/7 = stripes of

@@ -84,6 +84,7 @@ def lib():
         "hip_marshal_flag_bytes_0100": (sz, [i64, i64, i64, i64, sz]),
         "hip_marshal_version": (ctypes.c_char_p, []),
         "hip_marshal_set_tuning": (b, [ctypes.c_char_p, i64]),
+        "hip_marshal_autotune": (b, [vp, vp, sz, i64, i64, ctypes.POINTER(ctypes.c_int)]),
         "hip_marshal_plan_transpose": (b, [i64, i64, sz, ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_int)]),
@@ -130,7 +131,7 @@ EXPORTED = [
     "hip_unpadding_es", "hip_marshal_set_lds_budget", "hip_marshal_get_lds_budget",
     "hip_marshal_set_profiling", "hip_marshal_last_route", "hip_marshal_reserve",
     "hip_marshal_flag_bytes_0100", "hip_marshal_version", "hip_marshal_set_tuning",
-    "hip_marshal_get_tuning", "hip_marshal_plan_transpose", "hip_marshal_batch_create", "hip_marshal_batch_run",
+    "hip_marshal_get_tuning", "hip_marshal_plan_transpose", "hip_marshal_autotune", "hip_marshal_batch_create", "hip_marshal_batch_run",
     "hip_marshal_batch_info", "hip_marshal_batch_route", "hip_marshal_batch_destroy",
     "hip_marshal_server_create", "hip_marshal_server_submit", "hip_marshal_server_submit_at",
     "hip_marshal_server_now", "hip_marshal_server_status", "hip_marshal_server_wait",
@@ -269,6 +270,16 @@ def set_tuning(key, value):
 
 def get_tuning(key):
     return lib().hip_marshal_get_tuning(key.encode())
+
+
+def autotune(buf, M, N, elem_bytes=None, stream=None):
+    """Times the planner's candidate routes for an M x N transpose on `buf` (left unchanged)
+    and makes the fastest the plan for this shape. Returns its stage code."""
+    st = ctypes.c_int(0)
+    if lib().hip_marshal_autotune(_stream(stream), _ptr(buf), _es(buf, elem_bytes), M, N,
+                                  ctypes.byref(st)):
+        raise RuntimeError("hip_marshal_autotune failed")
+    return st.value
 
 
 def plan_transpose(M, N, elem_bytes):

@@ -86,6 +86,7 @@ Knob g_knobs[] = {
     {"st_tile_kb", {32}},
     {"st_pt_vw", {16}},
     {"st_rotate", {1}, 0},
+    {"autotune", {0}, 0},
 };
 Knob *find_knob(const char *key) {
   if (!key) return nullptr;
@@ -693,20 +694,13 @@ bool plan_debug() {
 
 std::mutex g_plan_mu;
 std::map<std::tuple<uint64_t, uint64_t, size_t, size_t, bool>, Choice> g_plan_cache;
+std::map<std::tuple<uint64_t, uint64_t, size_t, size_t, bool>, Choice> g_tuned;  // measured
 }  // namespace
 
-Choice plan_full(size_t es, uint64_t M, uint64_t N, bool allow_st) {
-  Choice best;
-  best.a = M;
-  best.b = N;
-  best.stages = -1;
-  if (M <= 1 || N <= 1) return best;
-  auto key = std::make_tuple(M, N, es, lds_budget(), allow_st);
-  {
-    std::lock_guard<std::mutex> lk(g_plan_mu);
-    auto it = g_plan_cache.find(key);
-    if (it != g_plan_cache.end()) return it->second;
-  }
+// Every candidate route the model considers, cheapest first (one entry per stage code and
+// tile factors).
+static std::vector<Choice> rank_routes(size_t es, uint64_t M, uint64_t N, bool allow_st) {
+  std::vector<Choice> all;
   std::string route;
   auto consider = [&](uint64_t A, uint64_t a, uint64_t B, uint64_t b, int stages) {
     std::vector<StageRec> v;
@@ -719,12 +713,12 @@ Choice plan_full(size_t es, uint64_t M, uint64_t N, bool allow_st) {
       fprintf(stderr, "[libmarshal] plan %llux%llu es=%zu: stages %d a=%llu b=%llu %s cost %.4g\n",
               (unsigned long long)M, (unsigned long long)N, es, stages, (unsigned long long)a,
               (unsigned long long)b, route.c_str(), c / double(M * N * es));
-    if (c < best.cost) {
-      best.cost = c;
-      best.stages = stages;
-      best.a = (stages == 22) ? M : a;
-      best.b = (stages == 2) ? N : b;
-    }
+    Choice ch;
+    ch.cost = c;
+    ch.stages = stages;
+    ch.a = (stages == 22) ? M : a;
+    ch.b = (stages == 2) ? N : b;
+    all.push_back(ch);
   };
   consider(1, M, 1, N, -1);  // element-granular fallback
   consider(1, M, 1, N, 5);   // row / column decomposition
@@ -743,16 +737,36 @@ Choice plan_full(size_t es, uint64_t M, uint64_t N, bool allow_st) {
       }
     }
   }
+  std::stable_sort(all.begin(), all.end(),
+                   [](const Choice &x, const Choice &y) { return x.cost < y.cost; });
+  return all;
+}
+
+Choice plan_full(size_t es, uint64_t M, uint64_t N, bool allow_st) {
+  Choice best;
+  best.a = M;
+  best.b = N;
+  best.stages = -1;
+  if (M <= 1 || N <= 1) return best;
+  auto key = std::make_tuple(M, N, es, lds_budget(), allow_st);
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) return it->second;
+    it = g_plan_cache.find(key);
+    if (it != g_plan_cache.end()) return it->second;
+  }
+  std::vector<Choice> all = rank_routes(es, M, N, allow_st);
+  if (!all.empty()) best = all.front();
   std::lock_guard<std::mutex> lk(g_plan_mu);
   if (g_plan_cache.size() > 65536) g_plan_cache.clear();
   g_plan_cache[key] = best;
   return best;
 }
 
-// Stage list of an M x N transpose as chosen by the planner (stages = 0).
-bool plan_transpose_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es,
-                           uint64_t M, uint64_t N, std::string &route, bool allow_st) {
-  Choice c = plan_full(es, M, N, allow_st);
+// The stage list of a planner choice.
+static bool choice_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es,
+                          uint64_t M, uint64_t N, const Choice &c, std::string &route) {
   if (c.stages == 2) return route_stages(out, buf, es, M / c.a, c.a, 1, N, 1, 2, route);
   if (c.stages == 22) return route_stages(out, buf, es, 1, M, N / c.b, c.b, 1, 22, route);
   if (c.stages == 3 || c.stages == 32)
@@ -760,6 +774,94 @@ bool plan_transpose_stages(std::vector<StageRec> &out, unsigned char *buf, size_
   if (c.stages >= 5 && c.stages <= 7)
     return route_stages(out, buf, es, 1, M, 1, N, 1, c.stages, route);
   return route_stages(out, buf, es, 1, M, 1, N, 1, -1, route);
+}
+
+// ---------------------------------------------------------------------------------------
+// Measured autotuner for full transposes: the model's cheapest candidates plus the best of
+// each route family (stripes by rows and by columns, row/column decomposition, tiled) are
+// run on the caller's buffer, each timed with events on the caller's stream and followed by
+// the inverse transpose, which restores the buffer bit for bit. The fastest becomes the
+// plan for (M, N, element size) from then on (hip_marshal_plan_transpose reports it).
+// ---------------------------------------------------------------------------------------
+static bool autotune_full(hipStream_t st, unsigned char *buf, size_t es, uint64_t M,
+                          uint64_t N, bool allow_st, Choice *out) {
+  std::vector<Choice> all = rank_routes(es, M, N, allow_st);
+  if (all.empty()) return true;
+  std::vector<Choice> cand;
+  auto add = [&](const Choice &c) {
+    for (const Choice &d : cand)
+      if (d.stages == c.stages && d.a == c.a && d.b == c.b) return;
+    cand.push_back(c);
+  };
+  for (size_t i = 0; i < all.size() && i < 3; i++) add(all[i]);
+  auto family_best = [&](auto pred) {
+    for (const Choice &c : all)
+      if (pred(c)) {
+        add(c);
+        return;
+      }
+  };
+  family_best([](const Choice &c) { return c.stages == 6; });
+  family_best([](const Choice &c) { return c.stages == 7; });
+  family_best([](const Choice &c) { return c.stages == 5; });
+  family_best([](const Choice &c) {
+    return c.stages == 2 || c.stages == 22 || c.stages == 3 || c.stages == 32;
+  });
+  // The restoring transpose and every candidate's scratch, reserved before timing.
+  std::vector<StageRec> inv;
+  std::string route;
+  if (!choice_stages(inv, buf, es, N, M, plan_full(es, N, M, allow_st), route)) return true;
+  size_t need = stages_scratch_bytes(inv);
+  std::vector<std::vector<StageRec>> lists(cand.size());
+  for (size_t i = 0; i < cand.size(); i++) {
+    if (!choice_stages(lists[i], buf, es, M, N, cand[i], route)) lists[i].clear();
+    need = std::max(need, stages_scratch_bytes(lists[i]));
+  }
+  {
+    Pool &p = pool_here();
+    std::lock_guard<std::mutex> lk(p.mu);
+    if (pool_grow(p, need)) return true;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  LM_CHECK_HIP(hipEventCreate(&e0));
+  LM_CHECK_HIP(hipEventCreate(&e1));
+  float best_ms = 1e30f;
+  size_t best = SIZE_MAX;
+  bool err = false;
+  const bool warm = double(M) * double(N) * double(es) < double(1u << 30);
+  for (size_t i = 0; i < cand.size() && !err; i++) {
+    if (lists[i].empty()) continue;
+    float ms = 0;
+    if (warm)  // first launches of a kernel form carry one-time costs: one untimed round trip
+      err = run_stages(lists[i], st) || run_stages(inv, st);
+    err = hipEventRecord(e0, st) != hipSuccess || run_stages(lists[i], st) ||
+          hipEventRecord(e1, st) != hipSuccess || run_stages(inv, st) ||
+          hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess || hipStreamSynchronize(st) != hipSuccess;
+    if (plan_debug())
+      fprintf(stderr, "[libmarshal] autotune %llux%llu es=%zu: stages %d a=%llu b=%llu %.3f ms\n",
+              (unsigned long long)M, (unsigned long long)N, es, cand[i].stages,
+              (unsigned long long)cand[i].a, (unsigned long long)cand[i].b, ms);
+    if (!err && ms < best_ms) {
+      best_ms = ms;
+      best = i;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (err || best == SIZE_MAX) return true;
+  Choice c = cand[best];
+  c.cost = double(best_ms);
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  g_tuned[std::make_tuple(M, N, es, lds_budget(), allow_st)] = c;
+  *out = c;
+  return false;
+}
+
+// Stage list of an M x N transpose as chosen by the planner (stages = 0).
+bool plan_transpose_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es,
+                           uint64_t M, uint64_t N, std::string &route, bool allow_st) {
+  return choice_stages(out, buf, es, M, N, plan_full(es, M, N, allow_st), route);
 }
 
 }  // namespace lm
@@ -781,6 +883,20 @@ bool hip_marshal_plan_transpose(int64_t M, int64_t N, size_t es, int64_t *a, int
   Choice c = plan_full(es, uint64_t(M), uint64_t(N));
   if (a) *a = int64_t(c.a);
   if (b) *b = int64_t(c.b);
+  if (stages) *stages = c.stages;
+  return false;
+}
+
+bool hip_marshal_autotune(hipStream_t st, void *buf, size_t es, int64_t M, int64_t N,
+                          int *stages) {
+  uint64_t n;
+  if (!dims_ok("hip_marshal_autotune", {M, N}, es, &n) ||
+      !aligned_ok("hip_marshal_autotune", buf, es))
+    return true;
+  Choice c = plan_full(es, uint64_t(M), uint64_t(N), true);
+  if (M > 1 && N > 1 &&
+      autotune_full(st, static_cast<unsigned char *>(buf), es, uint64_t(M), uint64_t(N), true, &c))
+    return true;
   if (stages) *stages = c.stages;
   return false;
 }
@@ -890,6 +1006,19 @@ bool hip_transpose_es(hipStream_t st, void *src, size_t es, int64_t A, int64_t a
   unsigned char *buf = static_cast<unsigned char *>(src);
   std::vector<StageRec> list;
   std::string route;
+  if (stages == 0 && tuning("autotune")) {  // measure once per shape, on this buffer
+    const uint64_t M = uint64_t(A) * a, N = uint64_t(B) * b;
+    bool tuned;
+    {
+      std::lock_guard<std::mutex> lk(g_plan_mu);
+      tuned = g_tuned.count(std::make_tuple(M, N, es, lds_budget(), true)) != 0;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hipStreamIsCapturing(st, &cs);
+    Choice c;
+    if (!tuned && cs == hipStreamCaptureStatusNone && M > 1 && N > 1)
+      autotune_full(st, buf, es, M, N, true, &c);
+  }
   bool ok = stages == 0
                 ? plan_transpose_stages(list, buf, es, uint64_t(A) * a, uint64_t(B) * b, route)
                 : route_stages(list, buf, es, A, a, B, b, R, stages, route);

@@ -386,3 +386,18 @@ def test_aos_asta_large_tiles_split(hip, orc, es):
         check(hip.aos_asta, x, orc.op_010(x, h // t, t, w, es=es), h, w, t, elem_bytes=es)
         y = orc.op_010(x, h // t, t, w, es=es)
         check(hip.asta_aos, y, x, h, w, t, elem_bytes=es)
+
+
+def test_autotune_restores_buffer_and_sets_plan(hip, orc):
+    """The measured autotuner times candidate routes in place: the buffer must come back
+    bit-identical, the chosen stage code becomes the plan, and the planned transpose matches
+    the oracle."""
+    for (M, N, es) in ((1000, 1000, 8), (1031, 997, 1), (1800, 7200, 4), (333, 4099, 16)):
+        say("autotune", M, N, es)
+        x = orc.tagged(M * N, es, seed=M + N + es)
+        d = dev(x)
+        stages = hip.autotune(d, M, N, elem_bytes=es)
+        assert np.array_equal(host(d), x)
+        assert hip.plan_transpose(M, N, es)[2] == stages
+        assert hip.transpose_full(d, M, N, elem_bytes=es) is False
+        assert np.array_equal(host(d), orc.op_transpose(x, M, N, es=es))
