@@ -45,11 +45,16 @@ def parse():
     return p.parse_args()
 
 
-def time_calls(fn, reps, stream):
+def time_calls(fn, reps, stream, warm=True):
     """Average device time of fn() over `reps` back-to-back calls, bracketed by one pair of
-    HIP events on `stream` (the stream the library launches on)."""
+    HIP events on `stream` (the stream the library launches on). With `warm`, one untimed
+    call first (plans the route and sizes the scratch pool outside the timed region)."""
     import torch
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    if warm:
+        if fn():
+            raise RuntimeError("library call failed")
+        stream.synchronize()
     with torch.cuda.stream(stream):
         e0.record(stream)
         for _ in range(reps):
@@ -111,8 +116,8 @@ def extra_batch(lm, reps=3):
             return False
         if batch.run() or plain():
             raise RuntimeError("batch warm-up failed")
-        sb = time_calls(batch.run, reps, stream)
-        sp = time_calls(plain, 2, stream)
+        sb = time_calls(batch.run, reps, stream, warm=False)
+        sp = time_calls(plain, 2, stream, warm=False)
         out = {"matrices": len(sel), "gbytes": round(total / 1e9, 2),
                "kernels_per_launch": batch.info("kernels"), "solo": batch.info("solo"),
                "batched_GBs": round(2 * total / sb / 1e9, 1), "batched_ms": round(sb * 1e3, 2),
@@ -190,10 +195,10 @@ def extra_large(lm, M, N, es):
     for _ in range(1):  # untimed: plans both directions and sizes the scratch pool
         if lm.transpose_full(x, M, N) or lm.transpose_full(x, N, M):
             raise RuntimeError("large transpose failed")
-    fwd = time_calls(lambda: lm.transpose_full(x, M, N), 1, stream)
+    fwd = time_calls(lambda: lm.transpose_full(x, M, N), 1, stream, warm=False)
     route = lm.last_route()
     ok = torch.equal(x[c * M + r], ref)
-    back = time_calls(lambda: lm.transpose_full(x, N, M), 1, stream)
+    back = time_calls(lambda: lm.transpose_full(x, N, M), 1, stream, warm=False)
     route_back = lm.last_route()
     ok = ok and torch.equal(x[idx], ref)
     del x, idx, ref, r, c

@@ -20,13 +20,17 @@
  * With profiling off (hip_marshal_set_profiling(0)) calls are fully stream-ordered and never
  * synchronize, and `elapsed_ns` is left untouched.
  *
- * Scratch: cycle-following stages need visited-bit flags. They come from a per-device pool
- * owned by the library that only grows (hipMalloc happens on growth, never per call once
- * the pool is large enough; hip_marshal_reserve() pre-sizes it). Flags are cleared with a
- * fill kernel on the caller's stream. Calls that overlap on two streams of one device must
- * not share the pool: the library serializes pool use per device with a mutex held until
- * the call's last launch has been enqueued, and the flags of one call live in a region no
- * concurrent call can be handed (see hip_marshal_reserve()).
+ * Scratch: cycle-following stages need visited-bit flags, and the stripe route a buffer of
+ * one stripe (at most "st_scratch_mb", 256 MiB by default, plus its bits). Both come from
+ * one per-device pool owned by the library that only grows (hipMalloc and hipFree happen on
+ * growth only, never per call once the pool is large enough; hip_marshal_reserve()
+ * pre-sizes it). Flags are cleared with a fill kernel on the caller's stream. A stage holds
+ * the pool's mutex while it enqueues, so concurrent host threads take turns; when a stage
+ * is enqueued on a different stream than the pool's previous user, that stream first waits
+ * (hipStreamWaitEvent) for everything the previous user had enqueued, so two calls on two
+ * streams never use the pool at the same time on the GPU - their pool-using stages run one
+ * after the other. Under stream capture the cross-stream wait is not recorded, so a
+ * captured graph must not run concurrently with other pool users on the same device.
  */
 #ifndef LIBMARSHAL_AMD_HIP_MARSHAL_H_
 #define LIBMARSHAL_AMD_HIP_MARSHAL_H_
