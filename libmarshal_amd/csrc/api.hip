@@ -87,6 +87,7 @@ Knob g_knobs[] = {
     {"st_pt_vw", {16}},
     {"st_rotate", {1}, 0},
     {"autotune", {0}, 0},
+    {"pad_shape", {0}, 0},
 };
 Knob *find_knob(const char *key) {
   if (!key) return nullptr;

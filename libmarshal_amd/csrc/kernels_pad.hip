@@ -26,17 +26,16 @@ constexpr int kPadThreads = 256;
 constexpr int kPadPer = 16;  // elements per thread
 constexpr uint64_t kPadTile = uint64_t(kPadThreads) * kPadPer;
 constexpr uint32_t kLoaded = 1;
-constexpr int kPadVecPer = 8;  // 16-byte vectors per lane in the 16-byte form
-constexpr uint64_t kPadVecTile = uint64_t(kPadThreads) * kPadVecPer;
-constexpr uint32_t kPadVecTileLog2 = 11;
-static_assert(kPadVecTile == (uint64_t(1) << kPadVecTileLog2), "group size");
+// 16-byte form: NT lanes x VPL vectors per group (NT*VPL a power of two, >= 1024); the
+// state words are sized for the smallest group, 1024 vectors (16 KiB).
+constexpr uint64_t kPadVecMinTile = 1024;
 static_assert(kPadTile == 4096, "element-form group size is 2^12");
 
 // Enough state words for either form: the element form's groups of kPadTile elements and the
-// 16-byte form's groups of kPadVecTile vectors.
+// 16-byte form's groups of at least kPadVecMinTile vectors.
 uint64_t pad_flag_words(uint64_t total_elems, size_t es) {
   const uint64_t a = (total_elems + kPadTile - 1) / kPadTile;
-  const uint64_t b = (total_elems * es + kPadVecTile * 16 - 1) / (kPadVecTile * 16);
+  const uint64_t b = (total_elems * es + kPadVecMinTile * 16 - 1) / (kPadVecMinTile * 16);
   return 1 + std::max(a, b);
 }
 
@@ -169,8 +168,11 @@ __global__ __launch_bounds__(kPadThreads) void k_pad(PadParams p) {
 // two aligned 16-byte chunks that cover it and shifted into place (v_alignbyte). Vectors that
 // touch a row end, the pad, or the end of the buffer are done element by element.
 
-template <int ES>
-__global__ __launch_bounds__(kPadThreads) void k_pad_vec(PadParams p) {
+template <int ES, int NT, int VPL>
+__global__ __launch_bounds__(NT) void k_pad_vec(PadParams p) {
+  constexpr uint64_t kPadVecTile = uint64_t(NT) * VPL;
+  constexpr uint32_t kPadVecTileLog2 = __builtin_ctzll(kPadVecTile);
+  static_assert((kPadVecTile & (kPadVecTile - 1)) == 0 && kPadVecTile >= kPadVecMinTile, "");
   using E = typename Elem<ES>::T;
   constexpr uint64_t EPV = 16 / ES;
   union Vec {
@@ -193,15 +195,15 @@ __global__ __launch_bounds__(kPadThreads) void k_pad_vec(PadParams p) {
   const E *m = reinterpret_cast<const E *>(p.base);
   const uint64_t rowlen = p.unpad ? X : P;
   const uint64_t src_pitch = p.unpad ? P : X;
-  Vec v[kPadVecPer];
-  U128 wlo[kPadVecPer], whi[kPadVecPer];
-  uint64_t wsb[kPadVecPer];
-  bool win[kPadVecPer];
+  Vec v[VPL];
+  U128 wlo[VPL], whi[VPL];
+  uint64_t wsb[VPL];
+  bool win[VPL];
   // Issue every load first (window chunks, or single elements near row ends and the pad),
   // then shift the windows into place: the loads of all vectors overlap.
 #pragma unroll
-  for (int j = 0; j < kPadVecPer; j++) {
-    const uint64_t vi = lo + threadIdx.x + uint64_t(j) * kPadThreads;
+  for (int j = 0; j < VPL; j++) {
+    const uint64_t vi = lo + threadIdx.x + uint64_t(j) * NT;
     v[j].v = U128{0u, 0u, 0u, 0u};
     win[j] = false;
     wsb[j] = 0;
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(kPadThreads) void k_pad_vec(PadParams p) {
     }
   }
 #pragma unroll
-  for (int j = 0; j < kPadVecPer; j++)
+  for (int j = 0; j < VPL; j++)
     if (win[j]) v[j].v = window_join(wsb[j], wlo[j], whi[j]);
   wait_vm_loads();
   __syncthreads();
@@ -237,8 +239,8 @@ __global__ __launch_bounds__(kPadThreads) void k_pad_vec(PadParams p) {
                  nvec, glo, ghi);
   pad_wait_readers(p.flags, k, glo, ghi);
 #pragma unroll
-  for (int j = 0; j < kPadVecPer; j++) {
-    const uint64_t vi = lo + threadIdx.x + uint64_t(j) * kPadThreads;
+  for (int j = 0; j < VPL; j++) {
+    const uint64_t vi = lo + threadIdx.x + uint64_t(j) * NT;
     if (vi >= hi) continue;
     if ((vi + 1) * 16 <= bytes) {
       *reinterpret_cast<U128 *>(p.base + vi * 16) = v[j].v;
@@ -249,6 +251,26 @@ __global__ __launch_bounds__(kPadThreads) void k_pad_vec(PadParams p) {
     }
   }
 }
+
+namespace {
+template <int ES, int NT, int VPL>
+hipError_t launch_pad_vec_t(const PadParams &p, uint64_t nvec, hipStream_t st) {
+  const uint64_t tile = uint64_t(NT) * VPL;
+  hipLaunchKernelGGL((k_pad_vec<ES, NT, VPL>), dim3(uint32_t((nvec + tile - 1) / tile)),
+                     dim3(NT), 0, st, p);
+  return hipGetLastError();
+}
+template <int ES>
+hipError_t launch_pad_vec(const PadParams &p, uint64_t nvec, int64_t shape, hipStream_t st) {
+  switch (shape) {
+    case 1: return launch_pad_vec_t<ES, 256, 4>(p, nvec, st);
+    case 2: return launch_pad_vec_t<ES, 256, 8>(p, nvec, st);
+    case 3: return launch_pad_vec_t<ES, 1024, 2>(p, nvec, st);
+    case 4: return launch_pad_vec_t<ES, 1024, 4>(p, nvec, st);
+    default: return launch_pad_vec_t<ES, 512, 4>(p, nvec, st);
+  }
+}
+}  // namespace
 
 hipError_t launch_padding(unsigned char *base, size_t es, uint64_t x, uint64_t y, uint64_t pad,
                           bool unpad, uint32_t *flags, hipStream_t st) {
@@ -268,16 +290,17 @@ hipError_t launch_padding(unsigned char *base, size_t es, uint64_t x, uint64_t y
   if (e != hipSuccess) return e;
   if ((reinterpret_cast<uintptr_t>(base) & 15u) == 0) {
     const uint64_t nvec = (p.total * es + 15) / 16;
-    dim3 grid(uint32_t((nvec + kPadVecTile - 1) / kPadVecTile)), blk(kPadThreads);
+    // lanes x vectors per lane: 0: 512 x 4 (default), 1: 256 x 4, 2: 256 x 8,
+    // 3: 1024 x 2, 4: 1024 x 4
+    const int64_t shape = tuning("pad_shape");
     switch (es) {
-      case 1: hipLaunchKernelGGL(k_pad_vec<1>, grid, blk, 0, st, p); break;
-      case 2: hipLaunchKernelGGL(k_pad_vec<2>, grid, blk, 0, st, p); break;
-      case 4: hipLaunchKernelGGL(k_pad_vec<4>, grid, blk, 0, st, p); break;
-      case 8: hipLaunchKernelGGL(k_pad_vec<8>, grid, blk, 0, st, p); break;
-      case 16: hipLaunchKernelGGL(k_pad_vec<16>, grid, blk, 0, st, p); break;
+      case 1: return launch_pad_vec<1>(p, nvec, shape, st);
+      case 2: return launch_pad_vec<2>(p, nvec, shape, st);
+      case 4: return launch_pad_vec<4>(p, nvec, shape, st);
+      case 8: return launch_pad_vec<8>(p, nvec, shape, st);
+      case 16: return launch_pad_vec<16>(p, nvec, shape, st);
       default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
   }
   dim3 grid(uint32_t(p.ntiles)), blk(kPadThreads);
   switch (es) {

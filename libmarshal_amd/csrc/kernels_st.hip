@@ -137,7 +137,50 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
         *reinterpret_cast<U128 *>(dst + (c0 + c) * p.dst_pitch + r0 * ES + (j << 4)) = x;
       }
     }
-  } else {
+  } else if (ES <= 2) {
+    // Destination rows not 16-byte aligned: per row, the elements before the first aligned
+    // address and after the last whole vector are stored one by one (lane k == nb of the
+    // row); the vectors in between are aligned 16-byte stores, each shifted together from
+    // the two LDS chunks it straddles (lanes k < nb).
+    const uint32_t vl = p.tr_log2 > EPV_LOG2 ? p.tr_log2 - EPV_LOG2 + 1 : 1;  // >= nb + 1
+    const uint32_t vpr = 1u << vl;
+    for (uint32_t i = threadIdx.x; i < (TC << vl); i += NT) {
+      const uint32_t c = i >> vl, k = i & (vpr - 1u);
+      if (c >= nc) continue;
+      unsigned char *row = dst + (c0 + c) * p.dst_pitch + r0 * ES;
+      const uint32_t L = nr * ES;                                         // bytes in the row
+      const uint32_t head = uint32_t((16u - (reinterpret_cast<uintptr_t>(row) & 15u)) & 15u);
+      const uint32_t nb = L > head ? (L - head) / 16 : 0;                 // whole vectors
+      const uint32_t m = oop_mask<ES, SW>(c);
+      const uint32_t s = (m >> 2) & 3u;
+      const unsigned char *lrow = lds + c * PB;
+      auto chunk = [&](uint32_t j) {  // logical 16-byte chunk j of this LDS row
+        const U128 y = *reinterpret_cast<const U128 *>(lrow + ((j << 4) ^ (m & ~15u)));
+        U128 x;
+        x.x = (s == 0) ? y.x : (s == 1) ? y.y : (s == 2) ? y.z : y.w;
+        x.y = (s == 0) ? y.y : (s == 1) ? y.x : (s == 2) ? y.w : y.z;
+        x.z = (s == 0) ? y.z : (s == 1) ? y.w : (s == 2) ? y.x : y.y;
+        x.w = (s == 0) ? y.w : (s == 1) ? y.z : (s == 2) ? y.y : y.x;
+        return x;
+      };
+      if (k < nb) {
+        const uint32_t o = head + 16 * k;  // row byte offset of this vector
+        const U128 lo = chunk(o >> 4);
+        U128 hi = lo;
+        if (o & 15u) hi = chunk((o >> 4) + 1);  // o + 15 < L: still inside the row
+        *reinterpret_cast<U128 *>(row + o) = window_join(o, lo, hi);
+      } else if (k == nb) {
+        const uint32_t tail0 = head + 16 * nb;
+        for (uint32_t b = 0; b < L; b += ES) {
+          if (b >= head && b < tail0) b = tail0;
+          if (b >= L) break;
+          const uint32_t r = b / ES;
+          *reinterpret_cast<E *>(row + b) =
+              *reinterpret_cast<const E *>(lrow + ((r * ES) ^ m));
+        }
+      }
+    }
+  } else {  // 4- to 8-byte elements: element stores already fill whole lines per row
     for (uint32_t i = threadIdx.x; i < (TC << p.tr_log2); i += NT) {
       const uint32_t c = i >> p.tr_log2, r = i & (TR - 1u);
       if (c < nc && r < nr)
