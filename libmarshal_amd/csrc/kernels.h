@@ -141,8 +141,9 @@ struct StPlan {
 };
 // False when no stripe height keeps one stripe within `scratch_cap` bytes or the striped
 // dimension holds fewer than two stripes.
-// rotate: each stripe is transposed into its neighbour's slot (no copy pass) and the 100
-// folds the one-slot rotation into its map; otherwise stripes go through scratch in groups.
+// rotate: when the stripes need more than one scratch fill, each stripe is transposed into
+// its neighbour's slot (no copy pass) and the 100 folds the one-slot rotation into its map;
+// otherwise (or with rotate false) stripes go through scratch in groups.
 bool plan_st(size_t es, uint64_t M, uint64_t N, int variant, uint64_t target_bytes,
              uint64_t scratch_cap, int64_t pt_vw, bool rotate, StPlan *out);
 // `scratch` must hold plan.scratch_bytes and not overlap `buf`.

@@ -389,8 +389,10 @@ bool plan_st(size_t es, uint64_t M, uint64_t N, int variant, uint64_t target_byt
   p.q = D / h;
   p.r = D - p.q * h;
   const uint64_t stripe = h * L * es;
-  p.rotate = rotate;
-  p.group = rotate ? 1 : std::max<uint64_t>(1, std::min<uint64_t>(p.q, scratch_cap / stripe));
+  // Rotation saves the copy pass but costs one launch per stripe: worth it once the stripes
+  // need more than one scratch fill; smaller matrices take one copy and one transpose launch.
+  p.rotate = rotate && p.q * stripe > scratch_cap;
+  p.group = p.rotate ? 1 : std::max<uint64_t>(1, std::min<uint64_t>(p.q, scratch_cap / stripe));
   if (p.q > 1) {
     if (variant == 0) plan_100(es, 1, p.q, N, h, &p.pt, pt_vw);
     else plan_100(es, 1, M, p.q, h, &p.pt, pt_vw);

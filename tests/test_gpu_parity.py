@@ -350,18 +350,21 @@ def test_stripe_route(hip, orc, es):
     """stages = 6 / 7: stripes of rows / of columns through bounded scratch, with and without a
     ragged remainder (prime dimensions included), several scratch fills per call."""
     shapes = [(768, 131), (131, 768), (1031, 997), (640, 4099), (4099, 640), (4096, 3001)]
-    keep = {k: hip.get_tuning(k) for k in ("st_super_bytes", "st_scratch_mb")}
+    keep = {k: hip.get_tuning(k) for k in ("st_super_bytes", "st_scratch_mb", "st_rotate")}
     try:
-        for super_bytes, scratch_mb in ((64, 1), (2048, 256)):
+        # small scratch: rotated stripes, or (rotation off) several scratch fills; large
+        # scratch: one fill
+        for super_bytes, scratch_mb, rotate in ((64, 1, 1), (64, 1, 0), (2048, 256, 1)):
             hip.set_tuning("st_super_bytes", super_bytes)
             hip.set_tuning("st_scratch_mb", scratch_mb)
+            hip.set_tuning("st_rotate", rotate)
             for (M, N) in shapes:
                 if M * N * es > 100 << 20:
                     continue
                 x = orc.tagged(M * N, es, seed=M * 3 + N + es)
                 want = orc.op_transpose(x, M, N, es=es)
                 for stages, tag in ((6, "ST_R-"), (7, "ST_C-")):
-                    say("stripes", es, M, N, stages, super_bytes, scratch_mb)
+                    say("stripes", es, M, N, stages, super_bytes, scratch_mb, rotate)
                     t = dev(x)
                     assert hip.transpose(t, 1, M, 1, N, 1, stages, elem_bytes=es) is False
                     route = hip.last_route()
