@@ -39,9 +39,22 @@ struct BatchPlan {
   double bytes = 0;         // algorithmic bytes of one run (2 x every matrix)
   std::vector<std::string> routes;
   // Matrices whose best route is the stripe route (no grouped form): their plain stage lists,
-  // run one after another on a second stream, overlapping the grouped kernels.
+  // dealt over kSoloLanes side streams (balanced by bytes) and run one after another on each,
+  // overlapping the grouped kernels and each other.
   std::vector<std::vector<StageRec>> solo;
-  size_t solo_scratch = 0;  // bytes of scratch the solo lists need (one at a time)
+  std::vector<int> solo_lane;
+  std::vector<double> solo_bytes;
+  size_t solo_scratch[4] = {0, 0, 0, 0};  // per lane: the most any of its lists needs
+  int solo_lanes = 0;                     // lanes in use
+};
+constexpr int kSoloLanes = 4;
+
+// Side streams, events and scratch for the solo lists of a batch.
+struct SoloExec {
+  hipStream_t st[kSoloLanes] = {};
+  hipEvent_t join[kSoloLanes] = {};
+  unsigned char *scr[kSoloLanes] = {};
+  hipEvent_t fork = nullptr;
 };
 
 // allow_solo: matrices best served by the stripe route go to `solo` (the batched form);
@@ -54,10 +67,9 @@ bool plan_desc(const hip_marshal_desc &d, std::vector<StageRec> &out, std::strin
 // Host image of the parameter buffer, with the flag offsets turned into pointers into `flags`.
 void build_image(const BatchPlan &b, const uint32_t *flags, std::vector<unsigned char> &host);
 // Clears the flags and launches every group, level by level, on `st`. Solo lists (if any)
-// run on `side` (ordered after the work already on `st`, which waits for them at the end)
-// with `solo_scratch`; fork / join are the events ev_fork / ev_join.
+// run on the side streams of `solo` (ordered after the work already on `st`, which waits for
+// all of them at the end).
 bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_t st,
-                   hipStream_t side = nullptr, hipEvent_t ev_fork = nullptr,
-                   hipEvent_t ev_join = nullptr, unsigned char *solo_scratch = nullptr);
+                   const SoloExec *solo = nullptr);
 
 }  // namespace lm

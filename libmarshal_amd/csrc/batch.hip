@@ -307,16 +307,20 @@ const void *lds_func(int ub, int upl) {
 }  // namespace
 
 bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_t st,
-                   hipStream_t side, hipEvent_t ev_fork, hipEvent_t ev_join,
-                   unsigned char *solo_scratch) {
+                   const SoloExec *solo) {
   const bool fork = !b.solo.empty();
   if (fork) {
-    if (!side || !ev_fork || !ev_join) return true;
-    LM_CHECK_HIP(hipEventRecord(ev_fork, st));
-    LM_CHECK_HIP(hipStreamWaitEvent(side, ev_fork, 0));
-    for (const auto &list : b.solo)
-      if (run_stages_on(list, side, solo_scratch)) return true;
-    LM_CHECK_HIP(hipEventRecord(ev_join, side));
+    if (!solo || !solo->fork) return true;
+    LM_CHECK_HIP(hipEventRecord(solo->fork, st));
+    for (int l = 0; l < b.solo_lanes; l++) {
+      if (!solo->st[l] || !solo->join[l]) return true;
+      LM_CHECK_HIP(hipStreamWaitEvent(solo->st[l], solo->fork, 0));
+    }
+    for (size_t i = 0; i < b.solo.size(); i++) {
+      const int l = b.solo_lane[i];
+      if (run_stages_on(b.solo[i], solo->st[l], solo->scr[l])) return true;
+    }
+    for (int l = 0; l < b.solo_lanes; l++) LM_CHECK_HIP(hipEventRecord(solo->join[l], solo->st[l]));
   }
   if (b.flag_words) LM_CHECK_HIP(launch_fill_u32(flags, b.flag_words, 0, st));
   unsigned char *base = static_cast<unsigned char *>(params);
@@ -330,7 +334,8 @@ bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_
       LM_CHECK_HIP(hipLaunchKernel(g.func, dim3(g.grid), dim3(g.block), args, g.lds, st));
     }
   }
-  if (fork) LM_CHECK_HIP(hipStreamWaitEvent(st, ev_join, 0));
+  if (fork)
+    for (int l = 0; l < b.solo_lanes; l++) LM_CHECK_HIP(hipStreamWaitEvent(st, solo->join[l], 0));
   return false;
 }
 
@@ -398,8 +403,8 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
       std::vector<StageRec> list;
       std::string r2;
       if (plan_desc(d, list, r2, true) && r2.find("ST_") != std::string::npos) {
-        b.solo_scratch = std::max(b.solo_scratch, stages_scratch_bytes(list));
         b.solo.push_back(std::move(list));
+        b.solo_bytes.push_back(double(d.rows) * double(d.cols) * double(d.elem_bytes));
         b.routes[i] = r2;
         b.stages += 1;
         b.bytes += 2.0 * double(d.rows) * double(d.cols) * double(d.elem_bytes);
@@ -415,6 +420,24 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
     b.routes[i] = route;
     b.stages += int64_t(per[i].size());
     b.bytes += 2.0 * double(h) * double(w) * double(es);
+  }
+  // Solo lists: largest first onto the least loaded lane.
+  if (!b.solo.empty()) {
+    std::vector<size_t> order(b.solo.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return b.solo_bytes[x] > b.solo_bytes[y]; });
+    double load[kSoloLanes] = {0, 0, 0, 0};
+    b.solo_lane.assign(b.solo.size(), 0);
+    b.solo_lanes = int(std::min<size_t>(kSoloLanes, b.solo.size()));
+    for (size_t i : order) {
+      int l = 0;
+      for (int k = 1; k < b.solo_lanes; k++)
+        if (load[k] < load[l]) l = k;
+      load[l] += b.solo_bytes[i];
+      b.solo_lane[i] = l;
+      b.solo_scratch[l] = std::max(b.solo_scratch[l], stages_scratch_bytes(b.solo[i]));
+    }
   }
   // 2. flag space for every global-bit stage
   size_t maxlev = 0;
@@ -555,9 +578,7 @@ struct hip_marshal_batch {
   BatchPlan b;
   void *params = nullptr;     // device copy of all item arrays and prefix sums
   uint32_t *flags = nullptr;  // visited bits of every global-bit stage
-  unsigned char *solo_scr = nullptr;  // scratch of the stripe-route matrices
-  hipStream_t side = nullptr;         // their stream, forked from and joined to the run's
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  SoloExec solo;  // side streams, events and scratch of the stripe-route matrices
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;
@@ -581,11 +602,14 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
   }
   bool ok = true;
   if (!b.solo.empty()) {
-    ok = hipStreamCreateWithFlags(&hb->side, hipStreamNonBlocking) == hipSuccess &&
-         hipEventCreateWithFlags(&hb->ev_fork, hipEventDisableTiming) == hipSuccess &&
-         hipEventCreateWithFlags(&hb->ev_join, hipEventDisableTiming) == hipSuccess;
-    if (ok && b.solo_scratch)
-      ok = hipMalloc(reinterpret_cast<void **>(&hb->solo_scr), b.solo_scratch) == hipSuccess;
+    ok = hipEventCreateWithFlags(&hb->solo.fork, hipEventDisableTiming) == hipSuccess;
+    for (int l = 0; ok && l < b.solo_lanes; l++) {
+      ok = hipStreamCreateWithFlags(&hb->solo.st[l], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&hb->solo.join[l], hipEventDisableTiming) == hipSuccess;
+      if (ok && b.solo_scratch[l])
+        ok = hipMalloc(reinterpret_cast<void **>(&hb->solo.scr[l]), b.solo_scratch[l]) ==
+             hipSuccess;
+    }
   }
   if (b.flag_words) ok = hipMalloc(&hb->flags, b.flag_words * 4) == hipSuccess;
   if (ok && b.param_bytes) ok = hipMalloc(&hb->params, b.param_bytes) == hipSuccess;
@@ -617,8 +641,7 @@ bool hip_marshal_batch_run(hip_marshal_batch *hb, hipStream_t st, uint64_t *elap
       hipStream_t cs = nullptr;
       LM_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
       hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-      bool err = e != hipSuccess || enqueue_batch(b, hb->params, hb->flags, cs, hb->side,
-                                                  hb->ev_fork, hb->ev_join, hb->solo_scr);
+      bool err = e != hipSuccess || enqueue_batch(b, hb->params, hb->flags, cs, &hb->solo);
       hipGraph_t g = nullptr;
       if (e == hipSuccess) e = hipStreamEndCapture(cs, &g);
       hipStreamDestroy(cs);
@@ -631,8 +654,7 @@ bool hip_marshal_batch_run(hip_marshal_batch *hb, hipStream_t st, uint64_t *elap
       LM_CHECK_HIP(hipGraphInstantiate(&hb->exec, hb->graph, nullptr, nullptr, 0));
     }
     LM_CHECK_HIP(hipGraphLaunch(hb->exec, st));
-  } else if (enqueue_batch(b, hb->params, hb->flags, st, hb->side, hb->ev_fork, hb->ev_join,
-                           hb->solo_scr)) {
+  } else if (enqueue_batch(b, hb->params, hb->flags, st, &hb->solo)) {
     return true;
   }
   if (elapsed_ns) {
@@ -674,10 +696,13 @@ void hip_marshal_batch_destroy(hip_marshal_batch *hb) {
   if (hb->graph) hipGraphDestroy(hb->graph);
   if (hb->params) hipFree(hb->params);
   if (hb->flags) hipFree(hb->flags);
-  if (hb->solo_scr) hipFree(hb->solo_scr);
-  if (hb->ev_fork) hipEventDestroy(hb->ev_fork);
-  if (hb->ev_join) hipEventDestroy(hb->ev_join);
-  if (hb->side) hipStreamDestroy(hb->side);
+  for (int l = 0; l < kSoloLanes; l++) {
+    if (hb->solo.st[l]) hipStreamSynchronize(hb->solo.st[l]);
+    if (hb->solo.scr[l]) hipFree(hb->solo.scr[l]);
+    if (hb->solo.join[l]) hipEventDestroy(hb->solo.join[l]);
+    if (hb->solo.st[l]) hipStreamDestroy(hb->solo.st[l]);
+  }
+  if (hb->solo.fork) hipEventDestroy(hb->solo.fork);
   delete hb;
 }
 
