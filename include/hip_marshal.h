@@ -252,7 +252,8 @@ bool hip_marshal_autotune(hipStream_t stream, void *buf, size_t elem_bytes, int6
 int64_t hip_marshal_get_tuning(const char *key);
 /* The planner's choice for an M x N transpose of elem_bytes elements: tile factors a | M,
  * b | N and the stage sequence (2, 22, 3, 32; 5 = row/column shuffles, 6/7 = stripes of
- * rows/columns through bounded scratch; -1 = element-granular fallback). Host-only
+ * rows/columns through bounded scratch, 8 = square tile-pair swaps; -1 = element-granular
+ * fallback). Host-only
  * (no device work). Replaces the tile heuristic of the reference's test
  * (test/cl_unittest.cc:396-495, "Alg. 5" and "Alg. 3"). */
 bool hip_marshal_plan_transpose(int64_t M, int64_t N, size_t elem_bytes, int64_t *a,

@@ -482,6 +482,19 @@ bool route_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es, uin
         return true;
       }
       break;
+    case 8:  // square: one pass of tile-pair swaps (no counterpart)
+      if (A * a == B * b && sq_fits(es, A * a)) {
+        StageRec r{};
+        r.kind = StageKind::SQ;
+        r.es = es;
+        r.buf = buf;
+        r.d1 = A * a;
+        r.d2 = B * b;
+        out.push_back(r);
+        route += "SQ-";
+        return true;
+      }
+      break;
     case 6:  // stripes of rows / of columns through bounded scratch (no counterpart)
     case 7:
       if (push_st(out, buf, es, A * a, B * b, stages - 6)) {
@@ -548,6 +561,9 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
                                st));
         break;
       }
+      case StageKind::SQ:
+        LM_CHECK_HIP(launch_sq_transpose(r.buf, r.es, r.d1, st));
+        break;
       case StageKind::ST: {
         std::unique_ptr<Lease> lease;
         unsigned char *buf = scr;
@@ -646,6 +662,7 @@ double stage_bw_tbs(const StageRec &r) {
     case StageKind::RC_ROW: return r.rc.ub >= 4 ? 3.0 : r.rc.ub == 2 ? 2.0 : 1.2;
     case StageKind::RC_COL1:
     case StageKind::RC_COL2: return r.rc.ub >= 4 ? 0.55 : r.rc.ub == 2 ? 0.4 : 0.3;
+    case StageKind::SQ: return es >= 4 ? 5.0 : es == 2 ? 4.0 : 3.0;
     case StageKind::ST: {  // copy + out-of-place transpose + 100 (+ padding and a 2-D copy)
       const StPlan &p = r.stp;
       StageRec pt{};
@@ -709,6 +726,7 @@ static std::vector<Choice> rank_routes(size_t es, uint64_t M, uint64_t N, bool a
     if (route.find("Fallback") != std::string::npos && stages != -1) return;
     if (route.find("RC-") == std::string::npos && stages == 5) return;
     if (route.find("ST_") == std::string::npos && (stages == 6 || stages == 7)) return;
+    if (route.find("SQ-") == std::string::npos && stages == 8) return;
     double c = route_cost(v);
     if (plan_debug())
       fprintf(stderr, "[libmarshal] plan %llux%llu es=%zu: stages %d a=%llu b=%llu %s cost %.4g\n",
@@ -723,6 +741,7 @@ static std::vector<Choice> rank_routes(size_t es, uint64_t M, uint64_t N, bool a
   };
   consider(1, M, 1, N, -1);  // element-granular fallback
   consider(1, M, 1, N, 5);   // row / column decomposition
+  if (allow_st && M == N) consider(1, M, 1, N, 8);  // square: tile-pair swaps
   if (allow_st) {            // stripes through bounded scratch
     consider(1, M, 1, N, 6);
     consider(1, M, 1, N, 7);
@@ -772,7 +791,7 @@ static bool choice_stages(std::vector<StageRec> &out, unsigned char *buf, size_t
   if (c.stages == 22) return route_stages(out, buf, es, 1, M, N / c.b, c.b, 1, 22, route);
   if (c.stages == 3 || c.stages == 32)
     return route_stages(out, buf, es, M / c.a, c.a, N / c.b, c.b, 1, c.stages, route);
-  if (c.stages >= 5 && c.stages <= 7)
+  if (c.stages >= 5 && c.stages <= 8)
     return route_stages(out, buf, es, 1, M, 1, N, 1, c.stages, route);
   return route_stages(out, buf, es, 1, M, 1, N, 1, -1, route);
 }

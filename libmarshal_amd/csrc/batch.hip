@@ -402,7 +402,8 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
     if (allow_solo && d.op == HIP_MARSHAL_OP_TRANSPOSE) {
       std::vector<StageRec> list;
       std::string r2;
-      if (plan_desc(d, list, r2, true) && r2.find("ST_") != std::string::npos) {
+      if (plan_desc(d, list, r2, true) &&
+          (r2.find("ST_") != std::string::npos || r2.find("SQ-") != std::string::npos)) {
         b.solo.push_back(std::move(list));
         b.solo_bytes.push_back(double(d.rows) * double(d.cols) * double(d.elem_bytes));
         b.routes[i] = r2;
@@ -509,6 +510,7 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
           break;
         }
         case StageKind::ST:  // never planned for a batch (allow_st = false)
+        case StageKind::SQ:
           fprintf(stderr, "[libmarshal] %s: stripe route in a batch\n", who);
           return false;
       }

@@ -128,6 +128,10 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
                                 uint64_t dst_stride, size_t tile_bytes, hipStream_t st,
                                 uint64_t src_end = 0);
 
+// ---- square transpose: one pass of tile-pair swaps ----------------------------------------
+bool sq_fits(size_t es, uint64_t n);
+hipError_t launch_sq_transpose(unsigned char *base, size_t es, uint64_t n, hipStream_t st);
+
 // ---- stripe route (full transpose through bounded scratch) -------------------------------
 struct StPlan {
   int variant;             // 0: stripes of h rows (M = q*h + r); 1: of h columns (N = q*h + r)
@@ -162,7 +166,7 @@ int64_t tuning(const char *key);
 
 // ---- stage records (a route as a list of elementary stages) -----------------------------
 enum class M010 { Auto, BS, PTTWAC, Global };
-enum class StageKind : int { BS010, PT010, PT100G, PT0100L, RC_ROW, RC_COL1, RC_COL2, ST };
+enum class StageKind : int { BS010, PT010, PT100G, PT0100L, RC_ROW, RC_COL1, RC_COL2, ST, SQ };
 struct StageRec {
   StageKind kind;
   size_t es;

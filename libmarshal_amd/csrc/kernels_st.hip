@@ -23,6 +23,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace lm {
 
@@ -188,6 +189,137 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
             *reinterpret_cast<const E *>(lds + c * PB + ((r * ES) ^ oop_mask<ES, SW>(c)));
     }
   }
+}
+
+// ---- square in-place transpose: swap tile pairs ------------------------------------------
+// For M == N the transpose is a set of independent swaps: tile (I, J) with its mirror (J, I).
+// One workgroup loads both tiles of a pair (one for a diagonal tile) into LDS, transposed
+// and swizzled as above, and writes each to the other's place: every element is read once
+// and written once, in one pass. Workgroups of the lower triangle (I > J) exit at once.
+template <int ES>
+struct SqTile {
+  static constexpr uint32_t T = ES == 1 ? 128 : ES <= 4 ? 64 : 32;  // tile side (elements)
+  static constexpr uint32_t PB = T * ES < 128 ? 128 : T * ES;       // LDS row pitch (bytes)
+};
+
+template <int ES, bool VEC, int NT>
+__device__ __forceinline__ void sq_load(unsigned char *lds, const unsigned char *base,
+                                        uint64_t n, uint32_t r0, uint32_t c0, uint32_t nr,
+                                        uint32_t nc) {
+  using E = typename Elem<ES>::T;
+  constexpr uint32_t T = SqTile<ES>::T, PB = SqTile<ES>::PB, EPV = 16 / ES;
+  constexpr uint32_t EPV_LOG2 = ES == 1 ? 4 : ES == 2 ? 3 : ES == 4 ? 2 : ES == 8 ? 1 : 0;
+  constexpr int SW = VEC ? int(EPV_LOG2) : 0;
+  if (VEC) {  // rows and tile columns 16-byte aligned
+    constexpr uint32_t VPR = T / EPV;
+    for (uint32_t i = threadIdx.x; i < T * VPR; i += NT) {
+      const uint32_t r = i / VPR, v = i % VPR;
+      if (r < nr && v * EPV < nc) {
+        union {
+          U128 x;
+          E e[EPV];
+        } u;
+        u.x = *reinterpret_cast<const U128 *>(base + ((uint64_t(r0) + r) * n + c0 + v * EPV) * ES);
+#pragma unroll
+        for (uint32_t k = 0; k < EPV; k++) {
+          const uint32_t c = v * EPV + k;
+          *reinterpret_cast<E *>(lds + c * PB + ((r * ES) ^ oop_mask<ES, SW>(c))) = u.e[k];
+        }
+      }
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < T * T; i += NT) {
+      const uint32_t r = i / T, c = i % T;
+      if (r < nr && c < nc)
+        *reinterpret_cast<E *>(lds + c * PB + ((r * ES) ^ oop_mask<ES, SW>(c))) =
+            *reinterpret_cast<const E *>(base + ((uint64_t(r0) + r) * n + c0 + c) * ES);
+    }
+  }
+}
+
+// Writes the staged tile (source rows r0.., columns c0..; nr x nc) transposed: destination
+// row c0 + c, columns r0 .. r0 + nr.
+template <int ES, bool VEC, int NT>
+__device__ __forceinline__ void sq_store(const unsigned char *lds, unsigned char *base,
+                                         uint64_t n, uint32_t r0, uint32_t c0, uint32_t nr,
+                                         uint32_t nc) {
+  using E = typename Elem<ES>::T;
+  constexpr uint32_t T = SqTile<ES>::T, PB = SqTile<ES>::PB, EPV = 16 / ES;
+  constexpr uint32_t EPV_LOG2 = ES == 1 ? 4 : ES == 2 ? 3 : ES == 4 ? 2 : ES == 8 ? 1 : 0;
+  constexpr int SW = VEC ? int(EPV_LOG2) : 0;
+  if (VEC) {
+    constexpr uint32_t VPR = T / EPV;
+    for (uint32_t i = threadIdx.x; i < T * VPR; i += NT) {
+      const uint32_t c = i / VPR, j = i % VPR;
+      if (c < nc && j * EPV < nr) {
+        const uint32_t m = oop_mask<ES, SW>(c);
+        const U128 y = *reinterpret_cast<const U128 *>(lds + c * PB + ((j << 4) ^ (m & ~15u)));
+        const uint32_t s = (m >> 2) & 3u;
+        U128 x;
+        x.x = (s == 0) ? y.x : (s == 1) ? y.y : (s == 2) ? y.z : y.w;
+        x.y = (s == 0) ? y.y : (s == 1) ? y.x : (s == 2) ? y.w : y.z;
+        x.z = (s == 0) ? y.z : (s == 1) ? y.w : (s == 2) ? y.x : y.y;
+        x.w = (s == 0) ? y.w : (s == 1) ? y.z : (s == 2) ? y.y : y.x;
+        *reinterpret_cast<U128 *>(base + ((uint64_t(c0) + c) * n + r0 + j * EPV) * ES) = x;
+      }
+    }
+  } else {
+    for (uint32_t i = threadIdx.x; i < T * T; i += NT) {
+      const uint32_t c = i / T, r = i % T;
+      if (c < nc && r < nr)
+        *reinterpret_cast<E *>(base + ((uint64_t(c0) + c) * n + r0 + r) * ES) =
+            *reinterpret_cast<const E *>(lds + c * PB + ((r * ES) ^ oop_mask<ES, SW>(c)));
+    }
+  }
+}
+
+template <int ES, bool VEC, int NT>
+__global__ __launch_bounds__(NT) void k_sq_swap(unsigned char *base, uint64_t n) {
+  constexpr uint32_t T = SqTile<ES>::T, PB = SqTile<ES>::PB;
+  __shared__ __align__(16) unsigned char lds[2 * T * PB];
+  const uint32_t I = blockIdx.y, J = blockIdx.x;
+  if (I > J) return;
+  const uint32_t ri = I * T, rj = J * T;
+  const uint32_t ni = uint32_t(std::min<uint64_t>(T, n - ri));
+  const uint32_t nj = uint32_t(std::min<uint64_t>(T, n - rj));
+  sq_load<ES, VEC, NT>(lds, base, n, ri, rj, ni, nj);                    // tile (I, J)
+  if (I != J) sq_load<ES, VEC, NT>(lds + T * PB, base, n, rj, ri, nj, ni);  // tile (J, I)
+  __syncthreads();
+  sq_store<ES, VEC, NT>(lds, base, n, ri, rj, ni, nj);  // (I, J)^T -> place of (J, I)
+  if (I != J) sq_store<ES, VEC, NT>(lds + T * PB, base, n, rj, ri, nj, ni);
+}
+
+hipError_t launch_sq_transpose(unsigned char *base, size_t es, uint64_t n, hipStream_t st) {
+  if (n < 2) return hipSuccess;
+  auto go = [&](auto tag, auto vec) -> hipError_t {
+    constexpr int E = decltype(tag)::value;
+    constexpr bool V = decltype(vec)::value;
+    constexpr uint32_t T = SqTile<E>::T;
+    const uint64_t nt = (n + T - 1) / T;
+    if (nt > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_sq_swap<E, V, 256>), dim3(uint32_t(nt), uint32_t(nt)), dim3(256), 0,
+                       st, base, n);
+    return hipGetLastError();
+  };
+  const bool vec = ((reinterpret_cast<uintptr_t>(base) | (n * es)) & 15u) == 0;
+  using std::integral_constant;
+  switch (es) {
+    case 1: return vec ? go(integral_constant<int, 1>{}, std::true_type{})
+                       : go(integral_constant<int, 1>{}, std::false_type{});
+    case 2: return vec ? go(integral_constant<int, 2>{}, std::true_type{})
+                       : go(integral_constant<int, 2>{}, std::false_type{});
+    case 4: return vec ? go(integral_constant<int, 4>{}, std::true_type{})
+                       : go(integral_constant<int, 4>{}, std::false_type{});
+    case 8: return vec ? go(integral_constant<int, 8>{}, std::true_type{})
+                       : go(integral_constant<int, 8>{}, std::false_type{});
+    case 16: return go(integral_constant<int, 16>{}, std::true_type{});
+  }
+  return hipErrorInvalidValue;
+}
+
+bool sq_fits(size_t es, uint64_t n) {
+  const uint64_t T = es == 1 ? 128 : es <= 4 ? 64 : 32;
+  return n >= 2 && (n + T - 1) / T <= 65535;
 }
 
 // ---- byte copies ------------------------------------------------------------------------

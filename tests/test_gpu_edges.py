@@ -98,10 +98,21 @@ def test_square_matrices_every_route(hip, orc, es):
         d = dev(x)
         assert hip.transpose_full(d, n, n, elem_bytes=es) is False
         assert np.array_equal(host(d), want), (n, "planner", hip.last_route())
-        for stages in (5, 6, 7, -1):
+        for stages in (8, 5, 6, 7, -1):
             d = dev(x)
             assert hip.transpose(d, 1, n, 1, n, 1, stages, elem_bytes=es) is False
             assert np.array_equal(host(d), want), (n, stages, hip.last_route())
+            if stages == 8:
+                assert hip.last_route() == "SQ-"
+    # rows of n * es bytes (not a multiple of 16 for es < 16) and a base one element past an
+    # allocation's start take the element form; the element before the matrix is untouched
+    n = 333
+    x = orc.tagged(n * n + 1, es, seed=11)  # flat bytes: one extra element in front
+    d = dev(x)
+    assert hip.transpose(d[es:], 1, n, 1, n, 1, 8, elem_bytes=es) is False
+    got = host(d)
+    assert np.array_equal(got[:es], x[:es])
+    assert np.array_equal(got[es:], orc.op_transpose(x[es:], n, n, es=es))
 
 
 def test_unknown_stage_code_takes_a_valid_route(hip, orc):
