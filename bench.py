@@ -35,6 +35,9 @@ def parse():
     p.add_argument("--no-extra", action="store_true")
     p.add_argument("--profile-batch", action="store_true",
                    help="only run the 4096-matrix batch (and the same matrices as plain calls)")
+    p.add_argument("--profile-st", action="store_true",
+                   help="only a 1 GiB calibration copy and one 98304 x 131071 f64 transpose "
+                        "(for HBM byte counters of the stripe-route kernels)")
     p.add_argument("--profile-large", action="store_true",
                    help="only run the two large prime-dimension transposes (for kernel traces)")
     p.add_argument("--no-large", action="store_true",
@@ -240,6 +243,21 @@ def main():
 
     torch.cuda.set_device(0)
     lm.set_profiling(False)  # asynchronous calls: no per-call synchronization
+    if args.profile_st:
+        c1 = torch.rand(1 << 28, device="cuda")  # 1 GiB
+        c2 = torch.empty_like(c1)
+        c2.copy_(c1)  # the calibration kernel: reads and writes exactly 1 GiB
+        del c1, c2
+        torch.cuda.empty_cache()
+        M, N = 98304, 131071
+        x = torch.empty(M * N, dtype=torch.float64, device="cuda")
+        for s in range(0, M * N, 1 << 30):
+            torch.rand(min(M * N, s + (1 << 30)) - s, dtype=torch.float64, out=x[s:s + (1 << 30)])
+        if lm.transpose_full(x, M, N):
+            raise RuntimeError("transpose failed")
+        torch.cuda.synchronize()
+        print(f"profile run: 1 GiB copy, {M} x {N} f64 transpose ({lm.last_route()})", flush=True)
+        return
     if args.profile_batch:
         print(json.dumps({"profile_batch": extra_batch(lm)}), flush=True)
         return

@@ -88,6 +88,7 @@ Knob g_knobs[] = {
     {"st_rotate", {1}, 0},
     {"autotune", {0}, 0},
     {"pad_shape", {0}, 0},
+    {"st_wstore", {0}, 0},
 };
 Knob *find_knob(const char *key) {
   if (!key) return nullptr;

@@ -48,6 +48,7 @@ struct OopParams {
   uint64_t R, C;                    // each block R x C (source) -> C x R (destination)
   uint32_t tr_log2, tc_log2;        // tile TR x TC elements, both powers of two
   uint32_t PB;                      // LDS row pitch in bytes (multiple of 128)
+  uint32_t wstore;                  // 4-/8-byte elements: aligned vector stores on misaligned rows
   uint64_t ntr, ntc, per_block, tiles, per_xcd;
 };
 
@@ -138,12 +139,14 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
         *reinterpret_cast<U128 *>(dst + (c0 + c) * p.dst_pitch + r0 * ES + (j << 4)) = x;
       }
     }
-  } else if (ES <= 2) {
+  } else if (ES <= 2 || p.wstore) {
     // Destination rows not 16-byte aligned: per row, the elements before the first aligned
     // address and after the last whole vector are stored one by one (lane k == nb of the
     // row); the vectors in between are aligned 16-byte stores, each shifted together from
     // the two LDS chunks it straddles (lanes k < nb).
-    const uint32_t vl = p.tr_log2 > EPV_LOG2 ? p.tr_log2 - EPV_LOG2 + 1 : 1;  // >= nb + 1
+    // A row of TR elements spans TR*ES/16 vectors: a misaligned one has at most one fewer
+    // whole vector, so its head/tail lane still fits.
+    const uint32_t vl = p.tr_log2 > EPV_LOG2 ? p.tr_log2 - EPV_LOG2 : 0;
     const uint32_t vpr = 1u << vl;
     for (uint32_t i = threadIdx.x; i < (TC << vl); i += NT) {
       const uint32_t c = i >> vl, k = i & (vpr - 1u);
@@ -455,6 +458,7 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
   p.tr_log2 = trl;
   p.tc_log2 = tcl;
   p.PB = PB;
+  p.wstore = uint32_t(tuning("st_wstore"));
   const uint32_t TC = 1u << tcl;
   p.ntr = (R + (1ull << trl) - 1) >> trl;
   p.ntc = (C + (1ull << tcl) - 1) >> tcl;
