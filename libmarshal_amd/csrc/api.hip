@@ -89,6 +89,7 @@ Knob g_knobs[] = {
     {"autotune", {0}, 0},
     {"pad_shape", {0}, 0},
     {"st_wstore", {0}, 0},
+    {"st_tile_rows", {0}, 0},
 };
 Knob *find_knob(const char *key) {
   if (!key) return nullptr;

@@ -437,9 +437,18 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
     tcl = (l + 1) / 2;
     trl = l - tcl;
   }
-  // Never larger than the matrix side rounded up to a power of two.
+  // Never larger than the matrix side rounded up to a power of two; optionally fewer tile
+  // rows (longer contiguous row segments per tile).
   trl = std::min(trl, log2_ceil(R));
   tcl = std::min(tcl, log2_ceil(C));
+  const int64_t max_rows = tuning("st_tile_rows");
+  if (max_rows > 0) {
+    const uint32_t cap = log2_floor(uint64_t(max_rows));
+    if (trl > cap) {
+      tcl = std::min(tcl + (trl - cap), log2_ceil(C));
+      trl = cap;
+    }
+  }
   // LDS rows: TR elements padded to a multiple of 128 bytes; TC rows within the tile budget.
   const uint32_t PB = uint32_t((((uint64_t(1) << trl) * es) + 127) / 128 * 128);
   while (tcl > 0 && (uint64_t(1) << tcl) * PB > std::max<size_t>(tile_bytes, PB)) tcl--;

@@ -99,3 +99,27 @@ def test_server_rejects_oversized_and_invalid(hip):
     with pytest.raises(ValueError):
         server.submit(d, hip.OP_AOS_ASTA, 4, 100, 3, 64)  # 100 rows not a multiple of 64
     server.close()
+
+
+def test_server_large_requests_run_alone_and_stay_exact(hip, orc):
+    """Requests of 64 MiB or more run alone on the plain path (the stripe route for prime
+    dimensions) next to small batched ones; every result must match the oracle."""
+    import torch
+    big = [(4099, 4111, 4, hip.OP_TRANSPOSE, 0), (2053, 6007, 8, hip.OP_TRANSPOSE, 0),
+           (1 << 18, 64, 4, hip.OP_AOS_ASTA, 64)]
+    reqs = []
+    for k, (rows, cols, es, op, tile) in enumerate(big):
+        x = orc.tagged(rows * cols, es, seed=500 + k)
+        if op == hip.OP_TRANSPOSE:
+            want = orc.op_transpose(x, rows, cols, es=es)
+        else:
+            want = orc.op_010(x, rows // tile, tile, cols, es=es)
+        reqs.append((0.0, op, es, rows, cols, tile, x, want))
+    small = small_requests(orc, 60, seed=9)
+    mixed = [reqs[0]] + small[:30] + [reqs[1]] + small[30:] + [reqs[2]]
+    server = hip.Server(64 << 30, streams=4, max_batch=32)
+    run_and_check(hip, mixed, server, realtime=False)
+    st = server.stats()
+    say("large+small", st)
+    assert st["completed"] == len(mixed) and st["failed"] == 0
+    server.close()
