@@ -484,6 +484,20 @@ bool route_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es, uin
         return true;
       }
       break;
+    case 9:  // matrices within the scratch cap: copy to scratch, transpose back (no counterpart)
+      if (double(A) * a * B * b * es <= double(uint64_t(tuning("st_scratch_mb")) << 20) &&
+          A * a > 1 && B * b > 1) {
+        StageRec r{};
+        r.kind = StageKind::OOP;
+        r.es = es;
+        r.buf = buf;
+        r.d1 = A * a;
+        r.d2 = B * b;
+        out.push_back(r);
+        route += "OOP-";
+        return true;
+      }
+      break;
     case 8:  // square: one pass of tile-pair swaps (no counterpart)
       if (A * a == B * b && sq_fits(es, A * a)) {
         StageRec r{};
@@ -530,6 +544,7 @@ size_t stages_scratch_bytes(const std::vector<StageRec> &stages) {
   for (const StageRec &r : stages) {
     if (r.kind == StageKind::PT100G) m = std::max<size_t>(m, r.pt.flag_words * 4);
     if (r.kind == StageKind::ST) m = std::max<size_t>(m, r.stp.scratch_bytes);
+    if (r.kind == StageKind::OOP) m = std::max<size_t>(m, size_t(r.d1 * r.d2 * r.es));
   }
   return m;
 }
@@ -566,6 +581,20 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
       case StageKind::SQ:
         LM_CHECK_HIP(launch_sq_transpose(r.buf, r.es, r.d1, st));
         break;
+      case StageKind::OOP: {
+        const uint64_t bytes = r.d1 * r.d2 * r.es;
+        std::unique_ptr<Lease> lease;
+        unsigned char *tmp = scr;
+        if (!tmp) {
+          lease.reset(new Lease(pool_here(), st));
+          tmp = reinterpret_cast<unsigned char *>(lease->get(bytes));
+        }
+        if (!tmp) return true;
+        LM_CHECK_HIP(launch_copy(tmp, r.buf, bytes, st));
+        LM_CHECK_HIP(launch_oop_transpose(tmp, r.buf, r.es, 1, r.d1, r.d2, 0, 0,
+                                          size_t(tuning("st_tile_kb")) << 10, st));
+        break;
+      }
       case StageKind::ST: {
         std::unique_ptr<Lease> lease;
         unsigned char *buf = scr;
@@ -665,6 +694,8 @@ double stage_bw_tbs(const StageRec &r) {
     case StageKind::RC_COL1:
     case StageKind::RC_COL2: return r.rc.ub >= 4 ? 0.55 : r.rc.ub == 2 ? 0.4 : 0.3;
     case StageKind::SQ: return es >= 4 ? 5.0 : es == 2 ? 4.0 : 3.0;
+    case StageKind::OOP:  // a copy pass and an out-of-place transpose pass
+      return 1.0 / (1.0 / 5.5 + 1.0 / (es >= 4 ? 4.5 : es == 2 ? 3.5 : 2.5));
     case StageKind::ST: {  // copy + out-of-place transpose + 100 (+ padding and a 2-D copy)
       const StPlan &p = r.stp;
       StageRec pt{};
@@ -729,6 +760,7 @@ static std::vector<Choice> rank_routes(size_t es, uint64_t M, uint64_t N, bool a
     if (route.find("RC-") == std::string::npos && stages == 5) return;
     if (route.find("ST_") == std::string::npos && (stages == 6 || stages == 7)) return;
     if (route.find("SQ-") == std::string::npos && stages == 8) return;
+    if (route.find("OOP-") == std::string::npos && stages == 9) return;
     double c = route_cost(v);
     if (plan_debug())
       fprintf(stderr, "[libmarshal] plan %llux%llu es=%zu: stages %d a=%llu b=%llu %s cost %.4g\n",
@@ -744,6 +776,7 @@ static std::vector<Choice> rank_routes(size_t es, uint64_t M, uint64_t N, bool a
   consider(1, M, 1, N, -1);  // element-granular fallback
   consider(1, M, 1, N, 5);   // row / column decomposition
   if (allow_st && M == N) consider(1, M, 1, N, 8);  // square: tile-pair swaps
+  if (allow_st) consider(1, M, 1, N, 9);            // through scratch (within its cap)
   if (allow_st) {            // stripes through bounded scratch
     consider(1, M, 1, N, 6);
     consider(1, M, 1, N, 7);
@@ -793,7 +826,7 @@ static bool choice_stages(std::vector<StageRec> &out, unsigned char *buf, size_t
   if (c.stages == 22) return route_stages(out, buf, es, 1, M, N / c.b, c.b, 1, 22, route);
   if (c.stages == 3 || c.stages == 32)
     return route_stages(out, buf, es, M / c.a, c.a, N / c.b, c.b, 1, c.stages, route);
-  if (c.stages >= 5 && c.stages <= 8)
+  if (c.stages >= 5 && c.stages <= 9)
     return route_stages(out, buf, es, 1, M, 1, N, 1, c.stages, route);
   return route_stages(out, buf, es, 1, M, 1, N, 1, -1, route);
 }
@@ -826,6 +859,7 @@ static bool autotune_full(hipStream_t st, unsigned char *buf, size_t es, uint64_
   family_best([](const Choice &c) { return c.stages == 6; });
   family_best([](const Choice &c) { return c.stages == 7; });
   family_best([](const Choice &c) { return c.stages == 5; });
+  family_best([](const Choice &c) { return c.stages == 8 || c.stages == 9; });
   family_best([](const Choice &c) {
     return c.stages == 2 || c.stages == 22 || c.stages == 3 || c.stages == 32;
   });

@@ -402,8 +402,13 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
     if (allow_solo && d.op == HIP_MARSHAL_OP_TRANSPOSE) {
       std::vector<StageRec> list;
       std::string r2;
+      // The stripe route has no grouped form; the square and through-scratch routes beat the
+      // grouped ones from a few MB on (smaller matrices gain more from sharing launches).
+      const double mbytes = double(d.rows) * double(d.cols) * double(d.elem_bytes);
       if (plan_desc(d, list, r2, true) &&
-          (r2.find("ST_") != std::string::npos || r2.find("SQ-") != std::string::npos)) {
+          (r2.find("ST_") != std::string::npos ||
+           (mbytes >= double(8 << 20) &&
+            (r2.find("SQ-") != std::string::npos || r2.find("OOP-") != std::string::npos)))) {
         b.solo.push_back(std::move(list));
         b.solo_bytes.push_back(double(d.rows) * double(d.cols) * double(d.elem_bytes));
         b.routes[i] = r2;
@@ -511,6 +516,7 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
         }
         case StageKind::ST:  // never planned for a batch (allow_st = false)
         case StageKind::SQ:
+        case StageKind::OOP:
           fprintf(stderr, "[libmarshal] %s: stripe route in a batch\n", who);
           return false;
       }

@@ -166,7 +166,7 @@ int64_t tuning(const char *key);
 
 // ---- stage records (a route as a list of elementary stages) -----------------------------
 enum class M010 { Auto, BS, PTTWAC, Global };
-enum class StageKind : int { BS010, PT010, PT100G, PT0100L, RC_ROW, RC_COL1, RC_COL2, ST, SQ };
+enum class StageKind : int { BS010, PT010, PT100G, PT0100L, RC_ROW, RC_COL1, RC_COL2, ST, SQ, OOP };
 struct StageRec {
   StageKind kind;
   size_t es;
@@ -181,7 +181,9 @@ struct StageRec {
 struct Choice {
   double cost = 1e300;
   uint64_t a = 1, b = 1;
-  int stages = 0;  // 2, 22, 3, 32, 5 (rows/columns), 6/7 (stripes), -1 (element-granular)
+  // 2, 22, 3, 32, 5 (rows/columns), 6/7 (stripes), 8 (square), 9 (through scratch),
+  // -1 (element-granular)
+  int stages = 0;
 };
 size_t current_lds_budget();
 M010 choose_010(size_t es, uint64_t A, uint64_t a, uint64_t B, int R, int P,

@@ -404,3 +404,18 @@ def test_autotune_restores_buffer_and_sets_plan(hip, orc):
         assert hip.plan_transpose(M, N, es)[2] == stages
         assert hip.transpose_full(d, M, N, elem_bytes=es) is False
         assert np.array_equal(host(d), orc.op_transpose(x, M, N, es=es))
+
+
+@pytest.mark.parametrize("es", ES)
+def test_through_scratch_route(hip, orc, es):
+    """stages = 9: a matrix within the scratch cap is copied to scratch and transposed back;
+    rows of any byte alignment, primes and vectors' neighbours included."""
+    for (M, N) in ((2, 3), (17, 19), (300, 200), (1031, 997), (64, 4099), (4099, 64)):
+        if M * N * es > 64 << 20:
+            continue
+        say("oop", es, M, N)
+        x = orc.tagged(M * N, es, seed=M * 5 + N)
+        d = dev(x)
+        assert hip.transpose(d, 1, M, 1, N, 1, 9, elem_bytes=es) is False
+        assert hip.last_route() == "OOP-"
+        assert np.array_equal(host(d), orc.op_transpose(x, M, N, es=es))
