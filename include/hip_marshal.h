@@ -233,10 +233,16 @@ void hip_marshal_set_profiling(int on);
  * "rc_col_blocks_per_cu" (grid caps), "pt100_vw_max" (lanes per cycle-following walker),
  * "lds0100_min_inst", "rc_max_super_bytes" (route thresholds), and for the stripe route
  * "st_super_bytes" (target super-element bytes), "st_scratch_mb" (scratch cap),
- * "st_tile_kb" (LDS tile of the out-of-place transpose), "st_pt_vw" and "st_rotate" (0/1).
+ * "st_tile_kb" (LDS tile of the out-of-place transpose), "st_pt_vw", "st_rotate" (0/1),
+ * "st_wstore" (0/1: window stores for every element size), "st_tile_rows" (0 = no cap on
+ * tile rows), "pad_shape" (0-4: workgroup shape of the vector padding kernel), "autotune"
+ * (0/1: measure each new full-transpose shape on its first call) and "batch_solo_kb"
+ * (matrices of at least this many KiB whose route is the square or through-scratch one run
+ * on the batch's side streams instead of in a grouped kernel).
  * Replaces the reference's hard-coded work-group tables (src/cl_marshal.cc:165-231,
  * 388-465). The setter returns true on an unknown key or a value outside [min, 2^30] (min is
- * 0 for "st_rotate", 1 otherwise); the getter returns 0 for an unknown key. */
+ * 0 for "st_rotate", "st_wstore", "st_tile_rows", "pad_shape" and "autotune", 1 otherwise);
+ * the getter returns 0 for an unknown key. */
 bool hip_marshal_set_tuning(const char *key, int64_t value);
 /* Measured autotuner for an M x N transpose of elem_bytes elements: the planner's cheapest
  * candidate routes and the best of each route family are run on `buf` (on `stream`), each

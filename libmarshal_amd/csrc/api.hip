@@ -90,6 +90,7 @@ Knob g_knobs[] = {
     {"pad_shape", {0}, 0},
     {"st_wstore", {0}, 0},
     {"st_tile_rows", {0}, 0},
+    {"batch_solo_kb", {1024}},
 };
 Knob *find_knob(const char *key) {
   if (!key) return nullptr;

@@ -407,7 +407,7 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
       const double mbytes = double(d.rows) * double(d.cols) * double(d.elem_bytes);
       if (plan_desc(d, list, r2, true) &&
           (r2.find("ST_") != std::string::npos ||
-           (mbytes >= double(8 << 20) &&
+           (mbytes >= double(uint64_t(tuning("batch_solo_kb")) << 10) &&
             (r2.find("SQ-") != std::string::npos || r2.find("OOP-") != std::string::npos)))) {
         b.solo.push_back(std::move(list));
         b.solo_bytes.push_back(double(d.rows) * double(d.cols) * double(d.elem_bytes));
