@@ -112,6 +112,12 @@ bool hip_transpose_010_bs_es(hipStream_t stream, void *src, size_t elem_bytes, i
                              int64_t a, int64_t B, uint64_t *elapsed_ns);
 bool hip_transpose_010_pttwac_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t A,
                                  int64_t a, int64_t B, uint64_t *elapsed_ns, int R, int P);
+/* 100 / 0100 pick their form: a single-instance 100 over 1-, 2-, 4-, 8- or 16-byte
+ * super-elements (buffer aligned to that size) is planned as a full transpose; otherwise,
+ * within the scratch cap, groups of instances go through scratch (a copy and an out-of-place
+ * gather); else the in-place forms (row/column decomposition, cycle following). The
+ * AoS/SoA conversions below (method 0) choose the same way; hip_marshal_last_route names
+ * the form taken. */
 bool hip_transpose_100_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t A,
                           int64_t B, int64_t b, uint64_t *elapsed_ns);
 bool hip_transpose_0100_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t A,
@@ -122,7 +128,8 @@ bool hip_transpose_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t 
 bool hip_transpose_full_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t M,
                            int64_t N, uint64_t *elapsed_ns);
 /* AoS [height][width] -> ASTA [height/tile][width][tile] (010) and back (ASTA -> AoS is
- * 010 with a and B exchanged). method: 0 = LDS-staged with cycle-following fallback,
+ * 010 with a and B exchanged). method: 0 = LDS-staged when the tile fits LDS, else through
+ * scratch within the scratch cap, else split / row-column / cycle following in place;
  * 1 = LDS-staged only, 2 = cycle following only. */
 bool hip_aos_asta_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t height,
                      int64_t width, int64_t tile_size, int method, uint64_t *elapsed_ns);
@@ -232,7 +239,9 @@ void hip_marshal_set_profiling(int on);
  * "bs_blocks_per_cu", "pt010_blocks_per_cu", "pt100_blocks_per_cu", "rc_row_blocks_per_cu",
  * "rc_col_blocks_per_cu" (grid caps), "pt100_vw_max" (lanes per cycle-following walker),
  * "lds0100_min_inst", "rc_max_super_bytes" (route thresholds), and for the stripe route
- * "st_super_bytes" (target super-element bytes), "st_scratch_mb" (scratch cap),
+ * "st_super_bytes" (target super-element bytes), "st_scratch_mb" (scratch cap of the
+ * stripe and through-scratch forms; 0 switches them off), "gather_tile_kb" (LDS tile of the
+ * out-of-place super-element gather),
  * "st_tile_kb" (LDS tile of the out-of-place transpose), "st_pt_vw", "st_rotate" (0/1),
  * "st_wstore" (0/1: window stores for every element size), "st_tile_rows" (0 = no cap on
  * tile rows), "pad_shape" (0-4: workgroup shape of the vector padding kernel), "autotune"
@@ -241,7 +250,8 @@ void hip_marshal_set_profiling(int on);
  * on the batch's side streams instead of in a grouped kernel).
  * Replaces the reference's hard-coded work-group tables (src/cl_marshal.cc:165-231,
  * 388-465). The setter returns true on an unknown key or a value outside [min, 2^30] (min is
- * 0 for "st_rotate", "st_wstore", "st_tile_rows", "pad_shape" and "autotune", 1 otherwise);
+ * 0 for "st_scratch_mb", "st_rotate", "st_wstore", "st_tile_rows", "pad_shape" and
+ * "autotune", 1 otherwise);
  * the getter returns 0 for an unknown key. */
 bool hip_marshal_set_tuning(const char *key, int64_t value);
 /* Measured autotuner for an M x N transpose of elem_bytes elements: the planner's cheapest
@@ -258,13 +268,14 @@ bool hip_marshal_autotune(hipStream_t stream, void *buf, size_t elem_bytes, int6
 int64_t hip_marshal_get_tuning(const char *key);
 /* The planner's choice for an M x N transpose of elem_bytes elements: tile factors a | M,
  * b | N and the stage sequence (2, 22, 3, 32; 5 = row/column shuffles, 6/7 = stripes of
- * rows/columns through bounded scratch, 8 = square tile-pair swaps; -1 = element-granular
- * fallback). Host-only
+ * rows/columns through bounded scratch, 8 = square tile-pair swaps, 9 = through scratch;
+ * -1 = element-granular fallback). Host-only
  * (no device work). Replaces the tile heuristic of the reference's test
  * (test/cl_unittest.cc:396-495, "Alg. 5" and "Alg. 3"). */
 bool hip_marshal_plan_transpose(int64_t M, int64_t N, size_t elem_bytes, int64_t *a,
                                 int64_t *b, int *stages);
-/* Route tag of the last hip_transpose* call on this thread ("3.1-010_BS" ...). */
+/* Route tag of the last transpose, 010 / 0100 / 100 or AoS / SoA call on this thread
+ * ("3.1-010_BS", "010_OOP-", "0100_FULL:SQ-" ...). */
 const char *hip_marshal_last_route(void);
 /* Grows the current device's flag pool to at least `bytes` (false on success). */
 bool hip_marshal_reserve(size_t bytes);

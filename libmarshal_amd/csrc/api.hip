@@ -81,7 +81,7 @@ Knob g_knobs[] = {
     {"rc_row_blocks_per_cu", {2}},
     {"rc_max_super_bytes", {64}},
     {"rc_col_blocks_per_cu", {1}},
-    {"st_scratch_mb", {256}},
+    {"st_scratch_mb", {256}, 0},  // 0: no routes that need scratch beyond visited bits
     {"st_super_bytes", {2048}},
     {"st_tile_kb", {32}},
     {"st_pt_vw", {16}},
@@ -90,6 +90,8 @@ Knob g_knobs[] = {
     {"pad_shape", {0}, 0},
     {"st_wstore", {0}, 0},
     {"st_tile_rows", {0}, 0},
+    {"gather_tile_kb", {24}},
+    {"gather_row_bytes", {256}, 0},
     {"batch_solo_kb", {1024}},
 };
 Knob *find_knob(const char *key) {
@@ -283,14 +285,33 @@ M010 choose_010(size_t es, uint64_t A, uint64_t a, uint64_t B, int R, int P,
   return M010::Global;
 }
 
+// Through scratch: groups of blocks are copied to scratch and moved back out of place,
+// [inst][R][C] super-elements of S bytes -> [inst][C][R]; every byte is read twice and
+// written twice by streaming kernels. One block must fit the scratch cap ("st_scratch_mb").
+// No counterpart in the reference, which has only in-place forms.
+static bool push_oop(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t inst,
+                     uint64_t R, uint64_t C, uint64_t S) {
+  if (inst == 0 || R <= 1 || C <= 1) return false;
+  const uint64_t cap = uint64_t(tuning("st_scratch_mb")) << 20;
+  const double block = double(R) * double(C) * double(S);
+  if (block > double(cap) || !oop_gather_fits(S)) return false;
+  StageRec r{};
+  r.kind = StageKind::OOP;
+  r.es = es;
+  r.buf = buf;
+  r.d0 = inst;
+  r.d1 = R;
+  r.d2 = C;
+  r.d3 = S;
+  r.chunk = std::max<uint64_t>(1, std::min<uint64_t>(inst, cap / uint64_t(block)));
+  out.push_back(r);
+  return true;
+}
+
 void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t inst,
-               uint64_t R, uint64_t C, uint64_t b) {
+               uint64_t R, uint64_t C, uint64_t b, bool allow_oop, const char **tag) {
+  if (tag) *tag = "";
   if (inst == 0 || R <= 1 || C <= 1) return;  // 1 x C and R x 1 grids are unchanged
-  // A single 100 over short super-elements is a full R x C transpose of (b*es)-byte
-  // elements: the LDS-staged row / column decomposition beats cycle following there.
-  if (inst == 1 && b * es < uint64_t(tuning("rc_max_super_bytes")) &&
-      push_rc(out, buf, es, R, C, b * es))
-    return;
   StageRec r{};
   r.es = es;
   r.buf = buf;
@@ -300,12 +321,47 @@ void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64
   r.d3 = b;
   plan_100(es, inst, R, C, b, &r.pt);
   // Many small instances: visited bits of each instance in LDS, one workgroup per instance.
-  if (inst >= uint64_t(tuning("lds0100_min_inst")) && R * C * r.pt.nsub < (1ull << 31) &&
-      lds0100_bytes(R, C, r.pt) <= lds_budget())
-    r.kind = StageKind::PT0100L;
-  else
-    r.kind = StageKind::PT100G;
+  const bool lds_form = inst >= uint64_t(tuning("lds0100_min_inst")) &&
+                        R * C * r.pt.nsub < (1ull << 31) &&
+                        lds0100_bytes(R, C, r.pt) <= lds_budget();
+  if (allow_oop) {
+    std::vector<StageRec> o;
+    if (push_oop(o, buf, es, inst, R, C, b * es)) {
+      StageRec l = r;
+      l.kind = StageKind::PT0100L;
+      if (!lds_form || stage_bw_tbs(l) < stage_bw_tbs(o[0])) {
+        out.push_back(o[0]);
+        if (tag) *tag = "0100_OOP-";
+        return;
+      }
+    }
+  }
+  // A single 100 over short super-elements is a full R x C transpose of (b*es)-byte
+  // elements: the LDS-staged row / column decomposition beats cycle following there.
+  if (inst == 1 && b * es < uint64_t(tuning("rc_max_super_bytes")) &&
+      push_rc(out, buf, es, R, C, b * es)) {
+    if (tag) *tag = "0100_RC-";
+    return;
+  }
+  r.kind = lds_form ? StageKind::PT0100L : StageKind::PT100G;
+  if (tag) *tag = lds_form ? "0100_LDS-" : "0100_PTTWAC-";
   out.push_back(r);
+}
+
+bool plan_step_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t inst,
+                    uint64_t R, uint64_t C, uint64_t b, bool allow_oop, std::string &tag) {
+  const uint64_t S = b * es;
+  if (inst == 1 && R > 1 && C > 1 && (S == 1 || S == 2 || S == 4 || S == 8 || S == 16) &&
+      reinterpret_cast<uintptr_t>(buf) % S == 0) {
+    std::string route;
+    if (!plan_transpose_stages(out, buf, S, R, C, route, allow_oop)) return false;
+    tag = "0100_FULL:" + route;
+    return true;
+  }
+  const char *t = "";
+  push_0100(out, buf, es, inst, R, C, b, allow_oop, &t);
+  tag = t;
+  return true;
 }
 
 // A 010 whose tile is too large for LDS, split along B = B1 * B2:
@@ -345,11 +401,25 @@ static bool push_010_split(std::vector<StageRec> &out, unsigned char *buf, size_
 }
 
 bool push_010(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t A,
-              uint64_t a, uint64_t B, M010 m, int R, int P, const char **tag) {
+              uint64_t a, uint64_t B, M010 m, int R, int P, const char **tag, bool allow_oop) {
   if (tag) *tag = "";
   if (A == 0 || a <= 1 || B <= 1) return true;  // 1 x B and a x 1 tiles are unchanged
   if (m == M010::Auto) {
     m = choose_010(es, A, a, B, R, P, reinterpret_cast<uintptr_t>(buf));
+    // Through scratch (two streaming passes) when allowed, for tiles too large for LDS and,
+    // from 1 MB on, for LDS-staged runs that would move single bytes or halves (no dword
+    // grouping) or fill fewer than half the CUs with tiles; else the in-place forms below.
+    bool oop = m != M010::BS;
+    if (!oop && allow_oop && double(A) * double(a) * double(B) * double(es) >= double(1 << 20)) {
+      Bs010Plan pl{};
+      if (plan_010_bs(es, A, a, B, lds_budget(), reinterpret_cast<uintptr_t>(buf),
+                      int(tuning("bs_upt")), &pl))
+        oop = (es < 4 && pl.G == 1) || pl.nchunks * 2 < uint64_t(device_info().cus);
+    }
+    if (oop && allow_oop && push_oop(out, buf, es, A, a, B, es)) {
+      if (tag) *tag = "010_OOP-";
+      return true;
+    }
     if (m != M010::BS && push_010_split(out, buf, es, A, a, B)) {
       if (tag) *tag = "010_SPLIT-";
       return true;
@@ -486,15 +556,7 @@ bool route_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es, uin
       }
       break;
     case 9:  // matrices within the scratch cap: copy to scratch, transpose back (no counterpart)
-      if (double(A) * a * B * b * es <= double(uint64_t(tuning("st_scratch_mb")) << 20) &&
-          A * a > 1 && B * b > 1) {
-        StageRec r{};
-        r.kind = StageKind::OOP;
-        r.es = es;
-        r.buf = buf;
-        r.d1 = A * a;
-        r.d2 = B * b;
-        out.push_back(r);
+      if (push_oop(out, buf, es, 1, A * a, B * b, es)) {
         route += "OOP-";
         return true;
       }
@@ -545,7 +607,7 @@ size_t stages_scratch_bytes(const std::vector<StageRec> &stages) {
   for (const StageRec &r : stages) {
     if (r.kind == StageKind::PT100G) m = std::max<size_t>(m, r.pt.flag_words * 4);
     if (r.kind == StageKind::ST) m = std::max<size_t>(m, r.stp.scratch_bytes);
-    if (r.kind == StageKind::OOP) m = std::max<size_t>(m, size_t(r.d1 * r.d2 * r.es));
+    if (r.kind == StageKind::OOP) m = std::max<size_t>(m, size_t(r.chunk * r.d1 * r.d2 * r.d3));
   }
   return m;
 }
@@ -583,17 +645,34 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
         LM_CHECK_HIP(launch_sq_transpose(r.buf, r.es, r.d1, st));
         break;
       case StageKind::OOP: {
-        const uint64_t bytes = r.d1 * r.d2 * r.es;
+        // groups of `chunk` blocks: copy to scratch, then out of place back into their place
+        const uint64_t S = r.d3, block = r.d1 * r.d2 * S;
         std::unique_ptr<Lease> lease;
         unsigned char *tmp = scr;
         if (!tmp) {
           lease.reset(new Lease(pool_here(), st));
-          tmp = reinterpret_cast<unsigned char *>(lease->get(bytes));
+          tmp = reinterpret_cast<unsigned char *>(lease->get(r.chunk * block));
         }
         if (!tmp) return true;
-        LM_CHECK_HIP(launch_copy(tmp, r.buf, bytes, st));
-        LM_CHECK_HIP(launch_oop_transpose(tmp, r.buf, r.es, 1, r.d1, r.d2, 0, 0,
-                                          size_t(tuning("st_tile_kb")) << 10, st));
+        const size_t tile = size_t(tuning("st_tile_kb")) << 10;
+        for (uint64_t g0 = 0; g0 < r.d0; g0 += r.chunk) {
+          const uint64_t n = std::min(r.chunk, r.d0 - g0);
+          unsigned char *dst = r.buf + g0 * block;
+          LM_CHECK_HIP(launch_copy(tmp, dst, n * block, st));
+          // the tiled transpose for 1- to 16-byte elements, unless destination rows are
+          // short and not 16-byte aligned: then the gather, whose tiles span whole columns
+          // and so write one contiguous range each
+          const uint64_t row_bytes = r.d1 * S;
+          const bool aligned_rows =
+              ((reinterpret_cast<uintptr_t>(dst) | row_bytes) & 15u) == 0;
+          if ((S == 1 || S == 2 || S == 4 || S == 8 || S == 16) &&
+              reinterpret_cast<uintptr_t>(dst) % S == 0 &&
+              (aligned_rows || row_bytes >= uint64_t(tuning("gather_row_bytes"))))
+            LM_CHECK_HIP(launch_oop_transpose(tmp, dst, S, n, r.d1, r.d2, block, block, tile, st));
+          else
+            LM_CHECK_HIP(launch_oop_gather(tmp, dst, S, n, r.d1, r.d2,
+                                           size_t(tuning("gather_tile_kb")) << 10, st));
+        }
         break;
       }
       case StageKind::ST: {
@@ -634,17 +713,23 @@ bool run_stages_on(const std::vector<StageRec> &stages, hipStream_t st, unsigned
   return run_stages_impl(stages, st, scratch);
 }
 
+// One 0100 / 010 step on its own (entry points of single stages and the AoS / SoA
+// conversions); the route taken is what hip_marshal_last_route reports afterwards.
 static bool stage_0100(unsigned char *buf, size_t es, uint64_t inst, uint64_t R, uint64_t C,
                        uint64_t b, hipStream_t st) {
   std::vector<StageRec> v;
-  push_0100(v, buf, es, inst, R, C, b);
+  std::string tag;
+  if (!plan_step_0100(v, buf, es, inst, R, C, b, true, tag)) return true;
+  t_route = tag;
   return run_stages(v, st);
 }
 
 static bool stage_010(unsigned char *buf, size_t es, uint64_t A, uint64_t a, uint64_t B,
                       M010 m, int R, int P, hipStream_t st) {
   std::vector<StageRec> v;
-  if (!push_010(v, buf, es, A, a, B, m, R, P, nullptr)) return true;
+  const char *tag = "";
+  if (!push_010(v, buf, es, A, a, B, m, R, P, &tag, m == M010::Auto)) return true;
+  t_route = tag;
   return run_stages(v, st);
 }
 
@@ -681,6 +766,8 @@ std::vector<uint64_t> some_divisors(uint64_t n, size_t cap) {
   return out;
 }
 
+}  // namespace
+
 double stage_bw_tbs(const StageRec &r) {
   size_t es = r.es;
   switch (r.kind) {
@@ -695,8 +782,13 @@ double stage_bw_tbs(const StageRec &r) {
     case StageKind::RC_COL1:
     case StageKind::RC_COL2: return r.rc.ub >= 4 ? 0.55 : r.rc.ub == 2 ? 0.4 : 0.3;
     case StageKind::SQ: return es >= 4 ? 5.0 : es == 2 ? 4.0 : 3.0;
-    case StageKind::OOP:  // a copy pass and an out-of-place transpose pass
-      return 1.0 / (1.0 / 5.5 + 1.0 / (es >= 4 ? 4.5 : es == 2 ? 3.5 : 2.5));
+    case StageKind::OOP: {  // a copy pass and an out-of-place transpose / gather pass
+      const uint64_t S = r.d3;
+      if (S == 1 || S == 2 || S == 4 || S == 8 || S == 16)
+        return 1.0 / (1.0 / 5.5 + 1.0 / (S >= 4 ? 4.5 : S == 2 ? 3.5 : 2.5));
+      const uint64_t u = (S & 15u) == 0 ? 16 : (S & 7u) == 0 ? 8 : (S & 3u) == 0 ? 4 : (S & 1u) == 0 ? 2 : 1;
+      return 1.0 / (1.0 / 5.5 + 1.0 / (u >= 8 ? 4.0 : u == 4 ? 3.5 : u == 2 ? 3.0 : 2.0));
+    }
     case StageKind::ST: {  // copy + out-of-place transpose + 100 (+ padding and a 2-D copy)
       const StPlan &p = r.stp;
       StageRec pt{};
@@ -712,6 +804,8 @@ double stage_bw_tbs(const StageRec &r) {
   return 0.1;
 }
 
+namespace {
+
 double stage_bytes(const StageRec &r) {
   switch (r.kind) {
     case StageKind::BS010:
@@ -722,6 +816,7 @@ double stage_bytes(const StageRec &r) {
     case StageKind::RC_ROW:
     case StageKind::RC_COL1:
     case StageKind::RC_COL2:  // d0 instances of d1 x d2 elements of d3 bytes
+    case StageKind::OOP:
       return double(std::max<uint64_t>(r.d0, 1)) * double(r.d1) * double(r.d2) * double(r.d3);
     default:  // ST: d1 x d2 elements of es bytes
       return double(r.d1) * double(r.d2) * double(r.es);

@@ -370,20 +370,24 @@ bool plan_desc(const hip_marshal_desc &d, std::vector<StageRec> &out, std::strin
   switch (d.op) {
     case HIP_MARSHAL_OP_TRANSPOSE:
       return plan_transpose_stages(out, buf, es, h, w, route, allow_st);
-    case HIP_MARSHAL_OP_AOS_ASTA:  // 010(h/t, t, w)
-      route = "aos_asta";
-      return push_010(out, buf, es, h / t, t, w, M010::Auto, 1, 1, nullptr);
-    case HIP_MARSHAL_OP_ASTA_AOS:  // 010(h/t, w, t)
-      route = "asta_aos";
-      return push_010(out, buf, es, h / t, w, t, M010::Auto, 1, 1, nullptr);
-    case HIP_MARSHAL_OP_SOA_ASTA:  // 100(w, h/t, t)
-      route = "soa_asta";
-      push_0100(out, buf, es, 1, w, h / t, t);
-      return true;
-    case HIP_MARSHAL_OP_ASTA_SOA:  // 100(h/t, w, t)
-      route = "asta_soa";
-      push_0100(out, buf, es, 1, h / t, w, t);
-      return true;
+    case HIP_MARSHAL_OP_AOS_ASTA:    // 010(h/t, t, w)
+    case HIP_MARSHAL_OP_ASTA_AOS: {  // 010(h/t, w, t)
+      const bool fwd = d.op == HIP_MARSHAL_OP_AOS_ASTA;
+      const char *tag = "";
+      const bool ok = push_010(out, buf, es, h / t, fwd ? t : w, fwd ? w : t, M010::Auto, 1, 1,
+                               &tag, allow_st);
+      route = std::string(fwd ? "aos_asta:" : "asta_aos:") + tag;
+      return ok;
+    }
+    case HIP_MARSHAL_OP_SOA_ASTA:    // 100(w, h/t, t)
+    case HIP_MARSHAL_OP_ASTA_SOA: {  // 100(h/t, w, t)
+      const bool fwd = d.op == HIP_MARSHAL_OP_SOA_ASTA;
+      std::string tag;
+      const bool ok = plan_step_0100(out, buf, es, 1, fwd ? w : h / t, fwd ? h / t : w, t,
+                                     allow_st, tag);
+      route = std::string(fwd ? "soa_asta:" : "asta_soa:") + tag;
+      return ok;
+    }
     default:
       return false;
   }
@@ -399,11 +403,11 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
   for (int64_t i = 0; i < count; i++) {
     const hip_marshal_desc &d = descs[i];
     std::string route;
-    if (allow_solo && d.op == HIP_MARSHAL_OP_TRANSPOSE) {
+    if (allow_solo) {
       std::vector<StageRec> list;
       std::string r2;
       // The stripe route has no grouped form; the square and through-scratch routes beat the
-      // grouped ones from a few MB on (smaller matrices gain more from sharing launches).
+      // grouped ones from about a MB on (smaller matrices gain more from sharing launches).
       const double mbytes = double(d.rows) * double(d.cols) * double(d.elem_bytes);
       if (plan_desc(d, list, r2, true) &&
           (r2.find("ST_") != std::string::npos ||

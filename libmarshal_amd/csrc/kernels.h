@@ -128,6 +128,15 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
                                 uint64_t dst_stride, size_t tile_bytes, hipStream_t st,
                                 uint64_t src_end = 0);
 
+// Out-of-place 100 over super-elements of any size: `inst` blocks of R x C super-elements of
+// S bytes, [inst][R][C] in src -> [inst][C][R] in dst (no overlap). src must be 16-byte
+// aligned; dst any address. Tiles of about `tile_bytes` are staged in LDS; the destination is
+// assembled in units of the largest power of two (up to 16) dividing S and dst.
+bool oop_gather_fits(uint64_t S);
+hipError_t launch_oop_gather(const unsigned char *src, unsigned char *dst, uint64_t S,
+                             uint64_t inst, uint64_t R, uint64_t C, size_t tile_bytes,
+                             hipStream_t st);
+
 // ---- square transpose: one pass of tile-pair swaps ----------------------------------------
 bool sq_fits(size_t es, uint64_t n);
 hipError_t launch_sq_transpose(unsigned char *base, size_t es, uint64_t n, hipStream_t st);
@@ -177,6 +186,7 @@ struct StageRec {
   Pt100Plan pt;
   RcPlan rc;                // RC_*: instances in d0, (M, N) in d1, d2, element bytes in d3
   StPlan stp;               // ST: the whole stripe route (M, N in d1, d2)
+  uint64_t chunk;           // OOP: blocks per scratch fill (d0 blocks of d1 x d2, d3 bytes each)
 };
 struct Choice {
   double cost = 1e300;
@@ -188,10 +198,20 @@ struct Choice {
 size_t current_lds_budget();
 M010 choose_010(size_t es, uint64_t A, uint64_t a, uint64_t B, int R, int P,
                 uint64_t base_addr);
+// allow_oop admits the through-scratch form (plain execution only: it needs scratch of up to
+// a scratch cap and has no grouped form for batches).
 void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t inst,
-               uint64_t R, uint64_t C, uint64_t b);
+               uint64_t R, uint64_t C, uint64_t b, bool allow_oop = false,
+               const char **tag = nullptr);
 bool push_010(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t A,
-              uint64_t a, uint64_t B, M010 m, int R, int P, const char **tag);
+              uint64_t a, uint64_t B, M010 m, int R, int P, const char **tag,
+              bool allow_oop = false);
+// One 0100 step called on its own: push_0100, except that a single-instance 100 whose
+// super-elements are 1, 2, 4, 8 or 16 bytes (with the buffer aligned to that size) is
+// planned as a full transpose of such elements. `tag` receives the route.
+bool plan_step_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t inst,
+                    uint64_t R, uint64_t C, uint64_t b, bool allow_oop, std::string &tag);
+double stage_bw_tbs(const StageRec &r);
 bool route_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t A,
                   uint64_t a, uint64_t B, uint64_t b, int R, int stages, std::string &route);
 // allow_st = false leaves out the stripe route (it needs scratch beyond the visited bits and

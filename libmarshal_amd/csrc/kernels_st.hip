@@ -61,6 +61,23 @@ __device__ __forceinline__ uint32_t oop_mask(uint32_t c) {
   return hi;
 }
 
+// Stores bytes [s, s + n) of a 16-byte value to p + s (n = 1, 2, 4 or 8; s a multiple of n).
+__device__ __forceinline__ void store_piece(unsigned char *p, const U128 &v, uint32_t s,
+                                            uint32_t n) {
+  const uint32_t dw = s >> 2;
+  const uint32_t lo = dw == 0 ? v.x : dw == 1 ? v.y : dw == 2 ? v.z : v.w;
+  if (n == 8) {
+    const uint32_t hi = dw == 0 ? v.y : v.w;
+    *reinterpret_cast<uint64_t *>(p + s) = (uint64_t(hi) << 32) | lo;
+  } else if (n == 4) {
+    *reinterpret_cast<uint32_t *>(p + s) = lo;
+  } else if (n == 2) {
+    *reinterpret_cast<uint16_t *>(p + s) = uint16_t(lo >> ((s & 2u) * 8));
+  } else {
+    p[s] = uint8_t(lo >> ((s & 3u) * 8));
+  }
+}
+
 template <int ES, bool VIN, bool VOUT, int NT>
 __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
   using E = typename Elem<ES>::T;
@@ -173,7 +190,7 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
         U128 hi = lo;
         if (o & 15u) hi = chunk((o >> 4) + 1);  // o + 15 < L: still inside the row
         *reinterpret_cast<U128 *>(row + o) = window_join(o, lo, hi);
-      } else if (k == nb) {
+      } else if (k == nb) {  // head and tail elements one by one
         const uint32_t tail0 = head + 16 * nb;
         for (uint32_t b = 0; b < L; b += ES) {
           if (b >= head && b < tail0) b = tail0;
@@ -490,6 +507,212 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
     case 16: return launch_oop_es<16>(p, vin, vout, lds, st);
   }
   return hipErrorInvalidValue;
+}
+
+// ---- out-of-place 100 over super-elements of any byte size ------------------------------
+// A tile of TR source rows x TC super-elements is staged in LDS as it lies in the source:
+// each source row segment (TC * S bytes at any byte alignment) arrives as the aligned 16-byte
+// chunks that cover it, stored from the start of its LDS row, so LDS row rr holds its segment
+// shifted by that segment's source address mod 16 (which steps by C * S mod 16 per row).
+// The destination is written as aligned 16-byte chunks, each gathered from LDS in units of
+// U bytes. Destination row c of the tile (TR * S bytes, source column c) is one segment;
+// when a tile spans whole columns (TR == R) the tile's destination rows are adjacent, and
+// the whole tile (TC * R * S bytes) is one segment, so short rows cost no partial chunks.
+// Unit x of a segment is byte (x mod S) of super-element (row, column) = (x / S, c), or in
+// whole-column tiles ((x mod RS) / S, x / RS). A chunk only partly inside its segment (the
+// rest belongs to a neighbouring tile) is stored in aligned pieces of 8, 4, 2 and 1 bytes.
+struct GatherParams {
+  const unsigned char *src;  // 16-byte aligned
+  unsigned char *dst;
+  uint64_t inst_bytes;       // R * C * S
+  uint64_t R, C;
+  uint32_t S, TR, TC, LP;    // LP: LDS row pitch (multiple of 16)
+  uint32_t nch, nchd;        // 16-byte chunks per LDS row / per destination segment
+  uint32_t fullcol;          // TR == R: one destination segment per tile (kernel form FC)
+  FastDiv32 div_nch, div_nchd, div_S, div_RS;
+  uint64_t ntc, per_inst, tiles, per_xcd;
+};
+
+template <int U, bool FC, int NT>
+__global__ __launch_bounds__(NT) void k_oop_gather(GatherParams p) {
+  using T = typename Elem<U>::T;
+  extern __shared__ __align__(16) unsigned char lds[];
+  const uint64_t t = uint64_t(blockIdx.x & 7u) * p.per_xcd + (blockIdx.x >> 3);
+  if (t >= p.tiles) return;
+  const uint64_t g = t / p.per_inst;
+  const uint64_t rem = t - g * p.per_inst;
+  const uint64_t tr = rem / p.ntc, tc = rem - tr * p.ntc;
+  const uint64_t r0 = tr * p.TR, c0 = tc * p.TC;
+  const uint32_t nr = uint32_t(std::min<uint64_t>(p.TR, p.R - r0));
+  const uint32_t nc = uint32_t(std::min<uint64_t>(p.TC, p.C - c0));
+  const uint32_t S = p.S, LP = p.LP;
+  const uint64_t CS = p.C * S;
+  const uint64_t sbase = g * p.inst_bytes + r0 * CS + uint64_t(c0) * S;  // tile row 0 segment
+  const uint32_t seg = nc * S;
+  for (uint32_t i = threadIdx.x; i < p.TR * p.nch; i += NT) {
+    const uint32_t r = p.div_nch.div(i), j = i - r * p.nch;
+    if (r >= nr) break;  // r grows with i
+    const uint64_t sb = sbase + r * CS;
+    const uint64_t a = (sb & ~uint64_t(15)) + 16u * j;
+    if (a < sb + seg)
+      *reinterpret_cast<U128 *>(lds + r * LP + 16u * j) =
+          *reinterpret_cast<const U128 *>(p.src + a);
+  }
+  __syncthreads();
+  const uint64_t RS64 = p.R * S;
+  unsigned char *dbase = p.dst + g * p.inst_bytes + c0 * RS64 + r0 * S;
+  const uint32_t RS = FC ? nr * S : 0;       // bytes of one destination row (whole columns)
+  const uint32_t nseg = FC ? 1u : nc;
+  const uint32_t L = FC ? nc * RS : nr * S;  // bytes per segment
+  const uint32_t o0 = uint32_t(sbase & 15u), cs16 = uint32_t(CS & 15u);
+  for (uint32_t i = threadIdx.x; i < nseg * p.nchd; i += NT) {
+    const uint32_t sg = FC ? 0u : p.div_nchd.div(i), k = i - sg * p.nchd;
+    if (sg >= nseg) break;  // sg grows with i
+    unsigned char *row = dbase + uint64_t(sg) * RS64;
+    const int32_t lead = int32_t(reinterpret_cast<uintptr_t>(row) & 15u);
+    const int32_t x0 = int32_t(16u * k) - lead;  // segment byte of this chunk's first byte
+    if (x0 >= int32_t(L)) continue;
+    const uint32_t xs = uint32_t(std::max(x0, 0));
+    const uint32_t xe = std::min<uint32_t>(uint32_t(x0 + 16), L);
+    // Position of unit xs: LDS row rr (base rb, shift ro), column c, byte beta; la = its
+    // LDS address. Kept incrementally from one unit to the next.
+    uint32_t c = sg, rr, beta;
+    if (FC) {
+      c = p.div_RS.div(xs);
+      const uint32_t w = xs - c * RS;
+      rr = p.div_S.div(w);
+      beta = w - rr * S;
+    } else {
+      rr = p.div_S.div(xs);
+      beta = xs - rr * S;
+    }
+    uint32_t rb = rr * LP, ro = (o0 + rr * cs16) & 15u, cS = c * S;
+    uint32_t la = rb + ro + cS + beta;
+    auto next = [&]() {
+      beta += U;
+      la += U;
+      if (beta == S) {
+        beta = 0;
+        rb += LP;
+        ro = (ro + cs16) & 15u;
+        if (FC && ++rr == nr) {
+          rr = 0;
+          rb = 0;
+          ro = o0;
+          cS += S;
+        }
+        la = rb + ro + cS;
+      }
+    };
+    union {
+      U128 v;
+      T u[16 / U];
+    } w;
+    if (x0 >= 0 && xe == uint32_t(x0 + 16)) {  // whole aligned chunk
+#pragma unroll
+      for (int q = 0; q < 16 / U; q++) {
+        w.u[q] = *reinterpret_cast<const T *>(lds + la);
+        next();
+      }
+      *reinterpret_cast<U128 *>(row + x0) = w.v;
+    } else {  // chunk bytes [xs - x0, xe - x0) only
+      const uint32_t s0 = xs - uint32_t(x0), e0 = xe - uint32_t(x0);
+      w.v = U128{0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < 16 / U; q++) {
+        if (uint32_t(q * U) >= s0 && uint32_t(q * U) < e0) {
+          w.u[q] = *reinterpret_cast<const T *>(lds + la);
+          next();
+        }
+      }
+      unsigned char *cb = row + x0;  // 16-byte aligned
+      for (uint32_t s = s0; s < e0;) {
+        const uint32_t n = ((s & 7u) == 0 && s + 8 <= e0)   ? 8u
+                           : ((s & 3u) == 0 && s + 4 <= e0) ? 4u
+                           : ((s & 1u) == 0 && s + 2 <= e0) ? 2u
+                                                            : 1u;
+        store_piece(cb, w.v, s, n);
+        s += n;
+      }
+    }
+  }
+}
+
+namespace {
+constexpr int kGatherThreads = 512;
+
+// Tile sides (super-elements) for S-byte super-elements within about `budget` bytes of LDS:
+// square when both sides are long, else the short side whole and the other grown to fill.
+void gather_tile(uint64_t S, uint64_t R, uint64_t C, size_t budget, uint32_t *TR, uint32_t *TC) {
+  const uint64_t cells = std::max<uint64_t>(1, budget / S);  // super-elements per tile
+  uint64_t t = 1;
+  while ((t + 1) * (t + 1) <= cells && t < 1024) t++;
+  uint64_t tr = std::min(R, t), tc = std::min(C, t);
+  if (tr < t) tc = std::min<uint64_t>(C, std::min<uint64_t>(8192, std::max<uint64_t>(1, cells / tr)));
+  if (tc < t) tr = std::min<uint64_t>(R, std::min<uint64_t>(8192, std::max<uint64_t>(1, cells / tc)));
+  *TR = uint32_t(tr);
+  *TC = uint32_t(tc);
+}
+}  // namespace
+
+bool oop_gather_fits(uint64_t S) {
+  return S >= 1 && S + 48 <= device_info().lds_per_block && S < (1ull << 24);
+}
+
+hipError_t launch_oop_gather(const unsigned char *src, unsigned char *dst, uint64_t S,
+                             uint64_t inst, uint64_t R, uint64_t C, size_t tile_bytes,
+                             hipStream_t st) {
+  if (inst == 0 || R == 0 || C == 0) return hipSuccess;
+  if ((reinterpret_cast<uintptr_t>(src) & 15u) || !oop_gather_fits(S)) return hipErrorInvalidValue;
+  GatherParams p{};
+  p.src = src;
+  p.dst = dst;
+  p.R = R;
+  p.C = C;
+  p.S = uint32_t(S);
+  p.inst_bytes = R * C * S;
+  gather_tile(S, R, C, tile_bytes, &p.TR, &p.TC);
+  // a row segment of TC * S bytes at any alignment spans at most (TC * S + 30) / 16 chunks
+  p.fullcol = p.TR == R ? 1u : 0u;
+  p.nch = uint32_t((uint64_t(p.TC) * S + 30) / 16);
+  // chunks of one destination segment at any alignment: a row of TR * S bytes, or the whole
+  // tile when it spans whole columns
+  p.nchd = uint32_t(((p.fullcol ? uint64_t(p.TC) * R : uint64_t(p.TR)) * S + 30) / 16);
+  p.LP = 16u * p.nch;
+  if ((p.LP / 16) % 2 == 0) p.LP += 16;  // odd number of chunks: rows start on different banks
+  const size_t lds = size_t(p.TR) * p.LP;
+  if (lds > device_info().lds_per_block) return hipErrorInvalidValue;
+  p.div_nch = FastDiv32(p.nch);
+  p.div_nchd = FastDiv32(p.nchd);
+  p.div_S = FastDiv32(p.S);
+  p.div_RS = FastDiv32(uint32_t(std::min<uint64_t>(R * S, 0xffffffffu)));
+  p.ntc = (C + p.TC - 1) / p.TC;
+  p.per_inst = ((R + p.TR - 1) / p.TR) * p.ntc;
+  p.tiles = p.per_inst * inst;
+  p.per_xcd = (p.tiles + 7) / 8;
+  if (p.per_xcd * 8 >= (1ull << 32)) return hipErrorInvalidValue;
+  uint64_t u = 16;
+  while (u > 1 && ((S | reinterpret_cast<uintptr_t>(dst)) & (u - 1))) u >>= 1;
+  auto go = [&](auto kern) -> hipError_t {
+    if (lds > 65536) {
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(uint32_t(p.per_xcd * 8)), dim3(kGatherThreads), lds, st, p);
+    return hipGetLastError();
+  };
+  auto by_u = [&](auto fc) -> hipError_t {
+    constexpr bool F = decltype(fc)::value;
+    switch (u) {
+      case 16: return go(k_oop_gather<16, F, kGatherThreads>);
+      case 8: return go(k_oop_gather<8, F, kGatherThreads>);
+      case 4: return go(k_oop_gather<4, F, kGatherThreads>);
+      case 2: return go(k_oop_gather<2, F, kGatherThreads>);
+      default: return go(k_oop_gather<1, F, kGatherThreads>);
+    }
+  };
+  return p.fullcol ? by_u(std::true_type{}) : by_u(std::false_type{});
 }
 
 // ---- host: planning ---------------------------------------------------------------------

@@ -337,12 +337,22 @@ def test_row_column_decomposition_route(hip, orc, es):
 
 @pytest.mark.parametrize("es", [1, 4, 16])
 def test_soa_small_tiles_take_row_column_route(hip, orc, es):
-    """SoA->ASTA with a small tile is a transpose of b*es-byte super-elements; it is routed
-    through the row/column decomposition and must match the oracle's 100."""
-    for (h, w, t) in ((999, 37, 1), (1002, 13, 3), (4096, 7, 2), (35, 4096, 5)):
-        say("soa small tile", es, h, w, t)
-        x = orc.tagged(h * w, es, seed=h + w + t)
-        check(hip.soa_asta, x, orc.op_100(x, w, h // t, t, es=es), h, w, t, elem_bytes=es)
+    """SoA->ASTA with a small tile is a transpose of b*es-byte super-elements; with the
+    through-scratch forms switched off (scratch cap 0) odd super-element sizes are routed
+    through the in-place row/column decomposition and must match the oracle's 100."""
+    keep = hip.get_tuning("st_scratch_mb")
+    hip.set_tuning("st_scratch_mb", 0)
+    try:
+        for (h, w, t) in ((999, 37, 1), (1002, 13, 3), (4096, 7, 2), (35, 4096, 5)):
+            say("soa small tile", es, h, w, t)
+            x = orc.tagged(h * w, es, seed=h + w + t)
+            check(hip.soa_asta, x, orc.op_100(x, w, h // t, t, es=es), h, w, t, elem_bytes=es)
+            route = hip.last_route()
+            assert "OOP" not in route and "ST_" not in route, route
+            if t * es not in (1, 2, 4, 8, 16) and t * es < hip.get_tuning("rc_max_super_bytes"):
+                assert route == "0100_RC-", route
+    finally:
+        hip.set_tuning("st_scratch_mb", keep)
 
 
 @pytest.mark.parametrize("es", ES)
@@ -379,16 +389,24 @@ def test_stripe_route(hip, orc, es):
 
 @pytest.mark.parametrize("es", ES)
 def test_aos_asta_large_tiles_split(hip, orc, es):
-    """AoS->ASTA tiles too large for LDS (t x w elements) take the split route 0100 + 010;
-    both directions must match the oracle's 010, with a ragged and a prime tile width."""
-    for (h, w, t) in ((640, 4096, 64), (128, 3000, 64), (192, 1024, 32), (256, 4099, 64)):
-        if h * w * es > 64 << 20:
-            continue
-        say("aos large tile", es, h, w, t)
-        x = orc.tagged(h * w, es, seed=h * 31 + w + t)
-        check(hip.aos_asta, x, orc.op_010(x, h // t, t, w, es=es), h, w, t, elem_bytes=es)
-        y = orc.op_010(x, h // t, t, w, es=es)
-        check(hip.asta_aos, y, x, h, w, t, elem_bytes=es)
+    """AoS->ASTA tiles too large for LDS (t x w elements) with the through-scratch form
+    switched off (scratch cap 0) take the in-place split route 0100 + 010 (or the row/column
+    decomposition); both directions must match the oracle's 010, with a ragged and a prime
+    tile width."""
+    keep = hip.get_tuning("st_scratch_mb")
+    hip.set_tuning("st_scratch_mb", 0)
+    try:
+        for (h, w, t) in ((640, 4096, 64), (128, 3000, 64), (192, 1024, 32), (256, 4099, 64)):
+            if h * w * es > 64 << 20:
+                continue
+            say("aos large tile", es, h, w, t)
+            x = orc.tagged(h * w, es, seed=h * 31 + w + t)
+            check(hip.aos_asta, x, orc.op_010(x, h // t, t, w, es=es), h, w, t, elem_bytes=es)
+            assert "OOP" not in hip.last_route()
+            y = orc.op_010(x, h // t, t, w, es=es)
+            check(hip.asta_aos, y, x, h, w, t, elem_bytes=es)
+    finally:
+        hip.set_tuning("st_scratch_mb", keep)
 
 
 def test_autotune_restores_buffer_and_sets_plan(hip, orc):
