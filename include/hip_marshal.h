@@ -20,8 +20,9 @@
  * With profiling off (hip_marshal_set_profiling(0)) calls are fully stream-ordered and never
  * synchronize, and `elapsed_ns` is left untouched.
  *
- * Scratch: cycle-following stages need visited-bit flags, and the stripe route a buffer of
- * one stripe (at most "st_scratch_mb", 256 MiB by default, plus its bits). Both come from
+ * Scratch: cycle-following stages need visited-bit flags, the stripe route a buffer of one
+ * or more stripes and the through-scratch forms one of a group of blocks (each at most
+ * "st_scratch_mb", 1 GiB by default: 0.35% of a 288 GB MI355X). All come from
  * one per-device pool owned by the library that only grows (hipMalloc and hipFree happen on
  * growth only, never per call once the pool is large enough; hip_marshal_reserve()
  * pre-sizes it). Flags are cleared with a fill kernel on the caller's stream. A stage holds
@@ -243,15 +244,18 @@ void hip_marshal_set_profiling(int on);
  * stripe and through-scratch forms; 0 switches them off), "gather_tile_kb" (LDS tile of the
  * out-of-place super-element gather),
  * "st_tile_kb" (LDS tile of the out-of-place transpose), "st_pt_vw", "st_rotate" (0/1),
- * "st_wstore" (0/1: window stores for every element size), "st_tile_rows" (0 = no cap on
+ * "st_wstore" (0/1: window stores for every element size), "st_skew" (0 off, 1 for 1- and
+ * 2-byte elements (default), 2 up to 8-byte ones: skewed tiles for misaligned destination
+ * rows), "gather_row_bytes" (destination rows shorter than this and misaligned take the
+ * gather), "st_tile_rows" (0 = no cap on
  * tile rows), "pad_shape" (0-4: workgroup shape of the vector padding kernel), "autotune"
  * (0/1: measure each new full-transpose shape on its first call) and "batch_solo_kb"
  * (matrices of at least this many KiB whose route is the square or through-scratch one run
  * on the batch's side streams instead of in a grouped kernel).
  * Replaces the reference's hard-coded work-group tables (src/cl_marshal.cc:165-231,
  * 388-465). The setter returns true on an unknown key or a value outside [min, 2^30] (min is
- * 0 for "st_scratch_mb", "st_rotate", "st_wstore", "st_tile_rows", "pad_shape" and
- * "autotune", 1 otherwise);
+ * 0 for "st_scratch_mb", "st_rotate", "st_wstore", "st_skew", "gather_row_bytes",
+ * "st_tile_rows", "pad_shape" and "autotune", 1 otherwise);
  * the getter returns 0 for an unknown key. */
 bool hip_marshal_set_tuning(const char *key, int64_t value);
 /* Measured autotuner for an M x N transpose of elem_bytes elements: the planner's cheapest

@@ -81,7 +81,7 @@ Knob g_knobs[] = {
     {"rc_row_blocks_per_cu", {2}},
     {"rc_max_super_bytes", {64}},
     {"rc_col_blocks_per_cu", {1}},
-    {"st_scratch_mb", {256}, 0},  // 0: no routes that need scratch beyond visited bits
+    {"st_scratch_mb", {1024}, 0},  // 0: no routes that need scratch beyond visited bits
     {"st_super_bytes", {2048}},
     {"st_tile_kb", {32}},
     {"st_pt_vw", {16}},
@@ -92,6 +92,7 @@ Knob g_knobs[] = {
     {"st_tile_rows", {0}, 0},
     {"gather_tile_kb", {24}},
     {"gather_row_bytes", {256}, 0},
+    {"st_skew", {1}, 0},
     {"batch_solo_kb", {1024}},
 };
 Knob *find_knob(const char *key) {
@@ -1060,6 +1061,9 @@ bool hip_marshal_set_tuning(const char *key, int64_t value) {
     return true;
   }
   k->value.store(value);
+  // Route thresholds and caps change what the model's candidates are: plan again.
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  g_plan_cache.clear();
   return false;
 }
 

@@ -50,6 +50,9 @@ struct OopParams {
   uint32_t PB;                      // LDS row pitch in bytes (multiple of 128)
   uint32_t wstore;                  // 4-/8-byte elements: aligned vector stores on misaligned rows
   uint64_t ntr, ntc, per_block, tiles, per_xcd;
+  // skewed form: workgroups from per_xcd * 8 on write the chunks holding row boundaries
+  uint64_t fix_rows;                // rows of all blocks (one more chunk: the end)
+  FastDiv64 div_C;
 };
 
 template <int ES, int SW>
@@ -78,13 +81,67 @@ __device__ __forceinline__ void store_piece(unsigned char *p, const U128 &v, uin
   }
 }
 
-template <int ES, bool VIN, bool VOUT, int NT>
+// Destination chunks that hold a row boundary, for the skewed form: with rows of R * ES
+// bytes back to back over all blocks, the aligned chunk holding the start of row k (when
+// misaligned; its first bytes end row k - 1) and, for k = fix_rows, the one holding the end
+// of the last row. Each is built from the source (out of place, so still intact) and
+// written once: whole when it lies inside the destination range, else only its bytes inside.
+template <int ES>
+__device__ __forceinline__ void oop_fix(const OopParams &p, uint64_t k) {
+  if (k > p.fix_rows) return;
+  const uint64_t RB = p.R * ES, total = p.fix_rows * RB, at = k * RB;
+  const uint32_t mis = uint32_t((reinterpret_cast<uintptr_t>(p.dst) + at) & 15u);
+  if (mis == 0) return;
+  const int64_t a = int64_t(at) - mis;  // chunk start, relative to dst
+  const unsigned char *src = p.src + p.src_off;
+  // source of byte x of destination row kk (block kk / C, source column kk mod C)
+  auto row_src = [&](uint64_t kk) {
+    const uint64_t g = p.div_C.div(kk), cc = kk - g * p.C;
+    return src + g * p.src_stride + cc * ES;
+  };
+  const unsigned char *prev = k ? row_src(k - 1) : nullptr;         // row k - 1
+  const unsigned char *cur = k < p.fix_rows ? row_src(k) : nullptr;  // row k
+  union {
+    U128 v;
+    uint8_t b[16];
+  } w;
+  w.v = U128{0, 0, 0, 0};
+#pragma unroll
+  for (uint32_t i = 0; i < 16; i++) {
+    const bool first = i < mis;
+    const unsigned char *rs = first ? prev : cur;
+    const uint64_t x = first ? RB - mis + i : i - mis;  // byte within that row
+    if (rs) w.b[i] = rs[(x / ES) * p.src_pitch + (x % ES)];
+  }
+  if (a >= 0 && uint64_t(a) + 16 <= total) {
+    *reinterpret_cast<U128 *>(p.dst + a) = w.v;
+  } else {
+#pragma unroll
+    for (uint32_t i = 0; i < 16; i++) {
+      const int64_t q = a + int64_t(i);
+      if (q >= 0 && uint64_t(q) < total) p.dst[q] = w.b[i];
+    }
+  }
+}
+
+// OUT: 1 = destination rows 16-byte aligned (vector stores); 0 = not aligned (window stores
+// for 1-/2-byte elements or when wstore is set, else element stores); 2 = not aligned, skewed
+// tiles: the tile covers, in every destination row, the aligned 16-byte chunks from its first
+// row on, so it loads 16 / ES - 1 source rows above its own and writes whole chunks only; the
+// chunks holding a row boundary are left to k_oop_fix.
+template <int ES, bool VIN, int OUT, int NT>
 __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
   using E = typename Elem<ES>::T;
   constexpr uint32_t EPV = 16 / ES;  // elements per 16-byte vector
   constexpr uint32_t EPV_LOG2 = ES == 1 ? 4 : ES == 2 ? 3 : ES == 4 ? 2 : ES == 8 ? 1 : 0;
   constexpr int SW = VIN ? int(EPV_LOG2) : 0;
+  constexpr bool VOUT = OUT == 1;
+  constexpr uint32_t KOFS = OUT == 2 ? EPV : 0;  // LDS element index of source row r0
   extern __shared__ __align__(16) unsigned char lds[];
+  if (OUT == 2 && blockIdx.x >= p.per_xcd * 8) {
+    oop_fix<ES>(p, (blockIdx.x - p.per_xcd * 8) * uint64_t(NT) + threadIdx.x);
+    return;
+  }
   // Consecutive tiles go to the same XCD (workgroups are dealt round-robin over 8 XCDs), so
   // partial lines shared by neighbouring tiles meet in one L2.
   const uint64_t t = uint64_t(blockIdx.x & 7u) * p.per_xcd + (blockIdx.x >> 3);
@@ -100,14 +157,16 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
   unsigned char *dst = p.dst + g * p.dst_stride;
   const uint32_t PB = p.PB;
 
-  // Load: source row r, column c -> LDS row c, element r.
+  // Load: source row r0 + r - KOFS, column c -> LDS row c, element r.
+  const uint32_t TRL = TR + KOFS;  // LDS elements per row that are loaded
   if (VIN) {  // one 16-byte window per lane (TC >= EPV)
     const uint32_t vl = p.tc_log2 - EPV_LOG2;
     const uint32_t vpr = 1u << vl;
-    for (uint32_t i = threadIdx.x; i < (TR << vl); i += NT) {
+    for (uint32_t i = threadIdx.x; i < (TRL << vl); i += NT) {
       const uint32_t r = i >> vl, v = i & (vpr - 1u);
-      if (r < nr && v * EPV < nc) {
-        const uint64_t sb = sblk + (r0 + r) * p.src_pitch + (c0 + v * EPV) * ES;
+      const int64_t rg = int64_t(r0) + r - KOFS;
+      if (rg >= 0 && rg < int64_t(p.R) && (KOFS || r < nr) && v * EPV < nc) {
+        const uint64_t sb = sblk + uint64_t(rg) * p.src_pitch + (c0 + v * EPV) * ES;
         union {
           U128 x;
           E e[EPV];
@@ -128,11 +187,12 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
       }
     }
   } else {
-    for (uint32_t i = threadIdx.x; i < (TR << p.tc_log2); i += NT) {
+    for (uint32_t i = threadIdx.x; i < (TRL << p.tc_log2); i += NT) {
       const uint32_t r = i >> p.tc_log2, c = i & (TC - 1u);
-      if (r < nr && c < nc)
+      const int64_t rg = int64_t(r0) + r - KOFS;
+      if (rg >= 0 && rg < int64_t(p.R) && (KOFS || r < nr) && c < nc)
         *reinterpret_cast<E *>(lds + c * PB + ((r * ES) ^ oop_mask<ES, SW>(c))) =
-            *reinterpret_cast<const E *>(p.src + sblk + (r0 + r) * p.src_pitch +
+            *reinterpret_cast<const E *>(p.src + sblk + uint64_t(rg) * p.src_pitch +
                                          (c0 + c) * ES);
     }
   }
@@ -155,6 +215,36 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
         x.w = (s == 0) ? y.w : (s == 1) ? y.z : (s == 2) ? y.y : y.x;
         *reinterpret_cast<U128 *>(dst + (c0 + c) * p.dst_pitch + r0 * ES + (j << 4)) = x;
       }
+    }
+  } else if (OUT == 2) {
+    // Chunk j of destination row c starts at row byte x0 = r0 * ES - m + 16 j (m: the row's
+    // misalignment), which is LDS row byte 16 - m + 16 j.
+    const uint32_t vl = p.tr_log2 - EPV_LOG2;  // TR >= EPV
+    const uint32_t vpr = 1u << vl;
+    const int64_t rowbytes = int64_t(p.R) * ES;
+    for (uint32_t i = threadIdx.x; i < (TC << vl); i += NT) {
+      const uint32_t c = i >> vl, j = i & (vpr - 1u);
+      if (c >= nc) continue;
+      unsigned char *row = dst + (c0 + c) * p.dst_pitch;
+      const uint32_t m = uint32_t(reinterpret_cast<uintptr_t>(row) & 15u);
+      const int64_t x0 = int64_t(r0) * ES - m + 16 * int64_t(j);
+      if (x0 < 0 || x0 + 16 > rowbytes) continue;  // holds a row boundary
+      const uint32_t mk = oop_mask<ES, SW>(c);
+      const uint32_t s = (mk >> 2) & 3u;
+      const unsigned char *lrow = lds + c * PB;
+      auto chunk = [&](uint32_t q) {  // logical 16-byte chunk q of this LDS row
+        const U128 y = *reinterpret_cast<const U128 *>(lrow + ((q << 4) ^ (mk & ~15u)));
+        U128 x;
+        x.x = (s == 0) ? y.x : (s == 1) ? y.y : (s == 2) ? y.z : y.w;
+        x.y = (s == 0) ? y.y : (s == 1) ? y.x : (s == 2) ? y.w : y.z;
+        x.z = (s == 0) ? y.z : (s == 1) ? y.w : (s == 2) ? y.x : y.y;
+        x.w = (s == 0) ? y.w : (s == 1) ? y.z : (s == 2) ? y.y : y.x;
+        return x;
+      };
+      const uint32_t o = 16u - m + 16u * j;
+      const U128 lo = chunk(o >> 4);
+      const U128 hi = (o & 15u) ? chunk((o >> 4) + 1) : lo;
+      *reinterpret_cast<U128 *>(row + x0) = window_join(o, lo, hi);
     }
   } else if (ES <= 2 || p.wstore) {
     // Destination rows not 16-byte aligned: per row, the elements before the first aligned
@@ -370,10 +460,45 @@ __global__ __launch_bounds__(256) void k_copy2d(unsigned char *dst, uint64_t dpi
   }
 }
 
+// Copy between any two addresses: the destination in aligned 16-byte chunks (the source read
+// as aligned vectors or, at another alignment, as 16-byte windows), the bytes before the
+// destination's first aligned address and after its last whole chunk by thread 0.
+template <bool SRC_ALIGNED>
+__global__ __launch_bounds__(256) void k_copy_any(unsigned char *dst, const unsigned char *src,
+                                                  uint64_t bytes, uint32_t head, uint64_t nb) {
+  const uint64_t stride = uint64_t(gridDim.x) * 256;
+  U128 *d = reinterpret_cast<U128 *>(dst + head);  // 16-byte aligned
+  // source of chunk i: bytes from src + head + 16 i, as an offset from an aligned base
+  const unsigned char *sbase =
+      reinterpret_cast<const unsigned char *>(reinterpret_cast<uintptr_t>(src + head) &
+                                              ~uintptr_t(15));
+  const uint64_t soff = reinterpret_cast<uintptr_t>(src + head) & 15u;
+  for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < nb; i += stride) {
+    if (SRC_ALIGNED)
+      d[i] = reinterpret_cast<const U128 *>(sbase)[i];
+    else
+      d[i] = load_window16(sbase, soff + 16 * i);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (uint32_t b = 0; b < head; b++) dst[b] = src[b];
+    for (uint64_t b = head + 16 * nb; b < bytes; b++) dst[b] = src[b];
+  }
+}
+
 hipError_t launch_copy(unsigned char *dst, const unsigned char *src, uint64_t bytes,
                        hipStream_t st) {
   if (bytes == 0) return hipSuccess;
   const uint32_t cap = uint32_t(device_info().cus) * 16;
+  if (bytes >= 64 && ((uintptr_t(dst) | uintptr_t(src) | bytes) & 15u) != 0) {
+    const uint32_t head = uint32_t((16u - (uintptr_t(dst) & 15u)) & 15u);
+    const uint64_t nb = (bytes - head) / 16;
+    const uint32_t blocks = uint32_t(std::min<uint64_t>(std::max<uint64_t>((nb + 255) / 256, 1), cap));
+    if (((uintptr_t(src) + head) & 15u) == 0)
+      hipLaunchKernelGGL(k_copy_any<true>, dim3(blocks), dim3(256), 0, st, dst, src, bytes, head, nb);
+    else
+      hipLaunchKernelGGL(k_copy_any<false>, dim3(blocks), dim3(256), 0, st, dst, src, bytes, head, nb);
+    return hipGetLastError();
+  }
   if (((uintptr_t(dst) | uintptr_t(src) | bytes) & 15u) == 0) {
     const uint64_t n = bytes / 16;
     const uint32_t blocks = uint32_t(std::min<uint64_t>((n + 255) / 256, cap));
@@ -412,24 +537,31 @@ uint32_t log2_ceil(uint64_t v) {
 
 constexpr int kOopThreads = 512;
 
-template <int ES, bool VIN, bool VOUT>
+template <int ES, bool VIN, int OUT>
 hipError_t launch_oop_t(const OopParams &p, size_t lds, hipStream_t st) {
-  auto k = k_oop_transpose<ES, VIN, VOUT, kOopThreads>;
+  auto k = k_oop_transpose<ES, VIN, OUT, kOopThreads>;
+  const uint64_t fix_blocks = OUT == 2 ? (p.fix_rows + 1 + kOopThreads - 1) / kOopThreads : 0;
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k, dim3(uint32_t(p.per_xcd * 8)), dim3(kOopThreads), lds, st, p);
+  if (p.per_xcd * 8 + fix_blocks >= (1ull << 32)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k, dim3(uint32_t(p.per_xcd * 8 + fix_blocks)), dim3(kOopThreads), lds, st,
+                     p);
   return hipGetLastError();
 }
 
 template <int ES>
-hipError_t launch_oop_es(const OopParams &p, bool vin, bool vout, size_t lds, hipStream_t st) {
-  if (vin && vout) return launch_oop_t<ES, true, true>(p, lds, st);
-  if (vin) return launch_oop_t<ES, true, false>(p, lds, st);
-  if (vout) return launch_oop_t<ES, false, true>(p, lds, st);
-  return launch_oop_t<ES, false, false>(p, lds, st);
+hipError_t launch_oop_es(const OopParams &p, bool vin, int out, size_t lds, hipStream_t st) {
+  if (out == 2) {
+    if (ES == 16) return hipErrorInvalidValue;  // 16-byte rows are always aligned
+    return vin ? launch_oop_t<ES, true, 2>(p, lds, st) : launch_oop_t<ES, false, 2>(p, lds, st);
+  }
+  if (vin && out) return launch_oop_t<ES, true, 1>(p, lds, st);
+  if (vin) return launch_oop_t<ES, true, 0>(p, lds, st);
+  if (out) return launch_oop_t<ES, false, 1>(p, lds, st);
+  return launch_oop_t<ES, false, 0>(p, lds, st);
 }
 }  // namespace
 
@@ -466,8 +598,21 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
       trl = cap;
     }
   }
-  // LDS rows: TR elements padded to a multiple of 128 bytes; TC rows within the tile budget.
-  const uint32_t PB = uint32_t((((uint64_t(1) << trl) * es) + 127) / 128 * 128);
+  const uint32_t EPV = uint32_t(16 / es);
+  auto al = [](uint64_t v) { return (v & 15u) == 0; };
+  const bool vout = al(reinterpret_cast<uintptr_t>(dst)) && al(R * es) && al(dst_stride) &&
+                    (1u << trl) >= EPV;
+  // Skewed tiles for misaligned destination rows ("st_skew": 0 off, 1 for 1- and 2-byte
+  // elements, 2 up to 8-byte ones); rows back to back and long enough that a chunk holds at
+  // most one row boundary.
+  const int64_t skew_knob = tuning("st_skew");
+  const bool skew = !vout && es < 16 && skew_knob > 0 && (es <= 2 || skew_knob >= 2) &&
+                    (1u << trl) >= EPV && R * es >= 32 &&
+                    (blocks == 1 || dst_stride == R * C * es);
+  // LDS rows: TR elements (plus 16 / es above them when skewed) padded to a multiple of 128
+  // bytes; TC rows within the tile budget.
+  const uint32_t PB =
+      uint32_t(((((uint64_t(1) << trl) + (skew ? EPV : 0)) * es) + 127) / 128 * 128);
   while (tcl > 0 && (uint64_t(1) << tcl) * PB > std::max<size_t>(tile_bytes, PB)) tcl--;
   OopParams p{};
   const uintptr_t sa = reinterpret_cast<uintptr_t>(src);
@@ -494,19 +639,21 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
   if (p.per_xcd * 8 >= (1ull << 32)) return hipErrorInvalidValue;
   const size_t lds = size_t(TC) * PB;
   if (lds > device_info().lds_per_block) return hipErrorInvalidValue;
-  const uint32_t EPV = uint32_t(16 / es);
-  auto al = [](uint64_t v) { return (v & 15u) == 0; };
   const bool vin = TC >= EPV;
-  const bool vout = al(reinterpret_cast<uintptr_t>(dst)) && al(p.dst_pitch) && al(dst_stride) &&
-                    (1u << trl) >= EPV;
-  switch (es) {
-    case 1: return launch_oop_es<1>(p, vin, vout, lds, st);
-    case 2: return launch_oop_es<2>(p, vin, vout, lds, st);
-    case 4: return launch_oop_es<4>(p, vin, vout, lds, st);
-    case 8: return launch_oop_es<8>(p, vin, vout, lds, st);
-    case 16: return launch_oop_es<16>(p, vin, vout, lds, st);
+  const int out = skew ? 2 : vout ? 1 : 0;
+  if (skew) {
+    p.fix_rows = blocks * C;
+    p.div_C = FastDiv64(C);
   }
-  return hipErrorInvalidValue;
+  hipError_t e = hipErrorInvalidValue;
+  switch (es) {
+    case 1: e = launch_oop_es<1>(p, vin, out, lds, st); break;
+    case 2: e = launch_oop_es<2>(p, vin, out, lds, st); break;
+    case 4: e = launch_oop_es<4>(p, vin, out, lds, st); break;
+    case 8: e = launch_oop_es<8>(p, vin, out, lds, st); break;
+    case 16: e = launch_oop_es<16>(p, vin, out, lds, st); break;
+  }
+  return e;
 }
 
 // ---- out-of-place 100 over super-elements of any byte size ------------------------------
