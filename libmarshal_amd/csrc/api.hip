@@ -407,15 +407,18 @@ bool push_010(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_
   if (A == 0 || a <= 1 || B <= 1) return true;  // 1 x B and a x 1 tiles are unchanged
   if (m == M010::Auto) {
     m = choose_010(es, A, a, B, R, P, reinterpret_cast<uintptr_t>(buf));
-    // Through scratch (two streaming passes) when allowed, for tiles too large for LDS and,
-    // from 1 MB on, for LDS-staged runs that would move single bytes or halves (no dword
-    // grouping) or fill fewer than half the CUs with tiles; else the in-place forms below.
+    // Through scratch (two streaming passes) when allowed, for tiles too large for LDS, for
+    // LDS-staged runs that would move single bytes or halves (no dword grouping: a tile of
+    // over half the LDS with a*B*es not a multiple of 4) and, from 1 MB on, for LDS-staged
+    // runs that would fill fewer than half the CUs with tiles; else the in-place forms below.
     bool oop = m != M010::BS;
-    if (!oop && allow_oop && double(A) * double(a) * double(B) * double(es) >= double(1 << 20)) {
+    if (!oop && allow_oop) {
       Bs010Plan pl{};
       if (plan_010_bs(es, A, a, B, lds_budget(), reinterpret_cast<uintptr_t>(buf),
                       int(tuning("bs_upt")), &pl))
-        oop = (es < 4 && pl.G == 1) || pl.nchunks * 2 < uint64_t(device_info().cus);
+        oop = (es < 4 && pl.G == 1) ||
+              (double(A) * double(a) * double(B) * double(es) >= double(1 << 20) &&
+               pl.nchunks * 2 < uint64_t(device_info().cus));
     }
     if (oop && allow_oop && push_oop(out, buf, es, A, a, B, es)) {
       if (tag) *tag = "010_OOP-";

@@ -2,14 +2,15 @@
 //
 // Clients submit conversions of device-resident buffers; a scheduler thread admits queued
 // requests in arrival order while the bytes of admitted, unfinished requests stay within the
-// HBM budget (conversions are in place, so a request's footprint is its buffer plus visited
-// bits, never a second copy), packs the small ones admitted together into one dynamic batch
-// and launches it as grouped persistent kernels (the batched form's work lists) on whichever
-// of its streams is idle; a request of 64 MiB or more runs alone with its plain stage list
-// (per-request kernels, the stripe route included) and the stream's own scratch. Requests
-// that arrive while batches are on the GPU are admitted as soon as budget and a stream free
-// up and run concurrently with them. Completion is observed with an event per batch;
-// latency is arrival -> completion.
+// HBM budget (a request's footprint is its buffer plus visited bits; the scratch of the
+// through-scratch and stripe forms is per stream, at most the scratch cap each), packs the
+// small ones admitted together into one dynamic batch and launches it as grouped persistent
+// kernels (the batched form's work lists) on whichever of its streams is idle; a request of
+// 64 MiB or more, or one admitted alone, runs with its plain stage list (per-request
+// kernels, the through-scratch and stripe forms included) and the stream's own scratch.
+// Requests that arrive while batches are on the GPU are admitted as soon as budget and a
+// stream free up and run concurrently with them. Completion is observed with an event per
+// batch; latency is arrival -> completion.
 //
 // No counterpart in the reference (a single-call library, src/cl_marshal.cc).
 #include "batch.h"
@@ -122,7 +123,7 @@ bool launch_batch(Server &s, Slot &sl, const std::vector<hip_marshal_desc> &desc
   return hipEventRecord(sl.ev, sl.st) == hipSuccess;
 }
 
-// A large request runs alone with the stage list its plain entry point would use (stripe
+// A large (or lone) request runs with the stage list its plain entry point would use (stripe
 // route included), on the slot's stream with the slot's own scratch: nothing to amortise,
 // and per-request kernels beat the grouped forms there.
 bool launch_solo(Server &s, Slot &sl, const hip_marshal_desc &d) {
@@ -198,7 +199,10 @@ void scheduler(Server *sp) {
         s.batches++;
         s.max_in_batch = std::max<int64_t>(s.max_in_batch, int64_t(ids.size()));
         lk.unlock();
-        bool ok = solo ? launch_solo(s, *free_slot, descs[0]) : launch_batch(s, *free_slot, descs);
+        // A request admitted alone gains nothing from the grouped kernels: it takes the
+        // plain path, with no parameter upload.
+        const bool plain = solo || ids.size() == 1;
+        bool ok = plain ? launch_solo(s, *free_slot, descs[0]) : launch_batch(s, *free_slot, descs);
         lk.lock();
         if (!ok) {  // mark failed; the slot is retired once its stream drains
           hipStreamSynchronize(free_slot->st);
