@@ -16,8 +16,8 @@
 // Every element is moved a bounded number of times by streaming kernels (copy, tiled
 // out-of-place transpose, padding) and once by cycle following over large super-elements,
 // so prime or otherwise awkward dimensions need no element-granular cycle following. The
-// scratch buffer holds a group of stripes (a few hundred MB), not a second copy of the
-// matrix. Not a reference algorithm: the reference transposes such matrices only by
+// scratch buffer holds one stripe or a group of stripes (up to a cap of a few GB), not a
+// second copy of the matrix. Not a reference algorithm: the reference transposes such matrices only by
 // element-granular cycle following (src/cl_marshal.cc:769-775); its padding kernels
 // (src/cl/cl_aos_asta.cl:813-938) are the padding/unpadding used here.
 #include "kernels.h"

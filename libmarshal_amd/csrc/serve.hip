@@ -251,9 +251,11 @@ hip_marshal_server *hip_marshal_server_create(size_t hbm_budget_bytes, int strea
   hipGetDevice(&s.device);
   s.slots.resize(size_t(streams));
   for (Slot &sl : s.slots) {
-    if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
-      fprintf(stderr, "[libmarshal] hip_marshal_server_create: stream creation failed\n");
+    hipError_t e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      fprintf(stderr, "[libmarshal] hip_marshal_server_create: stream creation failed: %s\n",
+              hipGetErrorString(e));
       delete h;
       return nullptr;
     }
