@@ -94,6 +94,7 @@ Knob g_knobs[] = {
     {"gather_row_bytes", {256}, 0},
     {"st_skew", {1}, 0},
     {"copy_nt", {1}, 0},
+    {"oop_nt", {2}, 0},
     {"batch_solo_kb", {1024}},
 };
 Knob *find_knob(const char *key) {

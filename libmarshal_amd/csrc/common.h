@@ -120,12 +120,15 @@ __device__ __forceinline__ U128 window_join(uint64_t s, const U128 &lo, const U1
   return r;
 }
 
+// NT: the aligned chunks are read with non-temporal loads (data read once).
+template <bool NT = false>
 __device__ __forceinline__ U128 load_window16(const unsigned char *base, uint64_t s) {
   const uint64_t a = s & ~uint64_t(15);
   const uint32_t o = uint32_t(s & 15u);
-  const U128 lo = *reinterpret_cast<const U128 *>(base + a);
+  const U128 *pa = reinterpret_cast<const U128 *>(base + a);
+  const U128 lo = NT ? __builtin_nontemporal_load(pa) : *pa;
   if (o == 0) return lo;
-  const U128 hi = *reinterpret_cast<const U128 *>(base + a + 16);
+  const U128 hi = NT ? __builtin_nontemporal_load(pa + 1) : pa[1];
   const uint32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
   const uint32_t q = o >> 2, b = o & 3u;
   uint32_t v[5];

@@ -53,6 +53,7 @@ struct OopParams {
   // skewed form: workgroups from per_xcd * 8 on write the chunks holding row boundaries
   uint64_t fix_rows;                // rows of all blocks (one more chunk: the end)
   FastDiv64 div_C;
+  uint32_t nt;                      // bit 0: non-temporal source loads; bit 1: stores
 };
 
 template <int ES, int SW>
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
         } u;
         const uint64_t need = (sb & 15u) ? (sb & ~uint64_t(15)) + 32 : sb + 16;
         if (need <= p.src_end) {
-          u.x = load_window16(p.src, sb);
+          u.x = (p.nt & 1u) ? load_window16<true>(p.src, sb) : load_window16(p.src, sb);
         } else {  // window would run past the readable end: whole elements only
 #pragma unroll
           for (uint32_t k = 0; k < EPV; k++)
@@ -213,7 +214,11 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
         x.y = (s == 0) ? y.y : (s == 1) ? y.x : (s == 2) ? y.w : y.z;
         x.z = (s == 0) ? y.z : (s == 1) ? y.w : (s == 2) ? y.x : y.y;
         x.w = (s == 0) ? y.w : (s == 1) ? y.z : (s == 2) ? y.y : y.x;
-        *reinterpret_cast<U128 *>(dst + (c0 + c) * p.dst_pitch + r0 * ES + (j << 4)) = x;
+        U128 *q = reinterpret_cast<U128 *>(dst + (c0 + c) * p.dst_pitch + r0 * ES + (j << 4));
+        if (p.nt & 2u)
+          __builtin_nontemporal_store(x, q);
+        else
+          *q = x;
       }
     }
   } else if (OUT == 2) {
@@ -244,7 +249,10 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
       const uint32_t o = 16u - m + 16u * j;
       const U128 lo = chunk(o >> 4);
       const U128 hi = (o & 15u) ? chunk((o >> 4) + 1) : lo;
-      *reinterpret_cast<U128 *>(row + x0) = window_join(o, lo, hi);
+      if (p.nt & 2u)
+        __builtin_nontemporal_store(window_join(o, lo, hi), reinterpret_cast<U128 *>(row + x0));
+      else
+        *reinterpret_cast<U128 *>(row + x0) = window_join(o, lo, hi);
     }
   } else if (ES <= 2 || p.wstore) {
     // Destination rows not 16-byte aligned: per row, the elements before the first aligned
@@ -669,6 +677,7 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
   p.tc_log2 = tcl;
   p.PB = PB;
   p.wstore = uint32_t(tuning("st_wstore"));
+  p.nt = uint32_t(tuning("oop_nt"));
   const uint32_t TC = 1u << tcl;
   p.ntr = (R + (1ull << trl) - 1) >> trl;
   p.ntc = (C + (1ull << tcl) - 1) >> tcl;
