@@ -75,7 +75,7 @@ __device__ __forceinline__ void load_piece(typename Elem<UB>::T (&d)[UPL],
 #pragma unroll
   for (int k = 0; k < UPL; k++) {
     uint32_t off = (sub + k * vw) * UB;
-    if (off < pbytes) d[k] = *reinterpret_cast<const V *>(src + off);
+    if (off < pbytes) d[k] = __builtin_nontemporal_load(reinterpret_cast<const V *>(src + off));
   }
 }
 
@@ -87,7 +87,7 @@ __device__ __forceinline__ void store_piece(unsigned char *dst,
 #pragma unroll
   for (int k = 0; k < UPL; k++) {
     uint32_t off = (sub + k * vw) * UB;
-    if (off < pbytes) *reinterpret_cast<V *>(dst + off) = d[k];
+    if (off < pbytes) __builtin_nontemporal_store(d[k], reinterpret_cast<V *>(dst + off));
   }
 }
 
