@@ -345,6 +345,8 @@ def main():
             d = ref = None
         extra["batch_4096_mixed"] = extra_batch(lm)
         extra["serve_poisson_2000"] = extra_serve(lm)
+        # three times the load for half a second: requests queue, so admission matters
+        extra["serve_poisson_6000"] = extra_serve(lm, rate=6000.0, duration=0.5)
     del ref, d
     torch.cuda.empty_cache()
 

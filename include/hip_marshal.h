@@ -246,6 +246,7 @@ void hip_marshal_set_profiling(int on);
  * "st_tile_kb" (LDS tile of the out-of-place transpose), "st_pt_vw", "st_rotate" (0/1),
  * "copy_nt" (0/1: non-temporal source loads in the copy into scratch), "oop_nt" (bit 0:
  * non-temporal loads, bit 1: non-temporal stores in the out-of-place transpose; default 2),
+ * "pad_nt" (0/1: non-temporal loads and stores in the vector padding kernel),
  * "st_wstore" (0/1: window stores for every element size), "st_skew" (0 off, 1 for 1- and
  * 2-byte elements (default), 2 up to 8-byte ones: skewed tiles for misaligned destination
  * rows), "gather_row_bytes" (destination rows shorter than this and misaligned take the
@@ -256,7 +257,7 @@ void hip_marshal_set_profiling(int on);
  * on the batch's side streams instead of in a grouped kernel).
  * Replaces the reference's hard-coded work-group tables (src/cl_marshal.cc:165-231,
  * 388-465). The setter returns true on an unknown key or a value outside [min, 2^30] (min is
- * 0 for "st_scratch_mb", "st_rotate", "st_wstore", "st_skew", "copy_nt", "oop_nt",
+ * 0 for "st_scratch_mb", "st_rotate", "st_wstore", "st_skew", "copy_nt", "oop_nt", "pad_nt",
  * "gather_row_bytes", "st_tile_rows", "pad_shape" and "autotune", 1 otherwise); the getter
  * returns 0 for an unknown key. */
 bool hip_marshal_set_tuning(const char *key, int64_t value);

@@ -48,6 +48,13 @@ def lib():
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `python -m libmarshal_amd.build` "
             "(or __graft_entry__.build())")
+    # One HIP runtime per process: torch ships its own libamdhip64 under the same soname. When
+    # torch is loaded first the library binds to it; loaded before torch, the library would
+    # pull in the system runtime and torch a second one, and the two do not share the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, i64, u64p, sz, i32, b = (ctypes.c_void_p, ctypes.c_int64,
                                  ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t,

@@ -95,6 +95,7 @@ Knob g_knobs[] = {
     {"st_skew", {1}, 0},
     {"copy_nt", {1}, 0},
     {"oop_nt", {2}, 0},
+    {"pad_nt", {1}, 0},
     {"batch_solo_kb", {1024}},
 };
 Knob *find_knob(const char *key) {

@@ -97,11 +97,13 @@ __device__ __forceinline__ uint32_t flag_or(uint32_t *p, uint32_t mask) {
 // multiple of 16 the load covers bytes up to (s & ~15) + 32, which the caller keeps in bounds.
 // Split form: window_raw issues the loads, window_join shifts (call it after other loads have
 // been issued so their latencies overlap).
+template <bool NT = false>
 __device__ __forceinline__ void window_raw(const unsigned char *base, uint64_t s, U128 &lo,
                                            U128 &hi) {
   const uint64_t a = s & ~uint64_t(15);
-  lo = *reinterpret_cast<const U128 *>(base + a);
-  if (s & 15u) hi = *reinterpret_cast<const U128 *>(base + a + 16);
+  const U128 *pa = reinterpret_cast<const U128 *>(base + a);
+  lo = NT ? __builtin_nontemporal_load(pa) : *pa;
+  if (s & 15u) hi = NT ? __builtin_nontemporal_load(pa + 1) : pa[1];
 }
 __device__ __forceinline__ U128 window_join(uint64_t s, const U128 &lo, const U128 &hi) {
   const uint32_t o = uint32_t(s & 15u);

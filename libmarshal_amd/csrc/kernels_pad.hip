@@ -47,6 +47,7 @@ struct PadParams {
   uint32_t *flags;  // [0] = id counter, [1 + k] = state of group k
   FastDiv64 div_pad, div_x;
   bool unpad;
+  uint32_t nt;  // vector form: non-temporal loads and stores
 };
 
 // Groups are numbered by a dynamic id in processing order (tail first for padding, front
@@ -216,7 +217,10 @@ __global__ __launch_bounds__(NT) void k_pad_vec(PadParams p) {
     if (c + EPV <= X && d + EPV <= live && (sb & ~uint64_t(15)) + 32 <= bytes) {
       win[j] = true;
       wsb[j] = sb;
-      window_raw(p.base, sb, wlo[j], whi[j]);
+      if (p.nt)
+        window_raw<true>(p.base, sb, wlo[j], whi[j]);
+      else
+        window_raw(p.base, sb, wlo[j], whi[j]);
     } else {
 #pragma unroll
       for (uint32_t e = 0; e < EPV; e++) {
@@ -243,7 +247,10 @@ __global__ __launch_bounds__(NT) void k_pad_vec(PadParams p) {
     const uint64_t vi = lo + threadIdx.x + uint64_t(j) * NT;
     if (vi >= hi) continue;
     if ((vi + 1) * 16 <= bytes) {
-      *reinterpret_cast<U128 *>(p.base + vi * 16) = v[j].v;
+      if (p.nt)
+        __builtin_nontemporal_store(v[j].v, reinterpret_cast<U128 *>(p.base + vi * 16));
+      else
+        *reinterpret_cast<U128 *>(p.base + vi * 16) = v[j].v;
     } else {
       E *out = reinterpret_cast<E *>(p.base);
       for (uint32_t e = 0; e < EPV; e++)
@@ -285,6 +292,7 @@ hipError_t launch_padding(unsigned char *base, size_t es, uint64_t x, uint64_t y
   p.unpad = unpad;
   p.div_pad = FastDiv64(pad ? pad : 1);
   p.div_x = FastDiv64(x ? x : 1);
+  p.nt = uint32_t(tuning("pad_nt"));
   if (p.ntiles == 0) return hipSuccess;
   hipError_t e = launch_fill_u32(flags, pad_flag_words(p.total, es), 0, st);
   if (e != hipSuccess) return e;

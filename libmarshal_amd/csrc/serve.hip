@@ -1,16 +1,17 @@
 // serve.hip - a request loop with continuous batching for in-place layout conversions.
 //
-// Clients submit conversions of device-resident buffers; a scheduler thread admits queued
-// requests in arrival order while the bytes of admitted, unfinished requests stay within the
-// HBM budget (a request's footprint is its buffer plus visited bits; the scratch of the
-// through-scratch and stripe forms is per stream, at most the scratch cap each), packs the
-// small ones admitted together into one dynamic batch and launches it as grouped persistent
-// kernels (the batched form's work lists) on whichever of its streams is idle; a request of
-// 64 MiB or more, or one admitted alone, runs with its plain stage list (per-request
-// kernels, the through-scratch and stripe forms included) and the stream's own scratch.
-// Requests that arrive while batches are on the GPU are admitted as soon as budget and a
-// stream free up and run concurrently with them. Completion is observed with an event per
-// batch; latency is arrival -> completion.
+// Clients submit conversions of device-resident buffers. Requests are admitted in arrival
+// order while the bytes of admitted, unfinished requests stay within the HBM budget (a
+// request's footprint is its buffer plus visited bits; the scratch of the through-scratch
+// and stripe forms is per stream, at most the scratch cap each). Small requests admitted
+// together form one dynamic batch of grouped persistent kernels (the batched form's work
+// lists) on an idle stream; a small request admitted alone, and every request of 64 MiB or
+// more, runs with its plain stage list and the stream's own scratch. Large requests run one
+// at a time (sharing HBM bandwidth between them only delays each); small requests may go
+// ahead of a large one that waits for its turn. A submission that finds a stream idle is
+// started on the submitting thread; otherwise the scheduler thread starts it as streams and
+// budget free up. Completion is observed with an event per stream, polled closely while work
+// is in flight; latency is arrival -> completion.
 //
 // No counterpart in the reference (a single-call library, src/cl_marshal.cc).
 #include "batch.h"
@@ -54,6 +55,8 @@ struct Slot {
   unsigned char *scr = nullptr;  // scratch of a request run alone (visited bits, stripes)
   size_t scr_cap = 0;
   bool busy = false;
+  bool launching = false;  // work is being enqueued (lock released): not yet retirable
+  bool solo = false;       // runs one large request
   std::vector<int64_t> reqs;
   size_t bytes = 0;
 };
@@ -71,6 +74,7 @@ struct Server {
   std::unordered_map<int64_t, Req> reqs;
   int64_t next_id = 0;
   size_t inflight = 0;
+  int solo_running = 0;  // large requests on the GPU (at most one at a time)
   std::vector<Slot> slots;
   std::thread th;
   bool stop = false;
@@ -137,93 +141,147 @@ bool launch_solo(Server &s, Slot &sl, const hip_marshal_desc &d) {
   return hipEventRecord(sl.ev, sl.st) == hipSuccess;
 }
 
+// Starts `ids` (already taken off the queue) on the idle slot `sl`. Called with the lock held;
+// releases it while the work is enqueued and returns with it held. `plain`: one request on
+// its plain stage list, else a dynamic batch of grouped kernels.
+void start_on_slot(Server &s, Slot &sl, const std::vector<int64_t> &ids,
+                   const std::vector<hip_marshal_desc> &descs, size_t add, bool plain,
+                   bool solo, std::unique_lock<std::mutex> &lk) {
+  const double t = now_s();
+  for (int64_t id : ids) {
+    s.reqs[id].state = 1;
+    s.reqs[id].t_launch = t;
+  }
+  s.inflight += add;
+  sl.busy = true;
+  sl.launching = true;
+  sl.solo = solo;
+  sl.bytes = add;
+  sl.reqs = ids;
+  if (solo) s.solo_running++;
+  s.batches++;
+  s.max_in_batch = std::max<int64_t>(s.max_in_batch, int64_t(ids.size()));
+  lk.unlock();
+  int prev = -1;
+  hipGetDevice(&prev);
+  if (prev != s.device) hipSetDevice(s.device);
+  const bool ok = plain ? launch_solo(s, sl, descs[0]) : launch_batch(s, sl, descs);
+  if (prev != s.device && prev >= 0) hipSetDevice(prev);
+  lk.lock();
+  sl.launching = false;
+  if (!ok) {  // mark failed once the stream has drained
+    hipStreamSynchronize(sl.st);
+    for (int64_t id : ids) {
+      s.reqs[id].state = -1;
+      s.failed++;
+    }
+    s.inflight -= add;
+    if (sl.solo) s.solo_running--;
+    sl.busy = false;
+    sl.solo = false;
+    sl.bytes = 0;
+    sl.reqs.clear();
+    s.cv.notify_all();
+  }
+}
+
+// Retires finished slots. Lock held.
+bool retire(Server &s) {
+  bool progressed = false;
+  for (Slot &sl : s.slots) {
+    if (!sl.busy || sl.launching) continue;
+    hipError_t q = hipEventQuery(sl.ev);
+    if (q == hipErrorNotReady) continue;
+    double t = now_s();
+    for (int64_t id : sl.reqs) {
+      Req &r = s.reqs[id];
+      r.t_done = t;
+      r.state = q == hipSuccess ? 2 : -1;
+      if (q == hipSuccess) {
+        s.lat.push_back(t - r.t_arrive);
+        s.bytes_done += 2.0 * double(r.d.rows) * double(r.d.cols) * double(r.d.elem_bytes);
+      } else {
+        s.failed++;
+      }
+      s.t_last = std::max(s.t_last, t);
+    }
+    s.inflight -= sl.bytes;
+    if (sl.solo) s.solo_running--;
+    sl.bytes = 0;
+    sl.reqs.clear();
+    sl.busy = false;
+    sl.solo = false;
+    progressed = true;
+  }
+  if (progressed) s.cv.notify_all();
+  return progressed;
+}
+
+// Admits queued requests onto idle slots while the budget allows. Lock held (released while
+// launching). Order: the queue head if it can start; a large request (>= solo_bytes) waits
+// while another large one runs, and small requests behind it may then go ahead of it, so
+// large requests never share the GPU with each other and small ones never wait behind them.
+// Small requests taken together form one dynamic batch; one taken alone runs on its plain
+// stage list.
+void admit(Server &s, std::unique_lock<std::mutex> &lk) {
+  while (!s.queue.empty()) {
+    Slot *free_slot = nullptr;
+    for (Slot &sl : s.slots)
+      if (!sl.busy) {
+        free_slot = &sl;
+        break;
+      }
+    if (!free_slot) return;
+    std::vector<int64_t> ids;
+    std::vector<hip_marshal_desc> descs;
+    size_t add = 0;
+    bool solo = false;
+    const Req &head = s.reqs[s.queue.front()];
+    if (head.bytes >= s.solo_bytes && s.solo_running == 0) {
+      if (s.inflight + head.bytes > s.budget) return;  // wait for budget to free up
+      solo = true;
+      ids.push_back(s.queue.front());
+      descs.push_back(head.d);
+      add = head.bytes;
+      s.queue.pop_front();
+    } else {
+      for (auto it = s.queue.begin(); it != s.queue.end() && int(ids.size()) < s.max_batch;) {
+        const Req &r = s.reqs[*it];
+        if (r.bytes >= s.solo_bytes) {  // a large one waiting for its turn stays queued
+          ++it;
+          continue;
+        }
+        if (s.inflight + add + r.bytes > s.budget) break;
+        add += r.bytes;
+        ids.push_back(*it);
+        descs.push_back(r.d);
+        it = s.queue.erase(it);
+      }
+      if (ids.empty()) return;
+    }
+    start_on_slot(s, *free_slot, ids, descs, add, solo || ids.size() == 1, solo, lk);
+  }
+}
+
 void scheduler(Server *sp) {
   Server &s = *sp;
   hipSetDevice(s.device);
   std::unique_lock<std::mutex> lk(s.mu);
   while (true) {
-    bool progressed = false;
-    // 1. retire finished batches
-    for (Slot &sl : s.slots) {
-      if (!sl.busy) continue;
-      hipError_t q = hipEventQuery(sl.ev);
-      if (q == hipErrorNotReady) continue;
-      double t = now_s();
-      for (int64_t id : sl.reqs) {
-        Req &r = s.reqs[id];
-        r.t_done = t;
-        r.state = q == hipSuccess ? 2 : -1;
-        if (q == hipSuccess) {
-          s.lat.push_back(t - r.t_arrive);
-          s.bytes_done += 2.0 * double(r.d.rows) * double(r.d.cols) * double(r.d.elem_bytes);
-        } else {
-          s.failed++;
-        }
-        s.t_last = std::max(s.t_last, t);
-      }
-      s.inflight -= sl.bytes;
-      sl.bytes = 0;
-      sl.reqs.clear();
-      sl.busy = false;
-      progressed = true;
-    }
-    if (progressed) s.cv.notify_all();
-    // 2. admit in arrival order and launch on an idle stream
-    Slot *free_slot = nullptr;
-    for (Slot &sl : s.slots)
-      if (!sl.busy) { free_slot = &sl; break; }
-    if (free_slot && !s.queue.empty()) {
-      std::vector<int64_t> ids;
-      std::vector<hip_marshal_desc> descs;
-      size_t add = 0;
-      const bool solo = s.reqs[s.queue.front()].bytes >= s.solo_bytes;
-      while (!s.queue.empty() && int(ids.size()) < s.max_batch) {
-        Req &r = s.reqs[s.queue.front()];
-        if (s.inflight + add + r.bytes > s.budget) break;  // wait for budget to free up
-        if (!ids.empty() && (solo || r.bytes >= s.solo_bytes)) break;  // large ones alone
-        add += r.bytes;
-        ids.push_back(s.queue.front());
-        descs.push_back(r.d);
-        s.queue.pop_front();
-      }
-      if (!ids.empty()) {
-        double t = now_s();
-        for (int64_t id : ids) {
-          s.reqs[id].state = 1;
-          s.reqs[id].t_launch = t;
-        }
-        s.inflight += add;
-        free_slot->busy = true;
-        free_slot->bytes = add;
-        free_slot->reqs = ids;
-        s.batches++;
-        s.max_in_batch = std::max<int64_t>(s.max_in_batch, int64_t(ids.size()));
-        lk.unlock();
-        // A request admitted alone gains nothing from the grouped kernels: it takes the
-        // plain path, with no parameter upload.
-        const bool plain = solo || ids.size() == 1;
-        bool ok = plain ? launch_solo(s, *free_slot, descs[0]) : launch_batch(s, *free_slot, descs);
-        lk.lock();
-        if (!ok) {  // mark failed; the slot is retired once its stream drains
-          hipStreamSynchronize(free_slot->st);
-          for (int64_t id : ids) {
-            s.reqs[id].state = -1;
-            s.failed++;
-          }
-          s.inflight -= add;
-          free_slot->busy = false;
-          free_slot->bytes = 0;
-          free_slot->reqs.clear();
-          s.cv.notify_all();
-        }
-        continue;  // look for more work right away
-      }
-    }
+    retire(s);
+    admit(s, lk);
     bool any_busy = false;
     for (Slot &sl : s.slots) any_busy |= sl.busy;
     if (s.stop && s.queue.empty() && !any_busy) break;
-    // 3. idle: wake on a submission, or poll running batches every 20 us
-    s.cv.wait_for(lk, std::chrono::microseconds(any_busy ? 20 : 1000));
+    if (any_busy) {
+      // Work on the GPU: poll its completion closely (a timed wait oversleeps by tens of
+      // microseconds), at the cost of one host core while requests are in flight.
+      lk.unlock();
+      for (int i = 0; i < 256; i++) __builtin_ia32_pause();
+      lk.lock();
+    } else {
+      s.cv.wait_for(lk, std::chrono::milliseconds(1));  // woken by a submission
+    }
   }
 }
 
@@ -281,7 +339,7 @@ int64_t hip_marshal_server_submit_at(hip_marshal_server *h, const hip_marshal_de
                     "the HBM budget\n");
     return -1;
   }
-  std::lock_guard<std::mutex> lk(s.mu);
+  std::unique_lock<std::mutex> lk(s.mu);
   int64_t id = s.next_id++;
   Req &r = s.reqs[id];
   r.d = *d;
@@ -289,6 +347,8 @@ int64_t hip_marshal_server_submit_at(hip_marshal_server *h, const hip_marshal_de
   r.t_arrive = arrival > 0 ? arrival : now_s();
   if (s.t_first < 0 || r.t_arrive < s.t_first) s.t_first = r.t_arrive;
   s.queue.push_back(id);
+  // Start it right here when a stream is idle: no hand-over to the scheduler thread.
+  admit(s, lk);
   s.cv.notify_all();
   return id;
 }
