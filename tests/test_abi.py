@@ -85,3 +85,39 @@ def test_lds_budget_knob():
     assert lm.lds_budget() == 4096
     lm.set_lds_budget(0)
     assert lm.lds_budget() == default
+
+
+def test_planner_route_choice_host_only():
+    """The cost-model planner runs on the host: square matrices take the tile-pair swaps,
+    non-square ones within the scratch cap the through-scratch route, ones beyond it the
+    stripe route; a scratch cap of 0 leaves only in-place routes."""
+    import libmarshal_amd as lm
+    keep = lm.get_tuning("st_scratch_mb")
+    try:
+        assert lm.plan_transpose(4096, 4096, 8)[2] == 8
+        assert lm.plan_transpose(3001, 7001, 4)[2] == 9
+        assert lm.plan_transpose(98304, 131071, 8)[2] in (6, 7)
+        assert lm.plan_transpose(250007, 199999, 1)[2] in (6, 7)
+        lm.set_tuning("st_scratch_mb", 0)
+        for (M, N, es) in ((3001, 7001, 4), (1031, 997, 1), (1800, 7200, 4)):
+            assert lm.plan_transpose(M, N, es)[2] not in (6, 7, 9), (M, N, es)
+    finally:
+        lm.set_tuning("st_scratch_mb", keep)
+    assert lm.plan_transpose(3001, 7001, 4)[2] == 9  # knob changes re-plan
+
+
+def test_tuning_keys_defaults_and_limits():
+    import libmarshal_amd as lm
+    defaults = {"st_scratch_mb": 2048, "batch_solo_kb": 1024, "st_skew": 1, "copy_nt": 1,
+                "oop_nt": 2, "pad_nt": 1, "gather_tile_kb": 24, "gather_row_bytes": 256,
+                "bs_upt": 16, "autotune": 0}
+    for k, v in defaults.items():
+        assert lm.get_tuning(k) == v, k
+    assert lm.get_tuning("no_such_key") == 0
+    with pytest.raises(ValueError):
+        lm.set_tuning("no_such_key", 1)
+    with pytest.raises(ValueError):
+        lm.set_tuning("bs_upt", 0)  # below its minimum of 1
+    lm.set_tuning("st_skew", 0)  # 0 allowed
+    assert lm.get_tuning("st_skew") == 0
+    lm.set_tuning("st_skew", 1)
