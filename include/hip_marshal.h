@@ -247,6 +247,9 @@ void hip_marshal_set_profiling(int on);
  * "copy_nt" (0/1: non-temporal source loads in the copy into scratch), "oop_nt" (bit 0:
  * non-temporal loads, bit 1: non-temporal stores in the out-of-place transpose; default 2),
  * "pad_nt" (0/1: non-temporal loads and stores in the vector padding kernel),
+ * "pl_min_bytes", "pl_max_order", "pl_min_walkers" (when a 100 / 0100 step takes the
+ * cycle-leader form: super-elements of at least this many bytes, an order of R modulo R*C - 1
+ * at most this, at least this many walkers),
  * "st_wstore" (0/1: window stores for every element size), "st_skew" (0 off, 1 for 1- and
  * 2-byte elements (default), 2 up to 8-byte ones: skewed tiles for misaligned destination
  * rows), "gather_row_bytes" (destination rows shorter than this and misaligned take the
@@ -286,6 +289,13 @@ bool hip_marshal_plan_transpose(int64_t M, int64_t N, size_t elem_bytes, int64_t
 const char *hip_marshal_last_route(void);
 /* Grows the current device's flag pool to at least `bytes` (false on success). */
 bool hip_marshal_reserve(size_t bytes);
+/* Cycles of length >= 2 of the in-place transpose of an R x C grid (the 100 map), counted
+ * from gcds without walking them, when the order of R modulo R*C - 1 (which every cycle length
+ * divides; stored in *order) is at most max_order and R*C < 2^32; else 0. These are the
+ * walkers of the claim-free cycle-leader form of 100 / 0100 (route tag "0100_CYC-"). No
+ * counterpart in the reference, which counts cycles by walking them
+ * (Transposition::GetNumCycles, include/plan.hpp:57-73). Host only. */
+uint64_t hip_marshal_cycle_count(int64_t R, int64_t C, uint32_t max_order, uint32_t *order);
 /* Bytes of visited-bit flags the given stage needs (0 for LDS-staged stages). */
 size_t hip_marshal_flag_bytes_0100(int64_t A, int64_t a, int64_t B, int64_t b,
                                    size_t elem_bytes);

@@ -89,6 +89,8 @@ def lib():
         "hip_marshal_last_route": (ctypes.c_char_p, []),
         "hip_marshal_reserve": (b, [sz]),
         "hip_marshal_flag_bytes_0100": (sz, [i64, i64, i64, i64, sz]),
+        "hip_marshal_cycle_count": (ctypes.c_uint64, [i64, i64, ctypes.c_uint32,
+                                                      ctypes.POINTER(ctypes.c_uint32)]),
         "hip_marshal_version": (ctypes.c_char_p, []),
         "hip_marshal_set_tuning": (b, [ctypes.c_char_p, i64]),
         "hip_marshal_autotune": (b, [vp, vp, sz, i64, i64, ctypes.POINTER(ctypes.c_int)]),
@@ -137,7 +139,8 @@ EXPORTED = [
     "hip_asta_aos_es", "hip_soa_asta_es", "hip_asta_soa_es", "hip_padding_es",
     "hip_unpadding_es", "hip_marshal_set_lds_budget", "hip_marshal_get_lds_budget",
     "hip_marshal_set_profiling", "hip_marshal_last_route", "hip_marshal_reserve",
-    "hip_marshal_flag_bytes_0100", "hip_marshal_version", "hip_marshal_set_tuning",
+    "hip_marshal_flag_bytes_0100", "hip_marshal_cycle_count", "hip_marshal_version",
+    "hip_marshal_set_tuning",
     "hip_marshal_get_tuning", "hip_marshal_plan_transpose", "hip_marshal_autotune", "hip_marshal_batch_create", "hip_marshal_batch_run",
     "hip_marshal_batch_info", "hip_marshal_batch_route", "hip_marshal_batch_destroy",
     "hip_marshal_server_create", "hip_marshal_server_submit", "hip_marshal_server_submit_at",
@@ -287,6 +290,14 @@ def autotune(buf, M, N, elem_bytes=None, stream=None):
                                   ctypes.byref(st)):
         raise RuntimeError("hip_marshal_autotune failed")
     return st.value
+
+
+def cycle_count(R, C, max_order=64):
+    """(cycles, order) of the R x C transpose map when its order is at most max_order (host
+    only), else (0, 0)."""
+    k = ctypes.c_uint32(0)
+    n = lib().hip_marshal_cycle_count(R, C, max_order, ctypes.byref(k))
+    return (n, k.value) if n else (0, 0)
 
 
 def plan_transpose(M, N, elem_bytes):

@@ -96,6 +96,9 @@ Knob g_knobs[] = {
     {"copy_nt", {1}, 0},
     {"oop_nt", {2}, 0},
     {"pad_nt", {1}, 0},
+    {"pl_min_bytes", {128}},
+    {"pl_max_order", {64}},
+    {"pl_min_walkers", {16384}},
     {"batch_solo_kb", {1024}},
 };
 Knob *find_knob(const char *key) {
@@ -146,6 +149,52 @@ bool pool_grow(Pool &p, size_t bytes) {
   LM_CHECK_HIP(hipMalloc(&p.ptr, want));
   p.cap = want;
   return false;
+}
+
+// Cycle-leader lists by (device, R, C): a list depends only on the grid, so a shape that
+// recurs skips the search. Entries live until hip_marshal_finalize, up to kLeaderCacheBytes
+// in all. An entry is created only outside stream capture; later users wait for the event
+// that follows its search (or, while capturing, use it only once that search has finished).
+struct LeaderEntry {
+  uint32_t *list = nullptr;
+  hipEvent_t ready = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_lc_mu;
+std::map<std::tuple<int, uint64_t, uint64_t>, LeaderEntry> g_lc;
+size_t g_lc_bytes = 0;
+constexpr size_t kLeaderCacheBytes = size_t(256) << 20;
+
+const uint32_t *cached_leaders(uint64_t R, uint64_t C, const PlPlan &pc, hipStream_t st) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipStreamIsCapturing(st, &cs);
+  const bool capturing = cs != hipStreamCaptureStatusNone;
+  std::lock_guard<std::mutex> lk(g_lc_mu);
+  const auto key = std::make_tuple(dev, R, C);
+  auto it = g_lc.find(key);
+  if (it != g_lc.end()) {
+    if (capturing) return hipEventQuery(it->second.ready) == hipSuccess ? it->second.list : nullptr;
+    if (hipStreamWaitEvent(st, it->second.ready, 0) != hipSuccess) return nullptr;
+    return it->second.list;
+  }
+  const size_t bytes = size_t(pc.cycles + 1) * 4;
+  if (capturing || g_lc_bytes + bytes > kLeaderCacheBytes) return nullptr;
+  LeaderEntry e;
+  e.bytes = bytes;
+  if (hipMalloc(reinterpret_cast<void **>(&e.list), bytes) != hipSuccess) return nullptr;
+  if (hipEventCreateWithFlags(&e.ready, hipEventDisableTiming) != hipSuccess ||
+      launch_find_leaders(e.list, R, C, pc, st) != hipSuccess ||
+      hipEventRecord(e.ready, st) != hipSuccess) {
+    hipStreamSynchronize(st);
+    hipFree(e.list);
+    if (e.ready) hipEventDestroy(e.ready);
+    return nullptr;
+  }
+  g_lc[key] = e;
+  g_lc_bytes += bytes;
+  return e.list;
 }
 
 struct Lease {
@@ -323,11 +372,24 @@ void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64
   r.d1 = R;
   r.d2 = C;
   r.d3 = b;
-  plan_100(es, inst, R, C, b, &r.pt);
+  plan_100(es, inst, R, C, b, &r.pt, 0, reinterpret_cast<uintptr_t>(buf));
   // Many small instances: visited bits of each instance in LDS, one workgroup per instance.
   const bool lds_form = inst >= uint64_t(tuning("lds0100_min_inst")) &&
                         R * C * r.pt.nsub < (1ull << 31) &&
                         lds0100_bytes(R, C, r.pt) <= lds_budget();
+  // Short cycles (small order of R modulo R*C - 1) over super-elements of some size: one
+  // in-place pass with no claims, no scratch beyond the leader list.
+  if (allow_oop && b * es >= uint64_t(tuning("pl_min_bytes"))) {
+    PlPlan pc;
+    if (plan_100_leaders(R, C, uint32_t(tuning("pl_max_order")), &pc) &&
+        pc.cycles * inst * r.pt.nsub >= uint64_t(tuning("pl_min_walkers"))) {
+      r.kind = StageKind::PT100C;
+      r.pc = pc;
+      if (tag) *tag = "0100_CYC-";
+      out.push_back(r);
+      return;
+    }
+  }
   if (allow_oop) {
     std::vector<StageRec> o;
     if (push_oop(o, buf, es, inst, R, C, b * es)) {
@@ -506,7 +568,7 @@ bool push_st(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t
   StageRec r{};
   if (!plan_st(es, M, N, variant, uint64_t(tuning("st_super_bytes")),
                uint64_t(tuning("st_scratch_mb")) << 20, tuning("st_pt_vw"),
-               tuning("st_rotate") != 0, &r.stp))
+               tuning("st_rotate") != 0, &r.stp, reinterpret_cast<uintptr_t>(buf)))
     return false;
   r.kind = StageKind::ST;
   r.es = es;
@@ -613,6 +675,7 @@ size_t stages_scratch_bytes(const std::vector<StageRec> &stages) {
   size_t m = 0;
   for (const StageRec &r : stages) {
     if (r.kind == StageKind::PT100G) m = std::max<size_t>(m, r.pt.flag_words * 4);
+    if (r.kind == StageKind::PT100C) m = std::max<size_t>(m, size_t(r.pc.cycles + 1) * 4);
     if (r.kind == StageKind::ST) m = std::max<size_t>(m, r.stp.scratch_bytes);
     if (r.kind == StageKind::OOP) m = std::max<size_t>(m, size_t(r.chunk * r.d1 * r.d2 * r.d3));
   }
@@ -692,6 +755,25 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
         if (!buf) return true;
         LM_CHECK_HIP(run_st(r.buf, r.stp, buf, size_t(tuning("st_tile_kb")) << 10,
                             grid_cap(uint32_t(tuning("pt100_blocks_per_cu"))), st));
+        break;
+      }
+      case StageKind::PT100C: {
+        const uint32_t cap = grid_cap(uint32_t(tuning("pt100_blocks_per_cu")));
+        const uint32_t *cached = cached_leaders(r.d1, r.d2, r.pc, st);
+        if (cached) {
+          LM_CHECK_HIP(launch_leader_walk(r.buf, cached, r.d0, r.d1, r.d2, r.d3 * r.es, r.pt,
+                                          r.pc, cap, st));
+          break;
+        }
+        std::unique_ptr<Lease> lease;
+        uint32_t *list = reinterpret_cast<uint32_t *>(scr);
+        if (!list) {
+          lease.reset(new Lease(pool_here(), st));
+          list = lease->get((r.pc.cycles + 1) * 4);
+        }
+        if (!list) return true;
+        LM_CHECK_HIP(launch_100_leaders(r.buf, list, r.d0, r.d1, r.d2, r.d3 * r.es, r.pt, r.pc,
+                                        cap, st));
         break;
       }
       case StageKind::PT100G: {
@@ -785,6 +867,7 @@ double stage_bw_tbs(const StageRec &r) {
       double S = double(r.d3 * es);
       return std::min(3.0, 0.12 + 0.011 * S);
     }
+    case StageKind::PT100C: return std::min(5.0, 0.02 * double(r.d3 * es));
     case StageKind::RC_ROW: return r.rc.ub >= 4 ? 3.0 : r.rc.ub == 2 ? 2.0 : 1.2;
     case StageKind::RC_COL1:
     case StageKind::RC_COL2: return r.rc.ub >= 4 ? 0.55 : r.rc.ub == 2 ? 0.4 : 0.3;
@@ -818,6 +901,7 @@ double stage_bytes(const StageRec &r) {
     case StageKind::BS010:
     case StageKind::PT010: return double(r.d0) * double(r.d1) * double(r.d2) * double(r.es);
     case StageKind::PT100G:
+    case StageKind::PT100C:
     case StageKind::PT0100L:
       return double(r.d0) * double(r.d1) * double(r.d2) * double(r.d3) * double(r.es);
     case StageKind::RC_ROW:
@@ -1084,6 +1168,13 @@ bool hip_marshal_reserve(size_t bytes) {
   return pool_grow(p, bytes);
 }
 
+uint64_t hip_marshal_cycle_count(int64_t R, int64_t C, uint32_t max_order, uint32_t *order) {
+  PlPlan pc;
+  if (R < 2 || C < 2 || !plan_100_leaders(uint64_t(R), uint64_t(C), max_order, &pc)) return 0;
+  if (order) *order = pc.order;
+  return pc.cycles;
+}
+
 size_t hip_marshal_flag_bytes_0100(int64_t A, int64_t a, int64_t B, int64_t b,
                                    size_t elem_bytes) {
   if (A < 1 || a < 1 || B < 1 || b < 1 || !valid_es(elem_bytes)) return 0;
@@ -1093,6 +1184,20 @@ size_t hip_marshal_flag_bytes_0100(int64_t A, int64_t a, int64_t B, int64_t b,
 }
 
 void hip_marshal_finalize(void) {
+  {
+    std::lock_guard<std::mutex> lk(g_lc_mu);
+    int cur = 0;
+    hipGetDevice(&cur);
+    for (auto &kv : g_lc) {
+      hipSetDevice(std::get<0>(kv.first));
+      hipEventSynchronize(kv.second.ready);
+      hipEventDestroy(kv.second.ready);
+      hipFree(kv.second.list);
+    }
+    g_lc.clear();
+    g_lc_bytes = 0;
+    hipSetDevice(cur);
+  }
   for (int d = 0; d < 64; d++) {
     Pool &p = g_pool[d];
     std::lock_guard<std::mutex> lk(p.mu);

@@ -412,7 +412,8 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
       if (plan_desc(d, list, r2, true) &&
           (r2.find("ST_") != std::string::npos ||
            (mbytes >= double(uint64_t(tuning("batch_solo_kb")) << 10) &&
-            (r2.find("SQ-") != std::string::npos || r2.find("OOP-") != std::string::npos)))) {
+            (r2.find("SQ-") != std::string::npos || r2.find("OOP-") != std::string::npos ||
+             r2.find("CYC-") != std::string::npos)))) {
         b.solo.push_back(std::move(list));
         b.solo_bytes.push_back(double(d.rows) * double(d.cols) * double(d.elem_bytes));
         b.routes[i] = r2;
@@ -521,6 +522,7 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
         case StageKind::ST:  // never planned for a batch (allow_st = false)
         case StageKind::SQ:
         case StageKind::OOP:
+        case StageKind::PT100C:
           fprintf(stderr, "[libmarshal] %s: stripe route in a batch\n", who);
           return false;
       }

@@ -70,13 +70,37 @@ struct Pt100Plan {
 };
 // vw_max > 0 overrides the "pt100_vw_max" knob (lanes per walker; pieces of up to 8 units of
 // 16 bytes per lane).
+// base_addr (0: unknown, taken as aligned): units never exceed the alignment of the buffer.
 void plan_100(size_t es, uint64_t inst, uint64_t R, uint64_t C, uint64_t b, Pt100Plan *out,
-              int64_t vw_max = 0);
+              int64_t vw_max = 0, uint64_t base_addr = 0);
 // rot_r / rot_c (< R / < C): the slot in row i, column j moves to row (j + rot_c) mod C,
 // column (i + rot_r) mod R of the transposed grid (a plain 100 has both 0).
 hipError_t launch_100_pttwac(unsigned char *base, uint32_t *flags, uint64_t inst, uint64_t R,
                              uint64_t C, uint64_t S, const Pt100Plan &pl, uint32_t max_blocks,
                              hipStream_t st, uint32_t rot_r = 0, uint32_t rot_c = 0);
+
+// ---- 100 / 0100 by cycle leaders: no visited bits ----------------------------------------
+// When the multiplicative order k of R modulo R*C - 1 is small, every cycle of the 100 map
+// has a length dividing k. Each cycle's smallest slot (its leader) is found on the device by
+// walking at most k steps from every slot; then one walker per (instance, leader, piece)
+// moves the whole cycle with no claims. `cycles` (of length >= 2, per instance) comes from
+// gcds on the host, which also sizes the leader list.
+struct PlPlan {
+  uint64_t cycles = 0;
+  uint32_t order = 0;
+};
+// False unless R, C >= 2, R*C < 2^32 and the order is at most max_order.
+bool plan_100_leaders(uint64_t R, uint64_t C, uint32_t max_order, PlPlan *out);
+// scratch: cycles + 1 32-bit words (the leader list and its counter).
+hipError_t launch_100_leaders(unsigned char *base, uint32_t *scratch, uint64_t inst,
+                              uint64_t R, uint64_t C, uint64_t S, const Pt100Plan &pl,
+                              const PlPlan &pc, uint32_t max_blocks, hipStream_t st);
+// The two halves: the search (into `list`, cycles + 1 words) and the walk over a list.
+hipError_t launch_find_leaders(uint32_t *list, uint64_t R, uint64_t C, const PlPlan &pc,
+                               hipStream_t st);
+hipError_t launch_leader_walk(unsigned char *base, const uint32_t *list, uint64_t inst,
+                              uint64_t R, uint64_t C, uint64_t S, const Pt100Plan &pl,
+                              const PlPlan &pc, uint32_t max_blocks, hipStream_t st);
 
 // 0100 with the visited bits of one instance in LDS (one workgroup per instance).
 size_t lds0100_bytes(uint64_t R, uint64_t C, const Pt100Plan &pl);
@@ -158,7 +182,8 @@ struct StPlan {
 // its neighbour's slot (no copy pass) and the 100 folds the one-slot rotation into its map;
 // otherwise (or with rotate false) stripes go through scratch in groups.
 bool plan_st(size_t es, uint64_t M, uint64_t N, int variant, uint64_t target_bytes,
-             uint64_t scratch_cap, int64_t pt_vw, bool rotate, StPlan *out);
+             uint64_t scratch_cap, int64_t pt_vw, bool rotate, StPlan *out,
+             uint64_t base_addr = 0);
 // `scratch` must hold plan.scratch_bytes and not overlap `buf`.
 hipError_t run_st(unsigned char *buf, const StPlan &p, unsigned char *scratch, size_t tile_bytes,
                   uint32_t pt_blocks, hipStream_t st);
@@ -175,7 +200,9 @@ int64_t tuning(const char *key);
 
 // ---- stage records (a route as a list of elementary stages) -----------------------------
 enum class M010 { Auto, BS, PTTWAC, Global };
-enum class StageKind : int { BS010, PT010, PT100G, PT0100L, RC_ROW, RC_COL1, RC_COL2, ST, SQ, OOP };
+enum class StageKind : int {
+  BS010, PT010, PT100G, PT0100L, PT100C, RC_ROW, RC_COL1, RC_COL2, ST, SQ, OOP
+};
 struct StageRec {
   StageKind kind;
   size_t es;
@@ -187,6 +214,7 @@ struct StageRec {
   RcPlan rc;                // RC_*: instances in d0, (M, N) in d1, d2, element bytes in d3
   StPlan stp;               // ST: the whole stripe route (M, N in d1, d2)
   uint64_t chunk;           // OOP: blocks per scratch fill (d0 blocks of d1 x d2, d3 bytes each)
+  PlPlan pc;                // PT100C: cycle count and order (pt: the walker shape)
 };
 struct Choice {
   double cost = 1e300;

@@ -31,6 +31,7 @@
 #include "kernels_dev.h"
 
 #include <algorithm>
+#include <vector>
 
 namespace lm {
 
@@ -78,9 +79,10 @@ static uint64_t next_pow2(uint64_t v) {
 }
 
 void plan_100(size_t es, uint64_t inst, uint64_t R, uint64_t C, uint64_t b, Pt100Plan *out,
-              int64_t vw_max) {
+              int64_t vw_max, uint64_t base_addr) {
   uint64_t S = b * es;
-  int ub = (S % 16 == 0) ? 16 : (S % 8 == 0) ? 8 : (S % 4 == 0) ? 4 : (S % 2 == 0) ? 2 : 1;
+  const uint64_t al = S | base_addr;
+  int ub = (al % 16 == 0) ? 16 : (al % 8 == 0) ? 8 : (al % 4 == 0) ? 4 : (al % 2 == 0) ? 2 : 1;
   uint64_t nU = S / ub;
   uint64_t nsub = 1, per = nU;
   int upl = 1;
@@ -238,6 +240,249 @@ hipError_t launch_100_pttwac(unsigned char *base, uint32_t *flags, uint64_t inst
     case 16: return launch_100_ub<16>(p, pl.upl, uint32_t(blocks), st);
   }
   return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------------------
+// 100 / 0100 by cycle leaders (no visited bits, no claims).
+//
+// On an R x C grid with m = R*C - 1, slot s < m moves to s*R mod m (slot m stays). If
+// R^k = 1 (mod m), every cycle length divides k. The slots with period dividing d are the
+// s in [0, m) with m | s (R^d - 1): gcd(m, R^d - 1) of them, so the number of slots of exact
+// period d follows by Moebius inversion over the divisors of d, and the number of cycles of
+// length d is that count / d. A slot is the leader of its cycle when the walk from it (at
+// most k steps) meets no smaller slot before returning to it. Not a reference algorithm: the
+// reference claims every slot with an atomic bit (src/cl/cl_aos_asta.cl:475-566).
+// ---------------------------------------------------------------------------------------
+namespace {
+uint64_t mulmod64(uint64_t a, uint64_t b, uint64_t m) {
+  return uint64_t((unsigned __int128)a * b % m);
+}
+uint64_t gcd64(uint64_t a, uint64_t b) {
+  while (b) {
+    const uint64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+int mobius(uint32_t d) {
+  int r = 1;
+  for (uint32_t q = 2; q * q <= d; q++) {
+    if (d % q) continue;
+    d /= q;
+    if (d % q == 0) return 0;
+    r = -r;
+  }
+  return d > 1 ? -r : r;
+}
+}  // namespace
+
+bool plan_100_leaders(uint64_t R, uint64_t C, uint32_t max_order, PlPlan *out) {
+  if (R < 2 || C < 2 || R * C >= (1ull << 32) || max_order < 1) return false;
+  const uint64_t m = R * C - 1;
+  const uint64_t r = R % m;
+  uint64_t pw = r;
+  uint32_t k = 0;
+  for (uint32_t j = 1; j <= max_order; j++) {
+    if (pw == 1) {
+      k = j;
+      break;
+    }
+    pw = mulmod64(pw, r, m);
+  }
+  if (!k) return false;
+  // F[d]: slots in [1, m) whose period divides d (d | k)
+  std::vector<int64_t> F(k + 1, 0);
+  uint64_t rd = 1;
+  for (uint32_t d = 1; d <= k; d++) {
+    rd = mulmod64(rd, r, m);  // R^d mod m
+    if (k % d) continue;
+    F[d] = int64_t(gcd64(m, (rd + m - 1) % m)) - 1;
+  }
+  uint64_t cycles = 0;
+  for (uint32_t d = 2; d <= k; d++) {
+    if (k % d) continue;
+    int64_t e = 0;
+    for (uint32_t f = 1; f <= d; f++)
+      if (d % f == 0) e += mobius(d / f) * F[f];
+    cycles += uint64_t(e) / d;
+  }
+  out->cycles = cycles;
+  out->order = k;
+  return true;
+}
+
+struct PlParams {
+  unsigned char *base;
+  uint32_t *leaders;   // [0, cycles): the list; [cycles]: its counter
+  uint64_t R, C, RC, S, nsub, sub_bytes, cycles, walkers;
+  uint32_t m, order, vw_log2;
+  FastDiv64 div_nsub, div_cycles;
+  FastDiv32 div_C32;
+};
+
+__device__ __forceinline__ uint32_t pl_next(const PlParams &p, uint32_t s) {
+  const uint32_t q = p.div_C32.div(s);
+  return (s - q * uint32_t(p.C)) * uint32_t(p.R) + q;
+}
+
+// Appends every cycle leader of slots [1, m) to the list (wave-aggregated counter updates).
+__global__ __launch_bounds__(256) void k_find_leaders(PlParams p) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t stride = uint64_t(gridDim.x) * 256;
+  for (uint64_t b = uint64_t(blockIdx.x) * 256 + (threadIdx.x & ~63u); b < p.m; b += stride) {
+    const uint64_t s64 = b + lane;
+    const uint32_t s = uint32_t(s64);
+    bool lead = false;
+    if (s64 >= 1 && s64 < p.m) {
+      uint32_t t = pl_next(p, s);
+      lead = t != s;
+      for (uint32_t j = 1; lead && t != s; j++) {
+        if (t < s || j > p.order) {
+          lead = false;
+          break;
+        }
+        t = pl_next(p, t);
+      }
+    }
+    const uint64_t mask = __ballot(lead);
+    if (mask) {
+      const uint32_t first = uint32_t(__builtin_ctzll(mask));
+      uint32_t base = 0;
+      if (lane == first)
+        base = __hip_atomic_fetch_add(&p.leaders[p.cycles], uint32_t(__builtin_popcountll(mask)),
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      base = __shfl(base, int(first));
+      if (lead) {
+        const uint64_t idx = base + uint64_t(__builtin_popcountll(mask & ((1ull << lane) - 1)));
+        if (idx < p.cycles) p.leaders[idx] = s;
+      }
+    }
+  }
+}
+
+// One walker (VW lanes) per (instance, leader, piece): carries the piece around the cycle.
+// The walk reads each slot before it writes it; cycles are disjoint, so no other walker
+// touches these bytes. A cycle of L slots takes L - 1 steps of the loop.
+template <int UB, int UPL>
+__global__ __launch_bounds__(256) void k_100_leaders(PlParams p) {
+  using V = typename Elem<UB>::T;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t vwl = p.vw_log2, vw = 1u << vwl, sub = lane & (vw - 1u);
+  const uint64_t gpw = 64u >> vwl;  // walkers per wave
+  const uint64_t wave = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const uint64_t nwaves = uint64_t(gridDim.x) * 4;
+  for (uint64_t wb = wave * gpw; wb < p.walkers; wb += nwaves * gpw) {
+    const uint64_t w = wb + (lane >> vwl);
+    if (w >= p.walkers) continue;
+    const uint64_t rest = p.div_nsub.div(w), c = w - rest * p.nsub;
+    const uint64_t x = p.div_cycles.div(rest), l = rest - x * p.cycles;
+    const uint32_t s0 = p.leaders[l];
+    unsigned char *ib = p.base + x * p.RC * p.S + c * p.sub_bytes;
+    const uint32_t pbytes = uint32_t(std::min<uint64_t>(p.sub_bytes, p.S - c * p.sub_bytes));
+    // `a` goes to slot t, whose own content `nx` is already loaded; the slot after t is
+    // loaded before `a` is stored, so two loads of the chain are in flight.
+    V a[UPL], nx[UPL], nn[UPL];
+    load_piece<UB, UPL>(a, ib + uint64_t(s0) * p.S, sub, vw, pbytes);
+    uint32_t t = pl_next(p, s0);  // != s0: leaders head cycles of length >= 2
+    load_piece<UB, UPL>(nx, ib + uint64_t(t) * p.S, sub, vw, pbytes);
+    for (uint32_t j = 0; j < p.order; j++) {  // a cycle has at most `order` slots
+      const uint32_t tn = pl_next(p, t);
+      const bool last = tn == s0;
+      if (!last) load_piece<UB, UPL>(nn, ib + uint64_t(tn) * p.S, sub, vw, pbytes);
+      store_piece<UB, UPL>(ib + uint64_t(t) * p.S, a, sub, vw, pbytes);
+      if (last) {
+        store_piece<UB, UPL>(ib + uint64_t(s0) * p.S, nx, sub, vw, pbytes);
+        break;
+      }
+#pragma unroll
+      for (int k = 0; k < UPL; k++) {
+        a[k] = nx[k];
+        nx[k] = nn[k];
+      }
+      t = tn;
+    }
+  }
+}
+
+namespace {
+template <int UB>
+hipError_t launch_pl_ub(const PlParams &p, int upl, uint32_t blocks, hipStream_t st) {
+  switch (upl) {
+    case 1: hipLaunchKernelGGL((k_100_leaders<UB, 1>), dim3(blocks), dim3(256), 0, st, p); break;
+    case 2: hipLaunchKernelGGL((k_100_leaders<UB, 2>), dim3(blocks), dim3(256), 0, st, p); break;
+    case 4: hipLaunchKernelGGL((k_100_leaders<UB, 4>), dim3(blocks), dim3(256), 0, st, p); break;
+    case 8: hipLaunchKernelGGL((k_100_leaders<UB, 8>), dim3(blocks), dim3(256), 0, st, p); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+}  // namespace
+
+namespace {
+PlParams make_pl(unsigned char *base, uint32_t *list, uint64_t inst, uint64_t R, uint64_t C,
+                 uint64_t S, const Pt100Plan &pl, const PlPlan &pc) {
+  PlParams p{};
+  p.base = base;
+  p.leaders = list;
+  p.R = R;
+  p.C = C;
+  p.RC = R * C;
+  p.S = S;
+  p.nsub = pl.nsub;
+  p.sub_bytes = pl.sub_bytes;
+  p.cycles = pc.cycles;
+  p.walkers = inst * pc.cycles * pl.nsub;
+  p.m = uint32_t(R * C - 1);
+  p.order = pc.order;
+  p.vw_log2 = pl.vw_log2;
+  p.div_nsub = FastDiv64(pl.nsub);
+  p.div_cycles = FastDiv64(pc.cycles);
+  p.div_C32 = FastDiv32(uint32_t(C));
+  return p;
+}
+}  // namespace
+
+hipError_t launch_find_leaders(uint32_t *list, uint64_t R, uint64_t C, const PlPlan &pc,
+                               hipStream_t st) {
+  if (pc.cycles == 0) return hipSuccess;
+  if (R * C >= (1ull << 32) || pc.cycles >= (1ull << 32)) return hipErrorInvalidValue;
+  Pt100Plan none{};
+  none.nsub = 1;
+  const PlParams p = make_pl(nullptr, list, 1, R, C, 1, none, pc);
+  hipError_t e = launch_fill_u32(list + pc.cycles, 1, 0, st);
+  if (e != hipSuccess) return e;
+  const uint32_t fblocks = uint32_t(std::min<uint64_t>((uint64_t(p.m) + 255) / 256,
+                                                       uint64_t(device_info().cus) * 16));
+  hipLaunchKernelGGL(k_find_leaders, dim3(fblocks), dim3(256), 0, st, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_leader_walk(unsigned char *base, const uint32_t *list, uint64_t inst,
+                              uint64_t R, uint64_t C, uint64_t S, const Pt100Plan &pl,
+                              const PlPlan &pc, uint32_t max_blocks, hipStream_t st) {
+  if (inst == 0 || pc.cycles == 0) return hipSuccess;
+  if (R * C >= (1ull << 32) || pc.cycles >= (1ull << 32)) return hipErrorInvalidValue;
+  const PlParams p = make_pl(base, const_cast<uint32_t *>(list), inst, R, C, S, pl, pc);
+  const uint64_t per_block = 4ull * (64u >> pl.vw_log2);
+  const uint32_t blocks =
+      uint32_t(std::min<uint64_t>((p.walkers + per_block - 1) / per_block, max_blocks));
+  switch (pl.ub) {
+    case 1: return launch_pl_ub<1>(p, pl.upl, blocks, st);
+    case 2: return launch_pl_ub<2>(p, pl.upl, blocks, st);
+    case 4: return launch_pl_ub<4>(p, pl.upl, blocks, st);
+    case 8: return launch_pl_ub<8>(p, pl.upl, blocks, st);
+    case 16: return launch_pl_ub<16>(p, pl.upl, blocks, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_100_leaders(unsigned char *base, uint32_t *scratch, uint64_t inst,
+                              uint64_t R, uint64_t C, uint64_t S, const Pt100Plan &pl,
+                              const PlPlan &pc, uint32_t max_blocks, hipStream_t st) {
+  hipError_t e = launch_find_leaders(scratch, R, C, pc, st);
+  if (e != hipSuccess) return e;
+  return launch_leader_walk(base, scratch, inst, R, C, S, pl, pc, max_blocks, st);
 }
 
 }  // namespace lm

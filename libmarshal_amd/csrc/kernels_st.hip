@@ -936,7 +936,8 @@ bool pick_height(size_t es, uint64_t D, uint64_t L, uint64_t target_bytes, uint6
 }  // namespace
 
 bool plan_st(size_t es, uint64_t M, uint64_t N, int variant, uint64_t target_bytes,
-             uint64_t scratch_cap, int64_t pt_vw, bool rotate, StPlan *out) {
+             uint64_t scratch_cap, int64_t pt_vw, bool rotate, StPlan *out,
+             uint64_t base_addr) {
   if (M < 2 || N < 2) return false;
   const uint64_t D = variant == 0 ? M : N;   // striped dimension
   const uint64_t L = variant == 0 ? N : M;   // the other one
@@ -957,8 +958,8 @@ bool plan_st(size_t es, uint64_t M, uint64_t N, int variant, uint64_t target_byt
   p.rotate = rotate && p.q * stripe > scratch_cap;
   p.group = p.rotate ? 1 : std::max<uint64_t>(1, std::min<uint64_t>(p.q, scratch_cap / stripe));
   if (p.q > 1) {
-    if (variant == 0) plan_100(es, 1, p.q, N, h, &p.pt, pt_vw);
-    else plan_100(es, 1, M, p.q, h, &p.pt, pt_vw);
+    if (variant == 0) plan_100(es, 1, p.q, N, h, &p.pt, pt_vw, base_addr);
+    else plan_100(es, 1, M, p.q, h, &p.pt, pt_vw, base_addr);
   }
   const uint64_t pt_words = p.q > 1 ? p.pt.flag_words : 0;
   const uint64_t pad_words = p.r ? pad_flag_words(M * N, es) : 0;

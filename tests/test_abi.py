@@ -121,3 +121,20 @@ def test_tuning_keys_defaults_and_limits():
     lm.set_tuning("st_skew", 0)  # 0 allowed
     assert lm.get_tuning("st_skew") == 0
     lm.set_tuning("st_skew", 1)
+
+
+def test_cycle_count_matches_the_oracle():
+    """The host's gcd / Moebius count of the transpose map's cycles equals the oracle's walk
+    (include/plan.hpp:57-73 restated) on every small grid, and gives config 2's SoA grid."""
+    import libmarshal_amd as lm
+    import oracle
+    for R in range(2, 34):
+        for C in range(2, 34):
+            n, k = lm.cycle_count(R, C, max_order=1 << 20)
+            assert n == oracle.num_cycles(R, C), (R, C)
+            assert k >= 1
+            # every cycle length divides the order: R^k = 1 (mod RC - 1)
+            assert pow(R, k, R * C - 1) == 1 % (R * C - 1), (R, C, k)
+    assert lm.cycle_count(16, 1 << 18) == (381300, 11)   # 100(16, 2^18, 64) of config 2
+    assert lm.cycle_count(1 << 18, 16) == (381300, 11)   # and its inverse
+    assert lm.cycle_count(1800, 7200, max_order=64) == (0, 0)  # order far above 64
