@@ -40,6 +40,9 @@ def parse():
                         "(for HBM byte counters of the stripe-route kernels)")
     p.add_argument("--profile-large", action="store_true",
                    help="only run the two large prime-dimension transposes (for kernel traces)")
+    p.add_argument("--profile-soa", type=int, default=0,
+                   help="only run N SoA->ASTA->SoA round trips of the 1 GiB struct array "
+                        "(the cycle-leader form), after one untimed round trip")
     p.add_argument("--no-large", action="store_true",
                    help="skip the 103 GB and 50 GB prime-dimension transposes")
     p.add_argument("--profile", type=int, default=0,
@@ -272,6 +275,16 @@ def main():
     d = torch.rand(n * w, device="cuda", generator=g) * 2 - 1  # U[-1, 1)
     ref = d.clone()
 
+    if args.profile_soa:
+        def soa_round():
+            return lm.soa_asta(d, n, w, t, elem_bytes=4) or lm.asta_soa(d, n, w, t, elem_bytes=4)
+        if soa_round():
+            raise RuntimeError("conversion failed")
+        s = time_calls(soa_round, args.profile_soa, stream, warm=False) / 2
+        ok = torch.equal(d, ref)
+        print(json.dumps({"profile_soa": {"route": lm.last_route(), "GBs": round(2 * nbytes / s / 1e9, 1),
+                                          "round_trips": args.profile_soa, "restored": ok}}), flush=True)
+        return
     if args.profile:
         d2 = torch.empty_like(d)
         for _ in range(args.profile):
