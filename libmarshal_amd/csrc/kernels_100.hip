@@ -378,6 +378,7 @@ __global__ __launch_bounds__(256) void k_100_leaders(PlParams p) {
     const uint64_t rest = p.div_nsub.div(w), c = w - rest * p.nsub;
     const uint64_t x = p.div_cycles.div(rest), l = rest - x * p.cycles;
     const uint32_t s0 = p.leaders[l];
+    if (s0 == 0 || s0 >= p.m) continue;  // never true for a complete list; keeps accesses in bounds
     unsigned char *ib = p.base + x * p.RC * p.S + c * p.sub_bytes;
     const uint32_t pbytes = uint32_t(std::min<uint64_t>(p.sub_bytes, p.S - c * p.sub_bytes));
     // `a` goes to slot t, whose own content `nx` is already loaded; the slot after t is
