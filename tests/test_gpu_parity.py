@@ -238,7 +238,8 @@ def test_full_flow_with_padding(hip, orc):
 @pytest.mark.parametrize("es", ES)
 def test_forced_routes_match(hip, orc, es):
     """Shrinking the LDS budget forces the LDS-flag and global-flag forms of 010; every route
-    must give the same bytes (SURVEY: host-forced fallbacks)."""
+    must give the same bytes (fallbacks forced from the host, as a device that cannot run the
+    LDS form would force them, src/cl_marshal.cc:814-818)."""
     A, a, B = 40, 37, 29
     x = orc.tagged(A * a * B, es, seed=es)
     want = orc.op_010(x, A, a, B, es=es)
