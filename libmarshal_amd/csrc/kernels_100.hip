@@ -326,38 +326,51 @@ __device__ __forceinline__ uint32_t pl_next(const PlParams &p, uint32_t s) {
   return (s - q * uint32_t(p.C)) * uint32_t(p.R) + q;
 }
 
-// Appends every cycle leader of slots [1, m) to the list (wave-aggregated counter updates).
+// Appends every cycle leader of slots [1, m) to the list. A workgroup takes chunks of
+// kLeadChunk slots, gathers their leaders in LDS (one LDS counter update per wave), then
+// reserves room in the list with one global counter update per chunk.
+constexpr uint32_t kLeadChunk = 4096;
 __global__ __launch_bounds__(256) void k_find_leaders(PlParams p) {
+  __shared__ uint32_t buf[kLeadChunk];
+  __shared__ uint32_t cnt, base;
   const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t stride = uint64_t(gridDim.x) * 256;
-  for (uint64_t b = uint64_t(blockIdx.x) * 256 + (threadIdx.x & ~63u); b < p.m; b += stride) {
-    const uint64_t s64 = b + lane;
-    const uint32_t s = uint32_t(s64);
-    bool lead = false;
-    if (s64 >= 1 && s64 < p.m) {
-      uint32_t t = pl_next(p, s);
-      lead = t != s;
-      for (uint32_t j = 1; lead && t != s; j++) {
-        if (t < s || j > p.order) {
-          lead = false;
-          break;
+  for (uint64_t c0 = uint64_t(blockIdx.x) * kLeadChunk; c0 < p.m;
+       c0 += uint64_t(gridDim.x) * kLeadChunk) {  // uniform over the workgroup
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kLeadChunk; i += 256) {
+      const uint64_t s64 = c0 + i;
+      const uint32_t s = uint32_t(s64);
+      bool lead = false;
+      if (s64 >= 1 && s64 < p.m) {
+        uint32_t t = pl_next(p, s);
+        lead = t != s;
+        for (uint32_t j = 1; lead && t != s; j++) {
+          if (t < s || j > p.order) {
+            lead = false;
+            break;
+          }
+          t = pl_next(p, t);
         }
-        t = pl_next(p, t);
+      }
+      const uint64_t mask = __ballot(lead);
+      if (mask) {
+        const uint32_t first = uint32_t(__builtin_ctzll(mask));
+        uint32_t wb = 0;
+        if (lane == first) wb = atomicAdd(&cnt, uint32_t(__builtin_popcountll(mask)));
+        wb = __shfl(wb, int(first));
+        if (lead) buf[wb + uint32_t(__builtin_popcountll(mask & ((1ull << lane) - 1)))] = s;
       }
     }
-    const uint64_t mask = __ballot(lead);
-    if (mask) {
-      const uint32_t first = uint32_t(__builtin_ctzll(mask));
-      uint32_t base = 0;
-      if (lane == first)
-        base = __hip_atomic_fetch_add(&p.leaders[p.cycles], uint32_t(__builtin_popcountll(mask)),
-                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      base = __shfl(base, int(first));
-      if (lead) {
-        const uint64_t idx = base + uint64_t(__builtin_popcountll(mask & ((1ull << lane) - 1)));
-        if (idx < p.cycles) p.leaders[idx] = s;
-      }
-    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      base = cnt ? __hip_atomic_fetch_add(&p.leaders[p.cycles], cnt, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT)
+                 : 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256)
+      if (uint64_t(base) + i < p.cycles) p.leaders[base + i] = buf[i];
+    __syncthreads();
   }
 }
 
@@ -453,8 +466,8 @@ hipError_t launch_find_leaders(uint32_t *list, uint64_t R, uint64_t C, const PlP
   const PlParams p = make_pl(nullptr, list, 1, R, C, 1, none, pc);
   hipError_t e = launch_fill_u32(list + pc.cycles, 1, 0, st);
   if (e != hipSuccess) return e;
-  const uint32_t fblocks = uint32_t(std::min<uint64_t>((uint64_t(p.m) + 255) / 256,
-                                                       uint64_t(device_info().cus) * 16));
+  const uint64_t chunks = (uint64_t(p.m) + kLeadChunk - 1) / kLeadChunk;
+  const uint32_t fblocks = uint32_t(std::min<uint64_t>(chunks, uint64_t(device_info().cus) * 8));
   hipLaunchKernelGGL(k_find_leaders, dim3(fblocks), dim3(256), 0, st, p);
   return hipGetLastError();
 }
