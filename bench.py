@@ -46,8 +46,9 @@ def parse():
     p.add_argument("--no-large", action="store_true",
                    help="skip the 103 GB and 50 GB prime-dimension transposes")
     p.add_argument("--profile", type=int, default=0,
-                   help="only run N LDS-staged AoS->ASTA launches and N plain 1 GiB device "
-                        "copies (the calibration kernel for HBM byte counters), then exit")
+                   help="only run N AoS->ASTA->AoS round trips (2N LDS-staged launches, as "
+                        "in the timed step) and N plain 1 GiB device copies (the calibration "
+                        "kernel for HBM byte counters), then exit")
     return p.parse_args()
 
 
@@ -288,11 +289,13 @@ def main():
     if args.profile:
         d2 = torch.empty_like(d)
         for _ in range(args.profile):
-            if lm.aos_asta(d, n, w, t, method=1, elem_bytes=4):
+            if lm.aos_asta(d, n, w, t, method=1, elem_bytes=4) or \
+                    lm.asta_aos(d, n, w, t, method=1, elem_bytes=4):
                 raise RuntimeError("conversion failed")
             d2.copy_(d)
         torch.cuda.synchronize()
-        print(f"profile run: {args.profile} x k_010_bs and {args.profile} x 1 GiB copy", flush=True)
+        print(f"profile run: {2 * args.profile} x k_010_bs and {args.profile} x 1 GiB copy",
+              flush=True)
         return
 
     def step():

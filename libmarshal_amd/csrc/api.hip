@@ -73,7 +73,7 @@ struct Knob {
 };
 Knob g_knobs[] = {
     {"bs_upt", {16}},
-    {"bs_blocks_per_cu", {12}},
+    {"bs_blocks_per_cu", {192}},
     {"pt010_blocks_per_cu", {8}},
     {"pt100_vw_max", {8}},
     {"pt100_blocks_per_cu", {32}},
@@ -860,7 +860,15 @@ std::vector<uint64_t> some_divisors(uint64_t n, size_t cap) {
 double stage_bw_tbs(const StageRec &r) {
   size_t es = r.es;
   switch (r.kind) {
-    case StageKind::BS010: return es >= 4 ? 5.5 : es == 2 ? 4.5 : 2.8;
+    case StageKind::BS010: {
+      if (es >= 4) return 5.5;
+      const uint64_t G = uint64_t(r.bs.G);
+      if (G <= 1) return es == 2 ? 4.5 : 2.8;
+      // Dword units: sides whose extent is a multiple of the unit take carry-free offsets.
+      const int whole = int(r.d1 % G == 0) + int(r.d2 % G == 0);
+      if (es == 2) return whole == 2 ? 5.3 : whole == 1 ? 4.6 : 4.3;
+      return whole == 2 ? 4.7 : whole == 1 ? 3.4 : 2.7;
+    }
     case StageKind::PT010: return 1.5;
     case StageKind::PT0100L: return std::min(3.0, 0.01 * double(r.d3 * es));
     case StageKind::PT100G: {

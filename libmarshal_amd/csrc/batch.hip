@@ -157,27 +157,7 @@ void add_item(Group &g, const P &p, uint64_t units) {
 }
 
 Bs010Params make_bs(const StageRec &r) {
-  constexpr int NT = 256;
-  Bs010Params p{};
-  p.base = r.buf;
-  p.A = r.d0;
-  p.a = uint32_t(r.d1);
-  p.B = uint32_t(r.d2);
-  p.Bp = r.bs.Bp;
-  p.n = uint32_t(r.d1 * r.d2);
-  p.g = uint32_t(r.bs.g);
-  p.nchunks = r.bs.nchunks;
-  p.div_n = FastDiv32(p.n);
-  p.div_B = FastDiv32(p.B);
-  p.div_a = FastDiv32(p.a);
-  uint32_t D = uint32_t(NT * r.bs.G - (r.bs.G - 1));
-  p.din_t = D / p.n;
-  p.din_i = (D % p.n) / p.B;
-  p.din_j = (D % p.n) % p.B;
-  p.dout_t = D / p.n;
-  p.dout_j = (D % p.n) / p.a;
-  p.dout_i = (D % p.n) % p.a;
-  return p;
+  return make_bs010_params(r.buf, r.d0, r.d1, r.d2, r.bs);
 }
 
 Pt010Params make_pt010(const StageRec &r) {

@@ -17,6 +17,7 @@ struct Bs010Params {
   FastDiv32 div_n, div_B, div_a;
   uint32_t din_t, din_i, din_j;     // per-iteration advance in (tile, row, col) radix
   uint32_t dout_t, dout_j, dout_i;  // per-iteration advance in (tile, col, row) radix
+  uint32_t aligned;                 // G > 1 and bit 0: B % G == 0, bit 1: a % G == 0
 };
 
 struct Bs010Plan {
@@ -32,6 +33,9 @@ uint32_t bs010_pitch(uint32_t B);
 // Chooses the chunking of an LDS-staged 010; false when one tile does not fit `lds_budget`.
 bool plan_010_bs(size_t es, uint64_t A, uint64_t a, uint64_t B, size_t lds_budget,
                  uint64_t base_addr, int upt, Bs010Plan *out);
+// Kernel parameters of one LDS-staged 010 over `base` with plan `pl`.
+Bs010Params make_bs010_params(unsigned char *base, uint64_t A, uint64_t a, uint64_t B,
+                              const Bs010Plan &pl);
 hipError_t launch_010_bs(unsigned char *base, size_t es, uint64_t A, uint64_t a, uint64_t B,
                          const Bs010Plan &pl, uint32_t max_blocks, hipStream_t st);
 

@@ -156,8 +156,8 @@ hipError_t launch_bs_g(const Bs010Params &p, const Bs010Plan &pl, uint32_t block
 }
 }  // namespace
 
-hipError_t launch_010_bs(unsigned char *base, size_t es, uint64_t A, uint64_t a, uint64_t B,
-                         const Bs010Plan &pl, uint32_t max_blocks, hipStream_t st) {
+Bs010Params make_bs010_params(unsigned char *base, uint64_t A, uint64_t a, uint64_t B,
+                              const Bs010Plan &pl) {
   constexpr int NT = 256;
   Bs010Params p{};
   p.base = base;
@@ -179,6 +179,19 @@ hipError_t launch_010_bs(unsigned char *base, size_t es, uint64_t A, uint64_t a,
   p.dout_t = D / p.n;
   p.dout_j = (D % p.n) / p.a;
   p.dout_i = (D % p.n) % p.a;
+  // When B (a) is a multiple of G, a unit never crosses an LDS row in the stage-in (a column
+  // in the gather), so its G elements take fixed LDS offsets with no carries.
+  p.aligned = 0;
+  if (pl.G > 1) {
+    if (B % uint64_t(pl.G) == 0) p.aligned |= 1u;
+    if (a % uint64_t(pl.G) == 0) p.aligned |= 2u;
+  }
+  return p;
+}
+
+hipError_t launch_010_bs(unsigned char *base, size_t es, uint64_t A, uint64_t a, uint64_t B,
+                         const Bs010Plan &pl, uint32_t max_blocks, hipStream_t st) {
+  const Bs010Params p = make_bs010_params(base, A, a, B, pl);
   uint32_t blocks = uint32_t(std::min<uint64_t>(pl.nchunks, max_blocks));
   switch (es) {
     case 1: return pl.G == 4 ? launch_bs_g<1, 4>(p, pl, blocks, st)

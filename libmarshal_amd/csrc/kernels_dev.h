@@ -134,11 +134,24 @@ __device__ __forceinline__ void bs_stage_in(const Bs010Params &p, uint64_t c,
     for (int k = 0; k < UPT; k++) {
       uint32_t u = r * RU + tid + k * NT;
       if (u < nU) {
+        if (G > 1 && (p.aligned & 1u)) {
+          // B is a multiple of G, so the unit's G elements share one LDS row. The rows sit
+          // at an odd pitch, so the unit is not dword-aligned in LDS: the empty asm keeps the
+          // compiler from fusing the element stores into one misaligned dword store.
+          E *row = lds + (t * a + i) * Bp + j;
 #pragma unroll
-        for (int e = 0; e < G; e++) {
-          lds[(t * a + i) * Bp + j] = unit_get<ES, G>(v[k], e);
-          if (e < G - 1) {
-            if (++j == B) { j = 0; if (++i == a) { i = 0; ++t; } }
+          for (int e = 0; e < G; e++) {
+            row[e] = unit_get<ES, G>(v[k], e);
+            asm volatile("" ::: "memory");
+          }
+          j += G - 1;
+        } else {
+#pragma unroll
+          for (int e = 0; e < G; e++) {
+            lds[(t * a + i) * Bp + j] = unit_get<ES, G>(v[k], e);
+            if (e < G - 1) {
+              if (++j == B) { j = 0; if (++i == a) { i = 0; ++t; } }
+            }
           }
         }
       }
@@ -177,11 +190,19 @@ __device__ __forceinline__ void bs_gather_out(const Bs010Params &p, uint64_t c,
   uint32_t i = rr - j * a;
   for (uint32_t u = tid; u < nU; u += NT) {
     U w{};
+    if (G > 1 && (p.aligned & 2u)) {
+      // a is a multiple of G, so the unit's G elements are rows i..i+G-1 of column j.
+      const E *col = lds + (t * a + i) * Bp + j;
 #pragma unroll
-    for (int e = 0; e < G; e++) {
-      unit_put<ES, G>(w, e, lds[(t * a + i) * Bp + j]);
-      if (e < G - 1) {
-        if (++i == a) { i = 0; if (++j == B) { j = 0; ++t; } }
+      for (int e = 0; e < G; e++) unit_put<ES, G>(w, e, col[e * Bp]);
+      i += G - 1;
+    } else {
+#pragma unroll
+      for (int e = 0; e < G; e++) {
+        unit_put<ES, G>(w, e, lds[(t * a + i) * Bp + j]);
+        if (e < G - 1) {
+          if (++i == a) { i = 0; if (++j == B) { j = 0; ++t; } }
+        }
       }
     }
     __builtin_nontemporal_store(w, gu + u);

@@ -80,6 +80,19 @@ def test_010_bs_bug533_shapes(hip, orc):
             check(hip.transpose_010_bs, x, want, h // t, t, w, elem_bytes=es)
 
 
+def test_010_bs_subdword_units(hip, orc):
+    # 1- and 2-byte elements move in dword units; tiles whose rows and columns are whole
+    # units take carry-free LDS offsets, the others step element by element.
+    for es in (1, 2):
+        for a, B in ((64, 16), (4, 4), (8, 12), (12, 8), (64, 6), (6, 64), (7, 16), (16, 5),
+                     (2, 2), (4, 2), (128, 36)):
+            for A in (1, 3, 1001):
+                say("bs subdword", es, A, a, B)
+                x = orc.tagged(A * a * B, es, seed=A + a * 7 + B * 131 + es)
+                check(hip.transpose_010_bs, x, orc.op_010(x, A, a, B, es=es), A, a, B,
+                      elem_bytes=es)
+
+
 def test_010_pttwac_bug536_shapes(hip, orc):
     # test/cl_unittest.cc:226-292: w in {197, 215}, t in {16, 32, 64}, R = 1..16, P = 1
     for w, h0 in ((197, 35588), (215, 44609)):
