@@ -84,6 +84,7 @@ Knob g_knobs[] = {
     {"st_scratch_mb", {2048}, 0},  // 0: no routes that need scratch beyond visited bits
     {"st_super_bytes", {2048}},
     {"st_tile_kb", {32}},
+    {"oop_elem_tile_kb", {8}, 0},  // 0: element-store tiles take st_tile_kb too
     {"st_pt_vw", {16}},
     {"st_rotate", {1}, 0},
     {"autotune", {0}, 0},
@@ -94,6 +95,7 @@ Knob g_knobs[] = {
     {"gather_row_bytes", {256}, 0},
     {"st_skew", {1}, 0},
     {"copy_nt", {1}, 0},
+    {"copy_blocks_per_cu", {64}},
     {"oop_nt", {2}, 0},
     {"pad_nt", {1}, 0},
     {"pl_min_bytes", {128}},

@@ -524,7 +524,7 @@ hipError_t launch_copy(unsigned char *dst, const unsigned char *src, uint64_t by
                        hipStream_t st) {
   if (bytes == 0) return hipSuccess;
   const uint32_t cap = uint32_t(device_info().cus) * 16;
-  const uint32_t cap16 = uint32_t(device_info().cus) * 64;  // 16-byte forms: 64 groups per CU
+  const uint32_t cap16 = uint32_t(device_info().cus) * uint32_t(tuning("copy_blocks_per_cu"));
   const uint64_t per_block = 256 * kCopyUnroll;
   if (bytes >= 64 && ((uintptr_t(dst) | uintptr_t(src) | bytes) & 15u) != 0) {
     const uint32_t head = uint32_t((16u - (uintptr_t(dst) & 15u)) & 15u);
@@ -617,21 +617,42 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
                                 uint64_t dst_stride, size_t tile_bytes, hipStream_t st,
                                 uint64_t src_end) {
   if (blocks == 0 || R == 0 || C == 0) return hipSuccess;
+  auto al = [](uint64_t v) { return (v & 15u) == 0; };
+  const bool dst_al = al(reinterpret_cast<uintptr_t>(dst)) && al(R * es) && al(dst_stride);
+  const int64_t skew_knob = tuning("st_skew");
+  const bool skew_es = es < 16 && skew_knob > 0 && (es <= 2 || skew_knob >= 2);
   // Tile: cover the shorter side whole when a tile of it stays within `tile_bytes` with at
   // least 64 bytes along the other side; otherwise a square-ish tile.
-  const uint64_t cap_el = std::max<uint64_t>(tile_bytes / es, 256);
   const uint64_t min_other = std::max<uint64_t>(1, 64 / es);
   uint32_t trl, tcl;
-  if (R <= C && (uint64_t(1) << log2_ceil(R)) * min_other <= cap_el) {
-    trl = log2_ceil(R);
-    tcl = log2_floor(cap_el >> trl);
-  } else if (C < R && (uint64_t(1) << log2_ceil(C)) * min_other <= cap_el) {
-    tcl = log2_ceil(C);
-    trl = log2_floor(cap_el >> tcl);
-  } else {
+  auto pick = [&](size_t bytes) {  // true when the tile covers the shorter side whole
+    const uint64_t cap_el = std::max<uint64_t>(bytes / es, 256);
+    if (R <= C && (uint64_t(1) << log2_ceil(R)) * min_other <= cap_el) {
+      trl = log2_ceil(R);
+      tcl = log2_floor(cap_el >> trl);
+      return true;
+    }
+    if (C < R && (uint64_t(1) << log2_ceil(C)) * min_other <= cap_el) {
+      tcl = log2_ceil(C);
+      trl = log2_floor(cap_el >> tcl);
+      return true;
+    }
     const uint32_t l = log2_floor(cap_el);
     tcl = (l + 1) / 2;
     trl = l - tcl;
+    return false;
+  };
+  // Element stores (misaligned destination rows, no skewed form for this size) into tiles
+  // that span a whole short side (the stripes of a column-stripe route) run faster with
+  // smaller tiles: on 131071 x 256 f64 stripes, 16 x 256 tiles (32 KiB) reach about
+  // 4.2 TB/s and 32 x 32 tiles (8 KiB) about 4.9. Square-ish tiles over both long sides
+  // lose from the same change, so they keep `tile_bytes`.
+  if (pick(tile_bytes) && !dst_al && !skew_es) {
+    const int64_t ekb = tuning("oop_elem_tile_kb");
+    if (ekb > 0 && (size_t(ekb) << 10) < tile_bytes) {
+      tile_bytes = size_t(ekb) << 10;
+      pick(tile_bytes);
+    }
   }
   // Never larger than the matrix side rounded up to a power of two; optionally fewer tile
   // rows (longer contiguous row segments per tile).
@@ -646,15 +667,11 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
     }
   }
   const uint32_t EPV = uint32_t(16 / es);
-  auto al = [](uint64_t v) { return (v & 15u) == 0; };
-  const bool vout = al(reinterpret_cast<uintptr_t>(dst)) && al(R * es) && al(dst_stride) &&
-                    (1u << trl) >= EPV;
+  const bool vout = dst_al && (1u << trl) >= EPV;
   // Skewed tiles for misaligned destination rows ("st_skew": 0 off, 1 for 1- and 2-byte
   // elements, 2 up to 8-byte ones); rows back to back and long enough that a chunk holds at
   // most one row boundary.
-  const int64_t skew_knob = tuning("st_skew");
-  const bool skew = !vout && es < 16 && skew_knob > 0 && (es <= 2 || skew_knob >= 2) &&
-                    (1u << trl) >= EPV && R * es >= 32 &&
+  const bool skew = !vout && skew_es && (1u << trl) >= EPV && R * es >= 32 &&
                     (blocks == 1 || dst_stride == R * C * es);
   // LDS rows: TR elements (plus 16 / es above them when skewed) padded to a multiple of 128
   // bytes; TC rows within the tile budget.

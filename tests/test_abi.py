@@ -110,7 +110,8 @@ def test_tuning_keys_defaults_and_limits():
     import libmarshal_amd as lm
     defaults = {"st_scratch_mb": 2048, "batch_solo_kb": 1024, "st_skew": 1, "copy_nt": 1,
                 "oop_nt": 2, "pad_nt": 1, "gather_tile_kb": 24, "gather_row_bytes": 256,
-                "bs_upt": 16, "autotune": 0}
+                "bs_upt": 16, "bs_blocks_per_cu": 192, "oop_elem_tile_kb": 8,
+                "copy_blocks_per_cu": 64, "autotune": 0}
     for k, v in defaults.items():
         assert lm.get_tuning(k) == v, k
     assert lm.get_tuning("no_such_key") == 0
@@ -121,6 +122,11 @@ def test_tuning_keys_defaults_and_limits():
     lm.set_tuning("st_skew", 0)  # 0 allowed
     assert lm.get_tuning("st_skew") == 0
     lm.set_tuning("st_skew", 1)
+    lm.set_tuning("oop_elem_tile_kb", 0)  # 0 allowed: element-store tiles use st_tile_kb
+    assert lm.get_tuning("oop_elem_tile_kb") == 0
+    lm.set_tuning("oop_elem_tile_kb", 8)
+    with pytest.raises(ValueError):
+        lm.set_tuning("copy_blocks_per_cu", 0)
 
 
 def test_cycle_count_matches_the_oracle():
