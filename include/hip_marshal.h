@@ -22,7 +22,7 @@
  *
  * Scratch: cycle-following stages need visited-bit flags, the stripe route a buffer of one
  * or more stripes and the through-scratch forms one of a group of blocks (each at most
- * "st_scratch_mb", 2 GiB by default: 0.7% of a 288 GB MI355X). All come from
+ * "st_scratch_mb", 8 GiB by default: 2.8% of a 288 GB MI355X). All come from
  * one per-device pool owned by the library that only grows (hipMalloc and hipFree happen on
  * growth only, never per call once the pool is large enough; hip_marshal_reserve()
  * pre-sizes it). Flags are cleared with a fill kernel on the caller's stream. A stage holds

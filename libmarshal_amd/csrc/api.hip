@@ -81,7 +81,7 @@ Knob g_knobs[] = {
     {"rc_row_blocks_per_cu", {2}},
     {"rc_max_super_bytes", {64}},
     {"rc_col_blocks_per_cu", {1}},
-    {"st_scratch_mb", {2048}, 0},  // 0: no routes that need scratch beyond visited bits
+    {"st_scratch_mb", {8192}, 0},  // 0: no routes that need scratch beyond visited bits
     {"st_super_bytes", {2048}},
     {"st_tile_kb", {32}},
     {"oop_elem_tile_kb", {8}, 0},  // 0: element-store tiles take st_tile_kb too
