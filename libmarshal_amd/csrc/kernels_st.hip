@@ -558,6 +558,106 @@ hipError_t launch_copy(unsigned char *dst, const unsigned char *src, uint64_t by
   return hipGetLastError();
 }
 
+// Contiguous copy out of a pitched buffer: destination bytes [0, total), rows of RB bytes
+// back to back, from source rows of RB bytes at pitch P (a multiple of 16; src 16-byte
+// aligned). The destination is written in aligned 16-byte chunks. A chunk inside one row is a
+// 16-byte window of that source row; a chunk that holds a row boundary (RB >= 16: at most
+// one) takes its bytes past the boundary from a second window that ends the row's padding and
+// starts the next row. The bytes before the destination's first aligned address and after
+// its last whole chunk are copied one by one by thread 0.
+__device__ __forceinline__ uint64_t unpitch_src(uint64_t d, uint64_t RB, uint64_t P,
+                                                const FastDiv64 &div_rb) {
+  const uint64_t row = div_rb.div(d);
+  return row * P + (d - row * RB);
+}
+
+template <bool NTS>
+__global__ __launch_bounds__(256) void k_unpitch(unsigned char *dst,
+                                                 const unsigned char *src, uint64_t RB,
+                                                 uint64_t P, uint64_t total, uint32_t head,
+                                                 uint64_t nb, FastDiv64 div_rb) {
+  const uint64_t stride = uint64_t(gridDim.x) * 256;
+  U128 *d16 = reinterpret_cast<U128 *>(dst + head);  // 16-byte aligned
+  for (uint64_t b = uint64_t(blockIdx.x) * 256 * kCopyUnroll + threadIdx.x; b < nb;
+       b += stride * kCopyUnroll) {
+    U128 lo[kCopyUnroll], hi[kCopyUnroll];
+    uint64_t s[kCopyUnroll];
+    uint32_t split[kCopyUnroll];  // bytes of the chunk before a row boundary (16: none)
+    // Phase 1: the window of each chunk's first row (branch-free; a lane past the end loads
+    // the source's first chunk).
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; u++) {
+      const uint64_t i = b + uint64_t(u) * 256;
+      const uint64_t dd = head + 16 * (i < nb ? i : 0);
+      const uint64_t row = div_rb.div(dd), off = dd - row * RB;
+      s[u] = row * P + off;
+      split[u] = uint32_t(off + 16 <= RB ? 16 : RB - off);
+      const uint64_t a = s[u] & ~uint64_t(15);
+      const U128 *pa = reinterpret_cast<const U128 *>(src + a);
+      lo[u] = pa[0];
+      hi[u] = (s[u] & 15u) ? pa[1] : pa[0];
+    }
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; u++) {
+      const uint64_t i = b + uint64_t(u) * 256;
+      if (i >= nb) continue;
+      U128 v = window_join(s[u], lo[u], hi[u]);
+      const uint32_t sp = split[u];
+      if (sp < 16) {  // row boundary inside the chunk: the rest comes from the next row
+        const uint64_t s2 = s[u] - (s[u] % P) + P - sp;  // so that byte sp is next row's 0
+        const U128 w = load_window16(src, s2);
+        const uint32_t vw[4] = {v.x, v.y, v.z, v.w}, ww[4] = {w.x, w.y, w.z, w.w};
+        uint32_t o[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+          const uint32_t lo_b = 4 * k;
+          if (sp >= lo_b + 4) {
+            o[k] = vw[k];
+          } else if (sp <= lo_b) {
+            o[k] = ww[k];
+          } else {
+            const uint32_t m = (1u << (8 * (sp - lo_b))) - 1u;
+            o[k] = (vw[k] & m) | (ww[k] & ~m);
+          }
+        }
+        v = U128{o[0], o[1], o[2], o[3]};
+      }
+      if (NTS)
+        __builtin_nontemporal_store(v, d16 + i);
+      else
+        d16[i] = v;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (uint64_t x = 0; x < head && x < total; x++) dst[x] = src[unpitch_src(x, RB, P, div_rb)];
+    for (uint64_t x = head + 16 * nb; x < total; x++)
+      dst[x] = src[unpitch_src(x, RB, P, div_rb)];
+  }
+}
+
+hipError_t launch_unpitch(unsigned char *dst, const unsigned char *src, uint64_t RB,
+                          uint64_t P, uint64_t rows, hipStream_t st) {
+  const uint64_t total = RB * rows;
+  if (total == 0) return hipSuccess;
+  if (RB < 16 || P < RB || (P & 15u) || (reinterpret_cast<uintptr_t>(src) & 15u))
+    return hipErrorInvalidValue;
+  const uint32_t head =
+      uint32_t(std::min<uint64_t>((16u - (reinterpret_cast<uintptr_t>(dst) & 15u)) & 15u,
+                                  total));
+  const uint64_t nb = (total - head) / 16;
+  const uint64_t per_block = 256 * kCopyUnroll;
+  const uint32_t blocks = uint32_t(std::min<uint64_t>(
+      std::max<uint64_t>((nb + per_block - 1) / per_block, 1),
+      uint64_t(device_info().cus) * uint64_t(tuning("copy_blocks_per_cu"))));
+  if (tuning("oop_nt") & 2)
+    hipLaunchKernelGGL(k_unpitch<true>, dim3(blocks), dim3(256), 0, st, dst, src, RB, P,
+                       total, head, nb, FastDiv64(RB));
+  else
+    hipLaunchKernelGGL(k_unpitch<false>, dim3(blocks), dim3(256), 0, st, dst, src, RB, P,
+                       total, head, nb, FastDiv64(RB));
+  return hipGetLastError();
+}
+
 hipError_t launch_copy2d(unsigned char *dst, uint64_t dpitch, const unsigned char *src,
                          uint64_t spitch, uint64_t width, uint64_t rows, hipStream_t st) {
   if (width == 0 || rows == 0) return hipSuccess;
@@ -615,10 +715,12 @@ hipError_t launch_oop_es(const OopParams &p, bool vin, int out, size_t lds, hipS
 hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, size_t es,
                                 uint64_t blocks, uint64_t R, uint64_t C, uint64_t src_stride,
                                 uint64_t dst_stride, size_t tile_bytes, hipStream_t st,
-                                uint64_t src_end) {
+                                uint64_t src_end, uint64_t dst_pitch) {
   if (blocks == 0 || R == 0 || C == 0) return hipSuccess;
+  if (dst_pitch == 0) dst_pitch = R * es;
+  if (dst_pitch < R * es) return hipErrorInvalidValue;
   auto al = [](uint64_t v) { return (v & 15u) == 0; };
-  const bool dst_al = al(reinterpret_cast<uintptr_t>(dst)) && al(R * es) && al(dst_stride);
+  const bool dst_al = al(reinterpret_cast<uintptr_t>(dst)) && al(dst_pitch) && al(dst_stride);
   const int64_t skew_knob = tuning("st_skew");
   const bool skew_es = es < 16 && skew_knob > 0 && (es <= 2 || skew_knob >= 2);
   // Tile: cover the shorter side whole when a tile of it stays within `tile_bytes` with at
@@ -672,7 +774,7 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
   // elements, 2 up to 8-byte ones); rows back to back and long enough that a chunk holds at
   // most one row boundary.
   const bool skew = !vout && skew_es && (1u << trl) >= EPV && R * es >= 32 &&
-                    (blocks == 1 || dst_stride == R * C * es);
+                    dst_pitch == R * es && (blocks == 1 || dst_stride == R * C * es);
   // LDS rows: TR elements (plus 16 / es above them when skewed) padded to a multiple of 128
   // bytes; TC rows within the tile budget.
   const uint32_t PB =
@@ -686,7 +788,7 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
   p.src_stride = src_stride;
   p.dst_stride = dst_stride;
   p.src_pitch = C * es;
-  p.dst_pitch = R * es;
+  p.dst_pitch = dst_pitch;
   p.src_end = p.src_off + (src_end ? src_end : (blocks - 1) * src_stride + R * C * es);
   p.R = R;
   p.C = C;

@@ -451,3 +451,39 @@ def test_through_scratch_route(hip, orc, es):
         assert hip.transpose(d, 1, M, 1, N, 1, 9, elem_bytes=es) is False
         assert hip.last_route() == "OOP-"
         assert np.array_equal(host(d), orc.op_transpose(x, M, N, es=es))
+
+
+@pytest.mark.parametrize("es", ES)
+def test_through_scratch_pitched_form(hip, orc, es):
+    """Destination rows that are not 16-byte aligned take the pitched form (transpose into
+    scratch rows of a 16-byte multiple pitch, then one contiguous copy back in aligned chunks,
+    rows meeting inside chunks). Bases offset by 0..15 elements, rows just above the length
+    threshold, and a 1 MB cap; the plain form gives the same bytes, and the bytes around the
+    matrix stay put."""
+    cases = ((257, 301), (1031, 997), (300, 200), (97, 4099), (4099, 65), (16 * 17 + 1, 129))
+    for pitched in (2, 0):
+        for cap in (0, 1):
+            for k, (M, N) in enumerate(cases):
+                if M * N * es > (1 << 20 if cap else 32 << 20) or M * es < 256:
+                    continue
+                off = (k * 5 + es) % 16  # elements before the matrix
+                n = M * N
+                x = orc.tagged(n + off + 3, es, seed=M + N * 7 + off)
+                d = dev(x)
+                sub = d[off * es:(off + n) * es]
+                say("pitched", pitched, cap, es, M, N, off)
+                keep = {kk: hip.get_tuning(kk) for kk in ("oop_pitched", "st_scratch_mb")}
+                try:
+                    hip.set_tuning("oop_pitched", pitched)
+                    if cap:
+                        hip.set_tuning("st_scratch_mb", 1)
+                    assert hip.transpose(sub, 1, M, 1, N, 1, 9, elem_bytes=es) is False
+                    assert hip.last_route() == "OOP-"
+                finally:
+                    for kk, v in keep.items():
+                        hip.set_tuning(kk, v)
+                got = host(d)
+                lo, hi = off * es, (off + n) * es
+                assert np.array_equal(got[:lo], x[:lo]) and np.array_equal(got[hi:], x[hi:])
+                assert np.array_equal(got[lo:hi], orc.op_transpose(x[lo:hi], M, N, es=es)), \
+                    (pitched, cap, es, M, N, off)
