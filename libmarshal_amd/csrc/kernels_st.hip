@@ -160,7 +160,65 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
 
   // Load: source row r0 + r - KOFS, column c -> LDS row c, element r.
   const uint32_t TRL = TR + KOFS;  // LDS elements per row that are loaded
-  if (VIN) {  // one 16-byte window per lane (TC >= EPV)
+  if (ES == 1 && VIN && (TRL & 3u) == 0) {
+    // 1-byte elements, four rows at a time: a lane loads the 16-byte windows of four
+    // consecutive rows over the same 16 columns, transposes each 4 x 4 byte block in
+    // registers and stores one dword per column (its four rows) to LDS: 16 dword stores per
+    // 64 bytes instead of 64 byte stores. Rows outside the tile load a clamped row instead;
+    // their LDS bytes are never stored to the destination as elements.
+    const uint32_t vl = p.tc_log2 - 4;
+    const uint32_t vpr = 1u << vl;
+    for (uint32_t i = threadIdx.x; i < ((TRL >> 2) << vl); i += NT) {
+      const uint32_t rb = (i >> vl) << 2, v = i & (vpr - 1u);
+      if (v * 16 >= nc) continue;
+      U128 lo[4], hi[4];
+      uint32_t off[4];
+#pragma unroll
+      for (uint32_t k = 0; k < 4; k++) {
+        int64_t rg = int64_t(r0) + int64_t(rb + k) - KOFS;
+        rg = rg < 0 ? 0 : rg >= int64_t(p.R) ? int64_t(p.R) - 1 : rg;
+        const uint64_t sb = sblk + uint64_t(rg) * p.src_pitch + c0 + v * 16;
+        const uint64_t a = sb & ~uint64_t(15);
+        // the upper chunk only when the window needs it and it starts before src_end (bytes
+        // at or past src_end belong to no element of the tile)
+        const uint64_t a2 = ((sb & 15u) && a + 16 < p.src_end) ? a + 16 : a;
+        if (p.nt & 1u) {
+          lo[k] = __builtin_nontemporal_load(reinterpret_cast<const U128 *>(p.src + a));
+          hi[k] = __builtin_nontemporal_load(reinterpret_cast<const U128 *>(p.src + a2));
+        } else {
+          lo[k] = *reinterpret_cast<const U128 *>(p.src + a);
+          hi[k] = *reinterpret_cast<const U128 *>(p.src + a2);
+        }
+        off[k] = uint32_t(sb & 15u);
+      }
+      uint32_t w[4][4];  // w[k][g]: bytes of columns 4g..4g+3 of row rb + k
+#pragma unroll
+      for (uint32_t k = 0; k < 4; k++) {
+        const U128 x = window_join(off[k], lo[k], hi[k]);
+        w[k][0] = x.x;
+        w[k][1] = x.y;
+        w[k][2] = x.z;
+        w[k][3] = x.w;
+      }
+#pragma unroll
+      for (uint32_t gq = 0; gq < 4; gq++) {
+        // 4 x 4 byte transpose: t[j] holds column 4 gq + j of rows rb..rb+3, row rb lowest
+        const uint32_t l01 = __builtin_amdgcn_perm(w[1][gq], w[0][gq], 0x05010400u);
+        const uint32_t h01 = __builtin_amdgcn_perm(w[1][gq], w[0][gq], 0x07030602u);
+        const uint32_t l23 = __builtin_amdgcn_perm(w[3][gq], w[2][gq], 0x05010400u);
+        const uint32_t h23 = __builtin_amdgcn_perm(w[3][gq], w[2][gq], 0x07030602u);
+        const uint32_t t[4] = {__builtin_amdgcn_perm(l23, l01, 0x05040100u),
+                               __builtin_amdgcn_perm(l23, l01, 0x07060302u),
+                               __builtin_amdgcn_perm(h23, h01, 0x05040100u),
+                               __builtin_amdgcn_perm(h23, h01, 0x07060302u)};
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) {
+          const uint32_t c = v * 16 + gq * 4 + j;  // c < TC; columns >= nc are never stored
+          *reinterpret_cast<uint32_t *>(lds + c * PB + (rb ^ oop_mask<ES, SW>(c))) = t[j];
+        }
+      }
+    }
+  } else if (VIN) {  // one 16-byte window per lane (TC >= EPV)
     const uint32_t vl = p.tc_log2 - EPV_LOG2;
     const uint32_t vpr = 1u << vl;
     for (uint32_t i = threadIdx.x; i < (TRL << vl); i += NT) {
