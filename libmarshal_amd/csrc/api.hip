@@ -86,6 +86,7 @@ Knob g_knobs[] = {
     {"st_tile_kb", {32}},
     {"oop_elem_tile_kb", {8}, 0},  // 0: element-store tiles take st_tile_kb too
     {"oop_pitched", {1}, 0},
+    {"oop_min_rows", {256}, 0},
     {"st_pt_vw", {16}},
     {"st_rotate", {1}, 0},
     {"autotune", {0}, 0},

@@ -454,6 +454,26 @@ def test_through_scratch_route(hip, orc, es):
 
 
 @pytest.mark.parametrize("es", ES)
+def test_through_scratch_tile_rows(hip, orc, es):
+    """Tile shapes of the out-of-place transpose: no minimum, the default, and 1024 rows
+    (few columns) give the same bytes on aligned and misaligned rows."""
+    for min_rows in (0, 1024):
+        for (M, N) in ((1031, 997), (512, 3000), (4099, 64), (300, 2048)):
+            if M * N * es > 32 << 20:
+                continue
+            say("tile rows", min_rows, es, M, N)
+            x = orc.tagged(M * N, es, seed=M + N + min_rows)
+            d = dev(x)
+            keep = hip.get_tuning("oop_min_rows")
+            try:
+                hip.set_tuning("oop_min_rows", min_rows)
+                assert hip.transpose(d, 1, M, 1, N, 1, 9, elem_bytes=es) is False
+            finally:
+                hip.set_tuning("oop_min_rows", keep)
+            assert np.array_equal(host(d), orc.op_transpose(x, M, N, es=es)), (min_rows, M, N)
+
+
+@pytest.mark.parametrize("es", ES)
 def test_through_scratch_pitched_form(hip, orc, es):
     """Destination rows that are not 16-byte aligned take the pitched form (transpose into
     scratch rows of a 16-byte multiple pitch, then one contiguous copy back in aligned chunks,
