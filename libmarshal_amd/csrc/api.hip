@@ -103,7 +103,7 @@ Knob g_knobs[] = {
     {"pl_min_bytes", {128}},
     {"pl_max_order", {64}},
     {"pl_min_walkers", {16384}},
-    {"batch_solo_kb", {1024}},
+    {"batch_solo_kb", {64}},
 };
 Knob *find_knob(const char *key) {
   if (!key) return nullptr;

@@ -380,6 +380,16 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
   // 1. routes
   std::vector<std::vector<StageRec>> per(count);
   b.routes.resize(count);
+  // Size from which a matrix whose plain route is the square or a through-scratch form runs
+  // on the side streams: "batch_solo_kb", except that a batch of less than 1 GiB in all keeps
+  // its sub-MiB matrices grouped. There, with no large work to hide them behind, their
+  // per-matrix launches cost more than the faster route saves (4096-matrix mix: 64 KiB
+  // measured best; its sub-MB matrices alone ran at 206 GB/s grouped, 150 on side streams).
+  double batch_bytes = 0;
+  for (int64_t i = 0; i < count; i++)
+    batch_bytes += double(descs[i].rows) * double(descs[i].cols) * double(descs[i].elem_bytes);
+  double solo_min = double(uint64_t(tuning("batch_solo_kb")) << 10);
+  if (batch_bytes < double(1ull << 30)) solo_min = std::max(solo_min, double(1u << 20));
   for (int64_t i = 0; i < count; i++) {
     const hip_marshal_desc &d = descs[i];
     std::string route;
@@ -387,11 +397,11 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
       std::vector<StageRec> list;
       std::string r2;
       // The stripe route has no grouped form; the square and through-scratch routes beat the
-      // grouped ones from about a MB on (smaller matrices gain more from sharing launches).
+      // grouped ones from `solo_min` on (smaller matrices gain more from sharing launches).
       const double mbytes = double(d.rows) * double(d.cols) * double(d.elem_bytes);
       if (plan_desc(d, list, r2, true) &&
           (r2.find("ST_") != std::string::npos ||
-           (mbytes >= double(uint64_t(tuning("batch_solo_kb")) << 10) &&
+           (mbytes >= solo_min &&
             (r2.find("SQ-") != std::string::npos || r2.find("OOP-") != std::string::npos ||
              r2.find("CYC-") != std::string::npos)))) {
         b.solo.push_back(std::move(list));
