@@ -264,7 +264,7 @@ void hip_marshal_set_profiling(int on);
  * (0/1: measure each new full-transpose shape on its first call) and "batch_solo_kb"
  * (matrices of at least this many KiB whose route is the square or through-scratch one run
  * on the batch's side streams instead of in a grouped kernel; below 1 MiB only in batches
- * of at least 1 GiB in all).
+ * of at least 1 GiB in all), "batch_solo_lanes" (how many side streams, 1-8, default 4).
  * Replaces the reference's hard-coded work-group tables (src/cl_marshal.cc:165-231,
  * 388-465). The setter returns true on an unknown key or a value outside [min, 2^30] (min is
  * 0 for "st_scratch_mb", "st_rotate", "st_wstore", "st_skew", "copy_nt", "oop_nt", "pad_nt",

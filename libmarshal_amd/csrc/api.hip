@@ -104,6 +104,7 @@ Knob g_knobs[] = {
     {"pl_max_order", {64}},
     {"pl_min_walkers", {16384}},
     {"batch_solo_kb", {64}},
+    {"batch_solo_lanes", {4}},
 };
 Knob *find_knob(const char *key) {
   if (!key) return nullptr;

@@ -44,10 +44,10 @@ struct BatchPlan {
   std::vector<std::vector<StageRec>> solo;
   std::vector<int> solo_lane;
   std::vector<double> solo_bytes;
-  size_t solo_scratch[4] = {0, 0, 0, 0};  // per lane: the most any of its lists needs
-  int solo_lanes = 0;                     // lanes in use
+  size_t solo_scratch[8] = {};  // per lane: the most any of its lists needs
+  int solo_lanes = 0;           // lanes in use
 };
-constexpr int kSoloLanes = 4;
+constexpr int kSoloLanes = 8;  // at most; the "batch_solo_lanes" knob picks how many
 
 // Side streams, events and scratch for the solo lists of a batch.
 struct SoloExec {

@@ -428,9 +428,10 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
     for (size_t i = 0; i < order.size(); i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t x, size_t y) { return b.solo_bytes[x] > b.solo_bytes[y]; });
-    double load[kSoloLanes] = {0, 0, 0, 0};
+    double load[kSoloLanes] = {};
     b.solo_lane.assign(b.solo.size(), 0);
-    b.solo_lanes = int(std::min<size_t>(kSoloLanes, b.solo.size()));
+    const size_t lanes = size_t(std::min<int64_t>(kSoloLanes, tuning("batch_solo_lanes")));
+    b.solo_lanes = int(std::min<size_t>(lanes, b.solo.size()));
     for (size_t i : order) {
       int l = 0;
       for (int k = 1; k < b.solo_lanes; k++)

@@ -108,7 +108,7 @@ def test_planner_route_choice_host_only():
 
 def test_tuning_keys_defaults_and_limits():
     import libmarshal_amd as lm
-    defaults = {"st_scratch_mb": 8192, "batch_solo_kb": 64, "st_skew": 1, "copy_nt": 1,
+    defaults = {"st_scratch_mb": 8192, "batch_solo_kb": 64, "batch_solo_lanes": 4, "st_skew": 1, "copy_nt": 1,
                 "oop_nt": 2, "pad_nt": 1, "gather_tile_kb": 24, "gather_row_bytes": 256,
                 "bs_upt": 16, "bs_blocks_per_cu": 192, "oop_elem_tile_kb": 8,
                 "copy_blocks_per_cu": 64, "oop_pitched": 1, "oop_min_rows": 256,
