@@ -240,7 +240,8 @@ void hip_marshal_set_profiling(int on);
  * "bs_blocks_per_cu", "pt010_blocks_per_cu", "pt100_blocks_per_cu", "rc_row_blocks_per_cu",
  * "rc_col_blocks_per_cu" (grid caps), "pt100_vw_max" (lanes per cycle-following walker),
  * "lds0100_min_inst", "rc_max_super_bytes" (route thresholds), and for the stripe route
- * "st_super_bytes" (target super-element bytes), "st_scratch_mb" (scratch cap of the
+ * "st_super_bytes" (target super-element bytes), "st_max_height" (at most this many
+ * elements per super-element), "st_scratch_mb" (scratch cap of the
  * stripe and through-scratch forms; 0 switches them off), "gather_tile_kb" (LDS tile of the
  * out-of-place super-element gather),
  * "st_tile_kb" (LDS tile of the out-of-place transpose), "oop_elem_tile_kb" (its tile when

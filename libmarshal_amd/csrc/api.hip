@@ -83,6 +83,7 @@ Knob g_knobs[] = {
     {"rc_col_blocks_per_cu", {1}},
     {"st_scratch_mb", {8192}, 0},  // 0: no routes that need scratch beyond visited bits
     {"st_super_bytes", {8192}},
+    {"st_max_height", {4096}},
     {"st_tile_kb", {32}},
     {"oop_elem_tile_kb", {8}, 0},  // 0: element-store tiles take st_tile_kb too
     {"oop_pitched", {1}, 0},

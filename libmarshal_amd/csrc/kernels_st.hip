@@ -1110,13 +1110,14 @@ size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 bool pick_height(size_t es, uint64_t D, uint64_t L, uint64_t target_bytes, uint64_t max_stripe,
                  uint64_t *h_out) {
   uint64_t target = std::max<uint64_t>(1, target_bytes / es);
-  target = std::min<uint64_t>(target, 1024);
+  const uint64_t hmax = uint64_t(std::max<int64_t>(1, tuning("st_max_height")));
+  target = std::min<uint64_t>(target, hmax);
   while (target > 1 && target * L * es > max_stripe) target >>= 1;
   if (target * L * es > max_stripe) return false;
   if (D < 2 * target) return false;  // fewer than two stripes: not worth a scratch copy
   // A divisor in [target/2, 2*target] (within the scratch limit) avoids the ragged step.
   uint64_t best = 0;
-  for (uint64_t h = target / 2 == 0 ? 1 : target / 2; h <= 2 * target && h <= 1024; h++) {
+  for (uint64_t h = target / 2 == 0 ? 1 : target / 2; h <= 2 * target && h <= hmax; h++) {
     if (D % h || h * L * es > max_stripe || (h * es) % 16) continue;
     if (!best || (h > best && h <= target) || (best > target && h < best)) best = h;
   }

@@ -374,15 +374,18 @@ def test_stripe_route(hip, orc, es):
     """stages = 6 / 7: stripes of rows / of columns through bounded scratch, with and without a
     ragged remainder (prime dimensions included), several scratch fills per call."""
     shapes = [(768, 131), (131, 768), (1031, 997), (640, 4099), (4099, 640), (4096, 3001)]
-    keep = {k: hip.get_tuning(k) for k in ("st_super_bytes", "st_scratch_mb", "st_rotate")}
+    keep = {k: hip.get_tuning(k)
+            for k in ("st_super_bytes", "st_scratch_mb", "st_rotate", "st_max_height")}
     try:
         # small scratch: rotated stripes, or (rotation off) several scratch fills; large
         # scratch: one fill
-        for super_bytes, scratch_mb, rotate in ((64, 1, 1), (64, 1, 0), (2048, 256, 1)):
+        for super_bytes, scratch_mb, rotate, hmax in ((64, 1, 1, 1024), (64, 1, 0, 1024),
+                                                      (2048, 256, 1, 1024), (8192, 256, 1, 4096)):
             hip.set_tuning("st_super_bytes", super_bytes)
             hip.set_tuning("st_scratch_mb", scratch_mb)
             hip.set_tuning("st_rotate", rotate)
-            for (M, N) in shapes:
+            hip.set_tuning("st_max_height", hmax)
+            for (M, N) in shapes + ([(9001, 1031), (1031, 9001)] if hmax > 1024 else []):
                 if M * N * es > 100 << 20:
                     continue
                 x = orc.tagged(M * N, es, seed=M * 3 + N + es)
