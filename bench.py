@@ -356,6 +356,14 @@ def main():
         extra["transpose_1800x7200_i32_planner_GBs"] = round(2 * M * N * 4 / s / 1e9, 1)
         extra["planner_route"] = lm.last_route()
         del x
+        # the other sizes of the reference's paper run (test/cl_unittest.cc:386-387), float
+        sizes = {}
+        for (M, N) in ((2500, 5100), (3200, 4000), (3900, 3300), (5100, 2500), (7200, 1800)):
+            x = torch.rand(M * N, device="cuda")
+            s = time_calls(lambda: lm.transpose_full(x, M, N), reps, stream)
+            sizes[f"{M}x{N}"] = {"GBs": round(2 * M * N * 4 / s / 1e9, 1), "route": lm.last_route()}
+            del x
+        extra["transpose_paper_sizes_f32_planner"] = sizes
         torch.cuda.empty_cache()
         extra["cpu_transpose_1800x7200_i32_3stage"] = cpu_config1()
         if not args.no_large:
