@@ -241,7 +241,7 @@ def cpu_config1():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+        print(json.dumps({"error": "this benchmark runs on one GPU only; --gpus must be 1 "
                           "(a transpose larger than one GPU shards by rows with one all-to-all exchange)"}))
         sys.exit(2)
     import torch

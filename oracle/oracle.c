@@ -1,8 +1,8 @@
 /*
  * oracle.c - CPU restatement of libmarshal's in-place layout conversions.
  *
- * TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP library: only the test
- * suite, the smoke check and the CPU-baseline leg of the benchmark load it. Nothing in the
+ * TEST INFRASTRUCTURE ONLY. This file is the oracle the HIP library is tested against: only
+ * the test suite, the smoke check and the CPU-baseline leg of the benchmark load it. Nothing in the
  * library links, calls or falls back to it.
  *
  * Elements are opaque bit patterns of `es` bytes (1, 2, 4, 8 or 16). All indices are 64-bit.

@@ -133,7 +133,7 @@ def test_tuning_keys_defaults_and_limits():
 
 def test_cycle_count_matches_the_oracle():
     """The host's gcd / Moebius count of the transpose map's cycles equals the oracle's walk
-    (include/plan.hpp:57-73 restated) on every small grid, and gives config 2's SoA grid."""
+    (include/plan.hpp:57-73 restated) on every small grid, and gives the 16 x 2^18 grid of the 2^24 x 16 SoA conversion."""
     import libmarshal_amd as lm
     import oracle
     for R in range(2, 34):
@@ -143,6 +143,6 @@ def test_cycle_count_matches_the_oracle():
             assert k >= 1
             # every cycle length divides the order: R^k = 1 (mod RC - 1)
             assert pow(R, k, R * C - 1) == 1 % (R * C - 1), (R, C, k)
-    assert lm.cycle_count(16, 1 << 18) == (381300, 11)   # 100(16, 2^18, 64) of config 2
+    assert lm.cycle_count(16, 1 << 18) == (381300, 11)   # 100(16, 2^18, 64): 2^24 structs
     assert lm.cycle_count(1 << 18, 16) == (381300, 11)   # and its inverse
     assert lm.cycle_count(1800, 7200, max_order=64) == (0, 0)  # order far above 64

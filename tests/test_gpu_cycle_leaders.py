@@ -27,7 +27,7 @@ def host(t):
 
 
 CASES = [  # (es, A, B, b): 100(A, B, b) = SoA [A][B][b] -> ASTA [B][A][b]
-    (4, 16, 1 << 14, 64),    # config 2's SoA shape at 1/16 size: order 9, 256-byte slots
+    (4, 16, 1 << 14, 64),    # 2^24 x 16 f32 SoA shape at 1/16 size: order 9, 256-byte slots
     (4, 1 << 14, 16, 64),    # its inverse
     (1, 64, 4096, 160),      # order 3, 160-byte slots of bytes
     (2, 64, 4096, 100),      # 200-byte slots, order 3
@@ -53,7 +53,7 @@ def test_100_cycle_leaders(hip, orc, case):
 
 
 def test_soa_asta_takes_cycle_leaders(hip, orc):
-    """SoA -> ASTA of 16 fields, ASTA width 64, f32 (config 2's conversion at 1/16 size)."""
+    """SoA -> ASTA of 16 fields, ASTA width 64, f32 (the 2^24-struct conversion at 1/16 size)."""
     h, w, t = 64 << 14, 16, 64
     x = orc.tagged(h * w, 4, seed=77)
     d = dev(x)
@@ -82,7 +82,7 @@ def test_0100_instances_and_misaligned_base(hip, orc, es):
 
 def test_long_cycles_keep_the_other_forms(hip, orc):
     """A grid whose order is far above the limit is not given to the leader form."""
-    es, A, B, b = 4, 1800, 100, 72  # config 1's first 100
+    es, A, B, b = 4, 1800, 100, 72  # the first 100 of the 1800 x 7200 route
     assert hip.cycle_count(A, B) == (0, 0)
     x = orc.tagged(A * B * b, es, seed=5)
     d = dev(x)
