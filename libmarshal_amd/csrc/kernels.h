@@ -17,7 +17,7 @@ struct Bs010Params {
   FastDiv32 div_n, div_B, div_a;
   uint32_t din_t, din_i, din_j;     // per-iteration advance in (tile, row, col) radix
   uint32_t dout_t, dout_j, dout_i;  // per-iteration advance in (tile, col, row) radix
-  uint32_t aligned;                 // G > 1 and bit 0: B % G == 0, bit 1: a % G == 0
+  uint32_t aligned;  // G > 1 and bit 0: B % G == 0, bit 1: a % G == 0, bit 2: Bp == B
 };
 
 struct Bs010Plan {

@@ -128,6 +128,13 @@ bool plan_010_bs(size_t es, uint64_t A, uint64_t a, uint64_t B, size_t lds_budge
     if ((base_addr % 4) == 0 && g0 * tile_lds <= lds_budget) G = int(4 / es);
     else g0 = 1;
   }
+  // Dword units whose rows are not whole units: unpadded rows (the chunk lies in LDS as it
+  // lies in HBM), so the stage-in stores each unit as one dword; the gather's lanes, G rows
+  // apart, then step B dwords, which for odd B hits distinct banks.
+  if (G > 1 && B % uint64_t(G) != 0) {
+    Bp = B;
+    tile_lds = a * Bp * es;
+  }
   // One round of 256 threads x upt units per chunk when tiles are small.
   uint64_t target = uint64_t(256) * upt * G;
   uint64_t g = std::max<uint64_t>(1, target / n);
@@ -185,6 +192,7 @@ Bs010Params make_bs010_params(unsigned char *base, uint64_t A, uint64_t a, uint6
   if (pl.G > 1) {
     if (B % uint64_t(pl.G) == 0) p.aligned |= 1u;
     if (a % uint64_t(pl.G) == 0) p.aligned |= 2u;
+    if (uint64_t(pl.Bp) == B) p.aligned |= 4u;  // unpadded rows: LDS holds the chunk verbatim
   }
   return p;
 }

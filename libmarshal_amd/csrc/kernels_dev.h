@@ -134,7 +134,11 @@ __device__ __forceinline__ void bs_stage_in(const Bs010Params &p, uint64_t c,
     for (int k = 0; k < UPT; k++) {
       uint32_t u = r * RU + tid + k * NT;
       if (u < nU) {
-        if (G > 1 && (p.aligned & 1u)) {
+        if (G > 1 && (p.aligned & 4u)) {
+          // Unpadded rows: unit u is elements u*G .. u*G+G-1 of the chunk in LDS too, so
+          // (t, i, j), which this form does not use, need not be kept in step.
+          reinterpret_cast<typename UnitOf<ES, G>::T *>(lds)[u] = v[k];
+        } else if (G > 1 && (p.aligned & 1u)) {
           // B is a multiple of G, so the unit's G elements share one LDS row. The rows sit
           // at an odd pitch, so the unit is not dword-aligned in LDS: the empty asm keeps the
           // compiler from fusing the element stores into one misaligned dword store.
