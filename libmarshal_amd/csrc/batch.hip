@@ -36,7 +36,7 @@ __device__ __forceinline__ uint32_t find_item(const uint64_t *prefix, uint32_t n
 
 // ---- group kernels -------------------------------------------------------------------------
 
-template <int ES, int G, int UPT>
+template <int ES, int G, int UPT, int LAYOUT>
 __global__ __launch_bounds__(256) void kg_010_bs(const Bs010Params *items,
                                                  const uint64_t *prefix, uint32_t n,
                                                  uint64_t total) {
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(256) void kg_010_bs(const Bs010Params *items,
   U v[UPT];
   bs_load_round<ES, G, 256, UPT>(p, c, 0, v);
   while (true) {
-    bs_stage_in<ES, G, 256, UPT>(p, c, v, lds);
+    bs_stage_in<ES, G, 256, UPT, LAYOUT>(p, c, v, lds);
     __syncthreads();
     // Prefetch the next claimed chunk (a disjoint region, possibly of another matrix).
     const uint64_t qn = q + gridDim.x;
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void kg_010_bs(const Bs010Params *items,
       cn = qn - prefix[in];
       bs_load_round<ES, G, 256, UPT>(pn, cn, 0, v);
     }
-    bs_gather_out<ES, G, 256>(p, c, lds);
+    bs_gather_out<ES, G, 256, LAYOUT>(p, c, lds);
     __syncthreads();
     if (qn >= total) break;
     q = qn;
@@ -192,22 +192,33 @@ Pt100Params make_pt100(const StageRec &r, uint32_t *flags) {
   return p;
 }
 
-template <int ES, int G>
-const void *bs_func(int upt) {
+template <int ES, int G, int LAYOUT>
+const void *bs_func_l(int upt) {
   switch (upt) {
-    case 4: return reinterpret_cast<const void *>(kg_010_bs<ES, G, 4>);
-    case 8: return reinterpret_cast<const void *>(kg_010_bs<ES, G, 8>);
-    default: return reinterpret_cast<const void *>(kg_010_bs<ES, G, 16>);
+    case 4: return reinterpret_cast<const void *>(kg_010_bs<ES, G, 4, LAYOUT>);
+    case 8: return reinterpret_cast<const void *>(kg_010_bs<ES, G, 8, LAYOUT>);
+    default: return reinterpret_cast<const void *>(kg_010_bs<ES, G, 16, LAYOUT>);
   }
 }
 
-const void *bs_func_es(int es, int G, int upt) {
+template <int ES, int G>
+const void *bs_func(int upt, int layout) {
+  if constexpr (G > 1) {
+    if (layout == 1) return bs_func_l<ES, G, 1>(upt);
+    if (layout == 2) return bs_func_l<ES, G, 2>(upt);
+  }
+  return bs_func_l<ES, G, 0>(upt);
+}
+
+// upt_layout: units per lane * 4 + LDS layout (the group key's last field)
+const void *bs_func_es(int es, int G, int upt_layout) {
+  const int upt = upt_layout / 4, layout = upt_layout % 4;
   switch (es) {
-    case 1: return G == 4 ? bs_func<1, 4>(upt) : bs_func<1, 1>(upt);
-    case 2: return G == 2 ? bs_func<2, 2>(upt) : bs_func<2, 1>(upt);
-    case 4: return bs_func<4, 1>(upt);
-    case 8: return bs_func<8, 1>(upt);
-    default: return bs_func<16, 1>(upt);
+    case 1: return G == 4 ? bs_func<1, 4>(upt, layout) : bs_func<1, 1>(upt, layout);
+    case 2: return G == 2 ? bs_func<2, 2>(upt, layout) : bs_func<2, 1>(upt, layout);
+    case 4: return bs_func<4, 1>(upt, layout);
+    case 8: return bs_func<8, 1>(upt, layout);
+    default: return bs_func<16, 1>(upt, layout);
   }
 }
 
@@ -460,7 +471,7 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
       const StageRec &r = per[i][lv];
       switch (r.kind) {
         case StageKind::BS010: {
-          GroupKey k{0, int(r.es), r.bs.G, r.bs.upt};
+          GroupKey k{0, int(r.es), r.bs.G, r.bs.upt * 4 + r.bs.layout};
           Group &g = groups[k];
           g.key = k;
           g.lds = std::max(g.lds, r.bs.lds_bytes);

@@ -17,7 +17,9 @@ struct Bs010Params {
   FastDiv32 div_n, div_B, div_a;
   uint32_t din_t, din_i, din_j;     // per-iteration advance in (tile, row, col) radix
   uint32_t dout_t, dout_j, dout_i;  // per-iteration advance in (tile, col, row) radix
-  uint32_t aligned;  // G > 1 and bit 0: B % G == 0, bit 1: a % G == 0, bit 2: Bp == B
+  // G > 1: bit 0 stage-in by rows (B % G == 0), bit 1 gather by columns (a % G == 0), bit 2
+  // the chunk in input order (stage-in copies units), bit 3 in output order (gather copies)
+  uint32_t aligned;
 };
 
 struct Bs010Plan {
@@ -25,6 +27,8 @@ struct Bs010Plan {
   int upt;            // units per thread per round (4, 8 or 16)
   uint64_t g;         // tiles per chunk
   uint32_t Bp;        // LDS row pitch (elements)
+  int layout = 0;     // dword units: 0 rows at pitch Bp, 1 the chunk unpadded in input order,
+                      // 2 unpadded in output order
   size_t lds_bytes;   // dynamic LDS per workgroup
   uint64_t nchunks;
 };

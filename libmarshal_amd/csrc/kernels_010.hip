@@ -28,7 +28,7 @@ namespace lm {
 // and advanced by constant deltas; the only divisions are one per lane per chunk.
 // ---------------------------------------------------------------------------------------
 
-template <int ES, int G, int NT, int UPT>
+template <int ES, int G, int NT, int UPT, int LAYOUT>
 __global__ __launch_bounds__(NT) void k_010_bs(Bs010Params p) {
   using E = typename Elem<ES>::T;
   using U = typename UnitOf<ES, G>::T;
@@ -39,12 +39,12 @@ __global__ __launch_bounds__(NT) void k_010_bs(Bs010Params p) {
   U v[UPT];
   bs_load_round<ES, G, NT, UPT>(p, c, 0, v);
   while (true) {
-    bs_stage_in<ES, G, NT, UPT>(p, c, v, lds);
+    bs_stage_in<ES, G, NT, UPT, LAYOUT>(p, c, v, lds);
     __syncthreads();
     // Prefetch the next chunk (a disjoint region) while this one is written back.
     const uint64_t cn = c + gridDim.x;
     if (cn < p.nchunks) bs_load_round<ES, G, NT, UPT>(p, cn, 0, v);
-    bs_gather_out<ES, G, NT>(p, c, lds);
+    bs_gather_out<ES, G, NT, LAYOUT>(p, c, lds);
     __syncthreads();
     if (cn >= p.nchunks) break;
     c = cn;
@@ -78,10 +78,10 @@ __global__ __launch_bounds__(NT) void k_010_pttwac(Pt010Params p) {
 // ---------------------------------------------------------------------------------------
 namespace {
 
-template <int ES, int G, int NT, int UPT>
+template <int ES, int G, int NT, int UPT, int LAYOUT>
 hipError_t launch_bs_t(const Bs010Params &p, size_t lds_bytes, uint32_t blocks,
                        hipStream_t st) {
-  auto k = k_010_bs<ES, G, NT, UPT>;
+  auto k = k_010_bs<ES, G, NT, UPT, LAYOUT>;
   if (lds_bytes > 65536) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -128,10 +128,16 @@ bool plan_010_bs(size_t es, uint64_t A, uint64_t a, uint64_t B, size_t lds_budge
     if ((base_addr % 4) == 0 && g0 * tile_lds <= lds_budget) G = int(4 / es);
     else g0 = 1;
   }
-  // Dword units whose rows are not whole units: unpadded rows (the chunk lies in LDS as it
-  // lies in HBM), so the stage-in stores each unit as one dword; the gather's lanes, G rows
-  // apart, then step B dwords, which for odd B hits distinct banks.
-  if (G > 1 && B % uint64_t(G) != 0) {
+  // Dword units on a side whose extent is not a whole number of units: the chunk sits in LDS
+  // unpadded, in that side's order, so that side copies units as dwords, while the other
+  // side, whose extent is whole, moves each unit's G elements at fixed offsets.
+  // B not whole: input order; the gather's lanes, G rows apart, step B dwords (distinct banks
+  // for odd B). a not whole (B whole): output order; the stage-in's lanes, G columns apart,
+  // step a dwords.
+  int layout = 0;
+  if (G > 1 && B % uint64_t(G) != 0) layout = 1;
+  else if (G > 1 && a % uint64_t(G) != 0) layout = 2;
+  if (layout) {
     Bp = B;
     tile_lds = a * Bp * es;
   }
@@ -146,20 +152,31 @@ bool plan_010_bs(size_t es, uint64_t A, uint64_t a, uint64_t B, size_t lds_budge
   out->g = g;
   out->upt = upt;
   out->Bp = uint32_t(Bp);
+  out->layout = layout;
   out->lds_bytes = size_t(g * tile_lds);
   out->nchunks = (A + g - 1) / g;
   return true;
 }
 
 namespace {
+template <int ES, int G, int LAYOUT>
+hipError_t launch_bs_l(const Bs010Params &p, const Bs010Plan &pl, uint32_t blocks,
+                       hipStream_t st) {
+  switch (pl.upt) {
+    case 4: return launch_bs_t<ES, G, 256, 4, LAYOUT>(p, pl.lds_bytes, blocks, st);
+    case 8: return launch_bs_t<ES, G, 256, 8, LAYOUT>(p, pl.lds_bytes, blocks, st);
+    default: return launch_bs_t<ES, G, 256, 16, LAYOUT>(p, pl.lds_bytes, blocks, st);
+  }
+}
+
 template <int ES, int G>
 hipError_t launch_bs_g(const Bs010Params &p, const Bs010Plan &pl, uint32_t blocks,
                        hipStream_t st) {
-  switch (pl.upt) {
-    case 4: return launch_bs_t<ES, G, 256, 4>(p, pl.lds_bytes, blocks, st);
-    case 8: return launch_bs_t<ES, G, 256, 8>(p, pl.lds_bytes, blocks, st);
-    default: return launch_bs_t<ES, G, 256, 16>(p, pl.lds_bytes, blocks, st);
+  if constexpr (G > 1) {
+    if (pl.layout == 1) return launch_bs_l<ES, G, 1>(p, pl, blocks, st);
+    if (pl.layout == 2) return launch_bs_l<ES, G, 2>(p, pl, blocks, st);
   }
+  return launch_bs_l<ES, G, 0>(p, pl, blocks, st);
 }
 }  // namespace
 
@@ -190,9 +207,14 @@ Bs010Params make_bs010_params(unsigned char *base, uint64_t A, uint64_t a, uint6
   // in the gather), so its G elements take fixed LDS offsets with no carries.
   p.aligned = 0;
   if (pl.G > 1) {
-    if (B % uint64_t(pl.G) == 0) p.aligned |= 1u;
-    if (a % uint64_t(pl.G) == 0) p.aligned |= 2u;
-    if (uint64_t(pl.Bp) == B) p.aligned |= 4u;  // unpadded rows: LDS holds the chunk verbatim
+    const uint64_t G = uint64_t(pl.G);
+    if (pl.layout == 2) {
+      p.aligned = 8u;
+    } else {
+      if (pl.layout == 1) p.aligned |= 4u;
+      else if (B % G == 0) p.aligned |= 1u;
+      if (a % G == 0) p.aligned |= 2u;
+    }
   }
   return p;
 }

@@ -109,7 +109,9 @@ __device__ __forceinline__ void bs_load_round(const Bs010Params &p, uint64_t c, 
   }
 }
 
-template <int ES, int G, int NT, int UPT>
+// LAYOUT (dword units only): 0 tile rows at pitch Bp, 1 the chunk in input order, 2 in
+// output order (both unpadded; see plan_010_bs).
+template <int ES, int G, int NT, int UPT, int LAYOUT>
 __device__ __forceinline__ void bs_stage_in(const Bs010Params &p, uint64_t c,
                                             typename UnitOf<ES, G>::T (&v)[UPT],
                                             typename Elem<ES>::T *lds) {
@@ -134,7 +136,14 @@ __device__ __forceinline__ void bs_stage_in(const Bs010Params &p, uint64_t c,
     for (int k = 0; k < UPT; k++) {
       uint32_t u = r * RU + tid + k * NT;
       if (u < nU) {
-        if (G > 1 && (p.aligned & 4u)) {
+        if constexpr (G > 1 && LAYOUT == 2) {
+          // Output order, unpadded: element (t, i, j) at (t*B + j)*a + i; B is a multiple of
+          // G, so the unit's G elements are columns j..j+G-1 of row i, a elements apart.
+          E *col = lds + (t * B + j) * a + i;
+#pragma unroll
+          for (int e = 0; e < G; e++) col[e * a] = unit_get<ES, G>(v[k], e);
+          j += G - 1;
+        } else if constexpr (G > 1 && LAYOUT == 1) {
           // Unpadded rows: unit u is elements u*G .. u*G+G-1 of the chunk in LDS too, so
           // (t, i, j), which this form does not use, need not be kept in step.
           reinterpret_cast<typename UnitOf<ES, G>::T *>(lds)[u] = v[k];
@@ -174,7 +183,7 @@ __device__ __forceinline__ void bs_stage_in(const Bs010Params &p, uint64_t c,
 }
 
 // Output position o = t*n + j*a + i of chunk c reads LDS (t, i, j); coalesced stores.
-template <int ES, int G, int NT>
+template <int ES, int G, int NT, int LAYOUT>
 __device__ __forceinline__ void bs_gather_out(const Bs010Params &p, uint64_t c,
                                               const typename Elem<ES>::T *lds) {
   using E = typename Elem<ES>::T;
@@ -194,7 +203,9 @@ __device__ __forceinline__ void bs_gather_out(const Bs010Params &p, uint64_t c,
   uint32_t i = rr - j * a;
   for (uint32_t u = tid; u < nU; u += NT) {
     U w{};
-    if (G > 1 && (p.aligned & 2u)) {
+    if constexpr (G > 1 && LAYOUT == 2) {
+      w = reinterpret_cast<const U *>(lds)[u];  // output order: unit u is LDS unit u
+    } else if (G > 1 && (p.aligned & 2u)) {
       // a is a multiple of G, so the unit's G elements are rows i..i+G-1 of column j.
       const E *col = lds + (t * a + i) * Bp + j;
 #pragma unroll
