@@ -904,9 +904,9 @@ double stage_bw_tbs(const StageRec &r) {
       if (es >= 4) return 5.5;
       const uint64_t G = uint64_t(r.bs.G);
       if (G <= 1) return es == 2 ? 4.5 : 2.8;
-      // Dword units: the stage-in is carry-free (rows of whole units, or unpadded rows), the
-      // gather when the tile's rows a are a multiple of the unit.
-      const int whole = 1 + int(r.d1 % G == 0);
+      // Dword units: both sides are carry-free when either extent is a whole number of units
+      // (the chunk is then staged in the other side's order), one side otherwise.
+      const int whole = (r.d1 % G == 0 || r.d2 % G == 0) ? 2 : 1;
       if (es == 2) return whole == 2 ? 5.3 : whole == 1 ? 4.6 : 4.3;
       return whole == 2 ? 4.7 : whole == 1 ? 3.4 : 2.7;
     }

@@ -17,8 +17,8 @@ struct Bs010Params {
   FastDiv32 div_n, div_B, div_a;
   uint32_t din_t, din_i, din_j;     // per-iteration advance in (tile, row, col) radix
   uint32_t dout_t, dout_j, dout_i;  // per-iteration advance in (tile, col, row) radix
-  // G > 1: bit 0 stage-in by rows (B % G == 0), bit 1 gather by columns (a % G == 0), bit 2
-  // the chunk in input order (stage-in copies units), bit 3 in output order (gather copies)
+  // G > 1, rows at pitch Bp: bit 0 stage-in by rows (B % G == 0), bit 1 gather by columns
+  // (a % G == 0). The unpadded layouts are template parameters (Bs010Plan::layout).
   uint32_t aligned;
 };
 

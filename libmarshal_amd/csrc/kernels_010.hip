@@ -206,15 +206,10 @@ Bs010Params make_bs010_params(unsigned char *base, uint64_t A, uint64_t a, uint6
   // When B (a) is a multiple of G, a unit never crosses an LDS row in the stage-in (a column
   // in the gather), so its G elements take fixed LDS offsets with no carries.
   p.aligned = 0;
-  if (pl.G > 1) {
+  if (pl.G > 1 && pl.layout != 2) {  // output order: both sides need no runtime choice
     const uint64_t G = uint64_t(pl.G);
-    if (pl.layout == 2) {
-      p.aligned = 8u;
-    } else {
-      if (pl.layout == 1) p.aligned |= 4u;
-      else if (B % G == 0) p.aligned |= 1u;
-      if (a % G == 0) p.aligned |= 2u;
-    }
+    if (pl.layout == 0 && B % G == 0) p.aligned |= 1u;
+    if (a % G == 0) p.aligned |= 2u;
   }
   return p;
 }
