@@ -399,7 +399,7 @@ def main():
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_010_bs<4,1,256,16>",
+                     "traffic": traffic, "kernel": "k_010_bs<4,1,256,16,0>",
                      "launch_us": round(launch_s * 1e6, 2),
                      "algorithmic_bytes_per_launch": 2 * nbytes},
         "wall_s": round(wall, 3),
