@@ -352,7 +352,8 @@ def main():
         x = torch.randint(-2**31, 2**31 - 1, (M * N,), dtype=torch.int32, device="cuda")
         s = time_calls(lambda: lm.transpose(x, M // 18, 18, N // 72, 72, 1, 3), reps, stream)
         extra["transpose_1800x7200_i32_3stage_GBs"] = round(2 * M * N * 4 / s / 1e9, 1)
-        s = time_calls(lambda: lm.transpose_full(x, M, N), reps, stream)
+        # ~45 us per call: 50 calls so that launch gaps and clock noise average out
+        s = time_calls(lambda: lm.transpose_full(x, M, N), 50, stream)
         extra["transpose_1800x7200_i32_planner_GBs"] = round(2 * M * N * 4 / s / 1e9, 1)
         extra["planner_route"] = lm.last_route()
         del x
@@ -360,7 +361,7 @@ def main():
         sizes = {}
         for (M, N) in ((2500, 5100), (3200, 4000), (3900, 3300), (5100, 2500), (7200, 1800)):
             x = torch.rand(M * N, device="cuda")
-            s = time_calls(lambda: lm.transpose_full(x, M, N), reps, stream)
+            s = time_calls(lambda: lm.transpose_full(x, M, N), 50, stream)
             sizes[f"{M}x{N}"] = {"GBs": round(2 * M * N * 4 / s / 1e9, 1), "route": lm.last_route()}
             del x
         extra["transpose_paper_sizes_f32_planner"] = sizes
