@@ -98,6 +98,10 @@ def test_planner_route_choice_host_only():
         assert lm.plan_transpose(3001, 7001, 4)[2] == 9
         assert lm.plan_transpose(98304, 131071, 8)[2] in (6, 7)
         assert lm.plan_transpose(250007, 199999, 1)[2] in (6, 7)
+        # 2.4 GB: within the default 8 GiB cap through scratch, beyond a 2 GiB cap in stripes
+        assert lm.plan_transpose(20011, 15013, 8)[2] == 9
+        lm.set_tuning("st_scratch_mb", 2048)
+        assert lm.plan_transpose(20011, 15013, 8)[2] in (6, 7)
         lm.set_tuning("st_scratch_mb", 0)
         for (M, N, es) in ((3001, 7001, 4), (1031, 997, 1), (1800, 7200, 4)):
             assert lm.plan_transpose(M, N, es)[2] not in (6, 7, 9), (M, N, es)
