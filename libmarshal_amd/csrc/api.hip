@@ -99,6 +99,7 @@ Knob g_knobs[] = {
     {"st_skew", {1}, 0},
     {"copy_nt", {1}, 0},
     {"copy_blocks_per_cu", {64}},
+    {"copy_nt_store_mb", {256}, 0},  // 0: non-temporal stores for every copy
     {"oop_nt", {2}, 0},
     {"pad_nt", {1}, 0},
     {"pl_min_bytes", {128}},

@@ -503,7 +503,8 @@ bool sq_fits(size_t es, uint64_t n) {
 // non-temporal loads (measured 6.2 TB/s against 4.6 for one plain vector per lane on a cold
 // 1 GiB buffer).
 constexpr int kCopyUnroll = 4;
-template <bool NTL>
+// NTL / NTS: non-temporal loads / stores.
+template <bool NTL, bool NTS>
 __global__ __launch_bounds__(256) void k_copy16(U128 *dst, const U128 *src, uint64_t n) {
   const uint64_t stride = uint64_t(gridDim.x) * 256 * kCopyUnroll;
   for (uint64_t b = uint64_t(blockIdx.x) * 256 * kCopyUnroll + threadIdx.x; b < n; b += stride) {
@@ -516,7 +517,12 @@ __global__ __launch_bounds__(256) void k_copy16(U128 *dst, const U128 *src, uint
 #pragma unroll
     for (int u = 0; u < kCopyUnroll; u++) {
       const uint64_t i = b + uint64_t(u) * 256;
-      if (i < n) dst[i] = v[u];
+      if (i < n) {
+        if (NTS)
+          __builtin_nontemporal_store(v[u], dst + i);
+        else
+          dst[i] = v[u];
+      }
     }
   }
 }
@@ -545,7 +551,7 @@ __global__ __launch_bounds__(256) void k_copy2d(unsigned char *dst, uint64_t dpi
 // Copy between any two addresses: the destination in aligned 16-byte chunks (the source read
 // as aligned vectors or, at another alignment, as 16-byte windows), the bytes before the
 // destination's first aligned address and after its last whole chunk by thread 0.
-template <bool SRC_ALIGNED, bool NTL>
+template <bool SRC_ALIGNED, bool NTL, bool NTS>
 __global__ __launch_bounds__(256) void k_copy_any(unsigned char *dst, const unsigned char *src,
                                                   uint64_t bytes, uint32_t head, uint64_t nb) {
   const uint64_t stride = uint64_t(gridDim.x) * 256;
@@ -569,7 +575,12 @@ __global__ __launch_bounds__(256) void k_copy_any(unsigned char *dst, const unsi
 #pragma unroll
     for (int u = 0; u < kCopyUnroll; u++) {
       const uint64_t i = b + uint64_t(u) * 256;
-      if (i < nb) d[i] = v[u];
+      if (i < nb) {
+        if (NTS)
+          __builtin_nontemporal_store(v[u], d + i);
+        else
+          d[i] = v[u];
+      }
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -584,31 +595,39 @@ hipError_t launch_copy(unsigned char *dst, const unsigned char *src, uint64_t by
   const uint32_t cap = uint32_t(device_info().cus) * 16;
   const uint32_t cap16 = uint32_t(device_info().cus) * uint32_t(tuning("copy_blocks_per_cu"));
   const uint64_t per_block = 256 * kCopyUnroll;
+  // Stores: non-temporal from "copy_nt_store_mb" on. Smaller copies land in the 256 MB
+  // last-level cache, where the pass that reads them back finds them (measured 4-5 % faster
+  // for 52-168 MB through-scratch transposes; 10 % slower for a 403 MB one).
+  const int64_t nts_mb = tuning("copy_nt_store_mb");
+  const bool nts = bytes >= (uint64_t(nts_mb) << 20);
+  const bool ntl = tuning("copy_nt") != 0;
   if (bytes >= 64 && ((uintptr_t(dst) | uintptr_t(src) | bytes) & 15u) != 0) {
     const uint32_t head = uint32_t((16u - (uintptr_t(dst) & 15u)) & 15u);
     const uint64_t nb = (bytes - head) / 16;
     const uint32_t blocks = uint32_t(
         std::min<uint64_t>(std::max<uint64_t>((nb + per_block - 1) / per_block, 1), cap16));
+    auto go = [&](auto k) {
+      hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, st, dst, src, bytes, head, nb);
+    };
     if (((uintptr_t(src) + head) & 15u) != 0)
-      hipLaunchKernelGGL((k_copy_any<false, false>), dim3(blocks), dim3(256), 0, st, dst, src,
-                         bytes, head, nb);
-    else if (tuning("copy_nt"))
-      hipLaunchKernelGGL((k_copy_any<true, true>), dim3(blocks), dim3(256), 0, st, dst, src,
-                         bytes, head, nb);
+      nts ? go(k_copy_any<false, false, true>) : go(k_copy_any<false, false, false>);
+    else if (ntl)
+      nts ? go(k_copy_any<true, true, true>) : go(k_copy_any<true, true, false>);
     else
-      hipLaunchKernelGGL((k_copy_any<true, false>), dim3(blocks), dim3(256), 0, st, dst, src,
-                         bytes, head, nb);
+      nts ? go(k_copy_any<true, false, true>) : go(k_copy_any<true, false, false>);
     return hipGetLastError();
   }
   if (((uintptr_t(dst) | uintptr_t(src) | bytes) & 15u) == 0) {
     const uint64_t n = bytes / 16;
     const uint32_t blocks = uint32_t(std::min<uint64_t>((n + per_block - 1) / per_block, cap16));
-    if (tuning("copy_nt"))
-      hipLaunchKernelGGL(k_copy16<true>, dim3(blocks), dim3(256), 0, st,
-                         reinterpret_cast<U128 *>(dst), reinterpret_cast<const U128 *>(src), n);
+    auto go = [&](auto k) {
+      hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, st, reinterpret_cast<U128 *>(dst),
+                         reinterpret_cast<const U128 *>(src), n);
+    };
+    if (ntl)
+      nts ? go(k_copy16<true, true>) : go(k_copy16<true, false>);
     else
-      hipLaunchKernelGGL(k_copy16<false>, dim3(blocks), dim3(256), 0, st,
-                         reinterpret_cast<U128 *>(dst), reinterpret_cast<const U128 *>(src), n);
+      nts ? go(k_copy16<false, true>) : go(k_copy16<false, false>);
   } else {
     const uint32_t blocks = uint32_t(std::min<uint64_t>((bytes + 255) / 256, cap));
     hipLaunchKernelGGL(k_copy1, dim3(blocks), dim3(256), 0, st, dst, src, bytes);
