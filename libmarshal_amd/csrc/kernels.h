@@ -156,15 +156,16 @@ hipError_t launch_copy2d(unsigned char *dst, uint64_t dpitch, const unsigned cha
 // 16-byte aligned) to dst with the rows back to back; dst is written in aligned 16-byte
 // chunks whatever its alignment.
 hipError_t launch_unpitch(unsigned char *dst, const unsigned char *src, uint64_t RB,
-                          uint64_t P, uint64_t rows, hipStream_t st);
+                          uint64_t P, uint64_t rows, hipStream_t st, int nt_store = -1);
 // Out-of-place transpose of `blocks` R x C matrices (src and dst must not overlap), tiles of
 // at most `tile_bytes` staged in LDS. src_end (0: just the blocks) is how many bytes from
 // src may be read: 16-byte window loads of misaligned rows read up to 31 bytes past a row.
-// dst_pitch (0: R * es) is the byte distance between destination rows.
+// dst_pitch (0: R * es) is the byte distance between destination rows; nt (-1: the "oop_nt"
+// knob) selects non-temporal loads (bit 0) and stores (bit 1).
 hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, size_t es,
                                 uint64_t blocks, uint64_t R, uint64_t C, uint64_t src_stride,
                                 uint64_t dst_stride, size_t tile_bytes, hipStream_t st,
-                                uint64_t src_end = 0, uint64_t dst_pitch = 0);
+                                uint64_t src_end = 0, uint64_t dst_pitch = 0, int nt = -1);
 
 // Out-of-place 100 over super-elements of any size: `inst` blocks of R x C super-elements of
 // S bytes, [inst][R][C] in src -> [inst][C][R] in dst (no overlap). src must be 16-byte

@@ -713,7 +713,7 @@ __global__ __launch_bounds__(256) void k_unpitch(unsigned char *dst,
 }
 
 hipError_t launch_unpitch(unsigned char *dst, const unsigned char *src, uint64_t RB,
-                          uint64_t P, uint64_t rows, hipStream_t st) {
+                          uint64_t P, uint64_t rows, hipStream_t st, int nt_store) {
   const uint64_t total = RB * rows;
   if (total == 0) return hipSuccess;
   if (RB < 16 || P < RB || (P & 15u) || (reinterpret_cast<uintptr_t>(src) & 15u))
@@ -726,7 +726,8 @@ hipError_t launch_unpitch(unsigned char *dst, const unsigned char *src, uint64_t
   const uint32_t blocks = uint32_t(std::min<uint64_t>(
       std::max<uint64_t>((nb + per_block - 1) / per_block, 1),
       uint64_t(device_info().cus) * uint64_t(tuning("copy_blocks_per_cu"))));
-  if (tuning("oop_nt") & 2)
+  if (nt_store < 0) nt_store = (tuning("oop_nt") & 2) ? 1 : 0;
+  if (nt_store)
     hipLaunchKernelGGL(k_unpitch<true>, dim3(blocks), dim3(256), 0, st, dst, src, RB, P,
                        total, head, nb, FastDiv64(RB));
   else
@@ -792,7 +793,7 @@ hipError_t launch_oop_es(const OopParams &p, bool vin, int out, size_t lds, hipS
 hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, size_t es,
                                 uint64_t blocks, uint64_t R, uint64_t C, uint64_t src_stride,
                                 uint64_t dst_stride, size_t tile_bytes, hipStream_t st,
-                                uint64_t src_end, uint64_t dst_pitch) {
+                                uint64_t src_end, uint64_t dst_pitch, int nt) {
   if (blocks == 0 || R == 0 || C == 0) return hipSuccess;
   if (dst_pitch == 0) dst_pitch = R * es;
   if (dst_pitch < R * es) return hipErrorInvalidValue;
@@ -886,7 +887,7 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
   p.tc_log2 = tcl;
   p.PB = PB;
   p.wstore = uint32_t(tuning("st_wstore"));
-  p.nt = uint32_t(tuning("oop_nt"));
+  p.nt = uint32_t(nt >= 0 ? nt : tuning("oop_nt"));
   const uint32_t TC = 1u << tcl;
   p.ntr = (R + (1ull << trl) - 1) >> trl;
   p.ntc = (C + (1ull << tcl) - 1) >> tcl;
