@@ -141,6 +141,10 @@ bool plan_010_bs(size_t es, uint64_t A, uint64_t a, uint64_t B, size_t lds_budge
     Bp = B;
     tile_lds = a * Bp * es;
   }
+  // At most 16 KiB per round (256 lanes x upt units of G x es bytes): 8- and 16-byte elements
+  // take 8 and 4 units per lane. With 16 units, 16-byte elements made 64 KiB chunks, two
+  // workgroups per CU by LDS, and ran 5.8 TB/s against 6.2 with 4 units.
+  while (upt > 4 && uint64_t(256) * upt * G * es > (16u << 10)) upt /= 2;
   // One round of 256 threads x upt units per chunk when tiles are small.
   uint64_t target = uint64_t(256) * upt * G;
   uint64_t g = std::max<uint64_t>(1, target / n);
