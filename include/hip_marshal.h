@@ -249,7 +249,10 @@ void hip_marshal_set_profiling(int on);
  * "oop_pitched" (through scratch with misaligned destination rows: 0 copy first, 2 transpose
  * first into 16-byte-aligned scratch rows and copy back, 1 (default) the latter for 1- and
  * 2-byte elements and from 512 MiB), "st_pt_vw", "st_rotate" (0/1), "copy_blocks_per_cu"
- * (grid cap of the 16-byte copy), "oop_min_rows" (at least this many tile rows in the
+ * (grid cap of the 16-byte copy), "oop_min_tiles" (calls that have the device to
+ * themselves: through-scratch tiles shrink, down to 4 KiB, until a group makes at least
+ * this many; 0 = no minimum),
+ * "oop_min_rows" (at least this many tile rows in the
  * out-of-place transpose of 1- to 4-byte elements; 0 = no minimum),
  * "copy_nt" (0/1: non-temporal source loads in the copy into scratch), "copy_nt_store_mb"
  * (copies of at least this many MiB store non-temporally; default 256), "oop_nt" (bit 0:
@@ -270,9 +273,9 @@ void hip_marshal_set_profiling(int on);
  * Replaces the reference's hard-coded work-group tables (src/cl_marshal.cc:165-231,
  * 388-465). The setter returns true on an unknown key or a value outside [min, 2^30] (min is
  * 0 for "st_scratch_mb", "st_rotate", "st_wstore", "st_skew", "copy_nt", "oop_nt", "pad_nt",
- * "oop_elem_tile_kb", "oop_pitched", "oop_min_rows", "copy_nt_store_mb", "gather_row_bytes",
- * "st_tile_rows", "pad_shape" and "autotune", 1 otherwise); the getter returns 0 for an
- * unknown key. */
+ * "oop_elem_tile_kb", "oop_pitched", "oop_min_rows", "oop_min_tiles", "copy_nt_store_mb",
+ * "gather_row_bytes", "st_tile_rows", "pad_shape" and "autotune", 1 otherwise); the getter
+ * returns 0 for an unknown key. */
 bool hip_marshal_set_tuning(const char *key, int64_t value);
 /* Measured autotuner for an M x N transpose of elem_bytes elements: the planner's cheapest
  * candidate routes and the best of each route family are run on `buf` (on `stream`), each

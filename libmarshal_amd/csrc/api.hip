@@ -88,6 +88,7 @@ Knob g_knobs[] = {
     {"oop_elem_tile_kb", {8}, 0},  // 0: element-store tiles take st_tile_kb too
     {"oop_pitched", {1}, 0},
     {"oop_min_rows", {256}, 0},
+    {"oop_min_tiles", {512}, 0},
     {"st_pt_vw", {16}},
     {"st_rotate", {1}, 0},
     {"autotune", {0}, 0},
@@ -718,7 +719,9 @@ size_t stages_scratch_bytes(const std::vector<StageRec> &stages) {
 // Runs stage records in order on one stream. Scratch (visited bits, the stripe route's
 // buffer) comes from `scr` when given (at least stages_scratch_bytes, private to the caller),
 // else from the device pool.
-static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
+// `alone`: nothing else of this library runs on the device meanwhile (a plain call, or a
+// server request with every other stream idle), which only tunes launch shapes.
+static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st, bool alone,
                             unsigned char *scr) {
   for (const StageRec &r : stages) {
     switch (r.kind) {
@@ -759,7 +762,17 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
           tmp = reinterpret_cast<unsigned char *>(lease->get(oop_scratch_bytes(r)));
         }
         if (!tmp) return true;
-        const size_t tile = size_t(tuning("st_tile_kb")) << 10;
+        // A call that runs alone: tiles small enough that a group makes at least
+        // "oop_min_tiles" of them (one workgroup each, on 256 CUs), down to 4 KiB. 512
+        // measured 2-7 % faster on 2-17 MB matrices (1031 x 997 f32 481 -> 511 GB/s,
+        // 4000 x 1000 f32 1142 -> 1211), but where several matrices run at once (a batch's
+        // side streams) the larger tiles did better (its 1-16 MB class 8 % slower).
+        size_t tile = size_t(tuning("st_tile_kb")) << 10;
+        const int64_t mt = alone ? tuning("oop_min_tiles") : 0;
+        if (mt) {
+          const uint64_t per = std::min(r.chunk, r.d0) * block / uint64_t(mt);
+          tile = std::min<size_t>(tile, std::max<size_t>(4096, size_t(per)));
+        }
         for (uint64_t g0 = 0; g0 < r.d0; g0 += r.chunk) {
           const uint64_t n = std::min(r.chunk, r.d0 - g0);
           unsigned char *dst = r.buf + g0 * block;
@@ -847,11 +860,12 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
 }
 
 bool run_stages(const std::vector<StageRec> &stages, hipStream_t st) {
-  return run_stages_impl(stages, st, nullptr);
+  return run_stages_impl(stages, st, true, nullptr);
 }
 
-bool run_stages_on(const std::vector<StageRec> &stages, hipStream_t st, unsigned char *scratch) {
-  return run_stages_impl(stages, st, scratch);
+bool run_stages_on(const std::vector<StageRec> &stages, hipStream_t st, unsigned char *scratch,
+                   bool alone) {
+  return run_stages_impl(stages, st, alone, scratch);
 }
 
 // One 0100 / 010 step on its own (entry points of single stages and the AoS / SoA

@@ -264,7 +264,9 @@ bool plan_transpose_stages(std::vector<StageRec> &out, unsigned char *buf, size_
 Choice plan_full(size_t es, uint64_t M, uint64_t N, bool allow_st = true);
 bool run_stages(const std::vector<StageRec> &stages, hipStream_t st);
 // Same, with caller-owned scratch of at least stages_scratch_bytes(stages) bytes.
-bool run_stages_on(const std::vector<StageRec> &stages, hipStream_t st, unsigned char *scratch);
+// alone: nothing else of this library runs on the device meanwhile (tunes launch shapes only).
+bool run_stages_on(const std::vector<StageRec> &stages, hipStream_t st, unsigned char *scratch,
+                   bool alone = false);
 size_t stages_scratch_bytes(const std::vector<StageRec> &stages);
 // Appends the row / column decomposition of an M x N transpose of S-byte elements; false
 // when it does not fit the LDS budget.

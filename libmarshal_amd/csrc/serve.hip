@@ -130,14 +130,14 @@ bool launch_batch(Server &s, Slot &sl, const std::vector<hip_marshal_desc> &desc
 // A large (or lone) request runs with the stage list its plain entry point would use (stripe
 // route included), on the slot's stream with the slot's own scratch: nothing to amortise,
 // and per-request kernels beat the grouped forms there.
-bool launch_solo(Server &s, Slot &sl, const hip_marshal_desc &d) {
+bool launch_solo(Server &s, Slot &sl, const hip_marshal_desc &d, bool alone) {
   (void)s;
   std::vector<StageRec> list;
   std::string route;
   if (!plan_desc(d, list, route, true)) return false;
   size_t need = stages_scratch_bytes(list);
   if (need && !grow(reinterpret_cast<void **>(&sl.scr), &sl.scr_cap, need, false)) return false;
-  if (run_stages_on(list, sl.st, sl.scr)) return false;
+  if (run_stages_on(list, sl.st, sl.scr, alone)) return false;
   return hipEventRecord(sl.ev, sl.st) == hipSuccess;
 }
 
@@ -159,13 +159,16 @@ void start_on_slot(Server &s, Slot &sl, const std::vector<int64_t> &ids,
   sl.bytes = add;
   sl.reqs = ids;
   if (solo) s.solo_running++;
+  bool alone = true;  // no other stream busy: the request has the device to itself
+  for (const Slot &o : s.slots)
+    if (&o != &sl && o.busy) alone = false;
   s.batches++;
   s.max_in_batch = std::max<int64_t>(s.max_in_batch, int64_t(ids.size()));
   lk.unlock();
   int prev = -1;
   hipGetDevice(&prev);
   if (prev != s.device) hipSetDevice(s.device);
-  const bool ok = plain ? launch_solo(s, sl, descs[0]) : launch_batch(s, sl, descs);
+  const bool ok = plain ? launch_solo(s, sl, descs[0], alone) : launch_batch(s, sl, descs);
   if (prev != s.device && prev >= 0) hipSetDevice(prev);
   lk.lock();
   sl.launching = false;

@@ -115,7 +115,7 @@ def test_tuning_keys_defaults_and_limits():
     defaults = {"st_scratch_mb": 8192, "batch_solo_kb": 64, "batch_solo_lanes": 4, "st_skew": 1, "copy_nt": 1,
                 "oop_nt": 2, "pad_nt": 1, "gather_tile_kb": 24, "gather_row_bytes": 256,
                 "bs_upt": 16, "bs_blocks_per_cu": 192, "oop_elem_tile_kb": 8,
-                "copy_blocks_per_cu": 64, "copy_nt_store_mb": 256, "oop_pitched": 1, "oop_min_rows": 256, "st_super_bytes": 8192,
+                "copy_blocks_per_cu": 64, "copy_nt_store_mb": 256, "oop_pitched": 1, "oop_min_rows": 256, "oop_min_tiles": 512, "st_super_bytes": 8192,
                 "st_max_height": 4096,
                 "autotune": 0}
     for k, v in defaults.items():
