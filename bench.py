@@ -45,6 +45,12 @@ def parse():
     p.add_argument("--profile-soa", type=int, default=0,
                    help="only run N SoA->ASTA->SoA round trips of the 1 GiB struct array "
                         "(the cycle-leader form), after one untimed round trip")
+    p.add_argument("--profile-3stage", type=int, default=0,
+                   help="only run N round trips of the reference's 3-stage route on 1800 x 7200 "
+                        "int32 (a=18, b=72) and back (a=72, b=18), after one untimed round trip")
+    p.add_argument("--profile-pt010", type=int, default=0,
+                   help="only run N AoS->ASTA->AoS round trips of the 1 GiB struct array with "
+                        "the cycle-following 010 kernel, after one untimed round trip")
     p.add_argument("--no-large", action="store_true",
                    help="skip the 103 GB and 50 GB prime-dimension transposes")
     p.add_argument("--profile", type=int, default=0,
@@ -278,6 +284,34 @@ def main():
     d = torch.rand(n * w, device="cuda", generator=g) * 2 - 1  # U[-1, 1)
     ref = d.clone()
 
+    if args.profile_3stage:
+        M, N = 1800, 7200
+        x = torch.randint(-2**31, 2**31 - 1, (M * N,), dtype=torch.int32, device="cuda")
+        x0 = x.clone()
+
+        def tri_round():
+            return lm.transpose(x, M // 18, 18, N // 72, 72, 1, 3) or \
+                lm.transpose(x, N // 72, 72, M // 18, 18, 1, 3)
+        if tri_round():
+            raise RuntimeError("3-stage transpose failed")
+        s = time_calls(tri_round, args.profile_3stage, stream, warm=False) / 2
+        print(json.dumps({"profile_3stage": {"route": lm.last_route(),
+                                             "GBs": round(2 * M * N * 4 / s / 1e9, 1),
+                                             "round_trips": args.profile_3stage,
+                                             "restored": bool(torch.equal(x, x0))}}), flush=True)
+        return
+    if args.profile_pt010:
+        def pt_round():
+            return lm.aos_asta(d, n, w, t, method=2, elem_bytes=4) or \
+                lm.asta_aos(d, n, w, t, method=2, elem_bytes=4)
+        if pt_round():
+            raise RuntimeError("conversion failed")
+        s = time_calls(pt_round, args.profile_pt010, stream, warm=False) / 2
+        print(json.dumps({"profile_pt010": {"route": lm.last_route(),
+                                            "GBs": round(2 * nbytes / s / 1e9, 1),
+                                            "round_trips": args.profile_pt010,
+                                            "restored": bool(torch.equal(d, ref))}}), flush=True)
+        return
     if args.profile_soa:
         def soa_round():
             return lm.soa_asta(d, n, w, t, elem_bytes=4) or lm.asta_soa(d, n, w, t, elem_bytes=4)

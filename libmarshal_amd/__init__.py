@@ -153,13 +153,24 @@ EXPORTED = [
 ]
 
 
-def _ptr(t):
+def _ptr(t, need=None):
+    """Device address of a contiguous ROCm tensor; with `need`, the tensor must hold at least
+    that many bytes (the C ABI trusts the dimensions it is given)."""
     import torch
     if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
         raise TypeError("expected a torch tensor on a ROCm device")
     if not t.is_contiguous():
         raise ValueError("tensor must be contiguous")
+    if need is not None and t.numel() * t.element_size() < need:
+        raise ValueError(f"tensor holds {t.numel() * t.element_size()} bytes, the call needs {need}")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _need(es, *dims):
+    n = es
+    for d in dims:
+        n *= max(int(d), 0)
+    return n
 
 
 def _stream(stream):
@@ -197,68 +208,80 @@ def _es(t, elem_bytes):
 
 def transpose(buf, A, a, B, b, R=1, stages=3, elapsed=None, elem_bytes=None, stream=None):
     """Full in-place transpose of an (A*a) x (B*b) matrix (cl_transpose)."""
-    return lib().hip_transpose_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), A, a, B, b,
+    es = _es(buf, elem_bytes)
+    return lib().hip_transpose_es(_stream(stream), _ptr(buf, _need(es, A, a, B, b)), es, A, a, B, b,
                                   R, stages, _et(elapsed))
 
 
 def transpose_full(buf, M, N, elapsed=None, elem_bytes=None, stream=None):
     """M x N transpose with planner-chosen tile factors and route."""
-    return lib().hip_transpose_full_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), M, N,
+    es = _es(buf, elem_bytes)
+    return lib().hip_transpose_full_es(_stream(stream), _ptr(buf, _need(es, M, N)), es, M, N,
                                        _et(elapsed))
 
 
 def transpose_010_bs(buf, A, a, B, elapsed=None, elem_bytes=None, stream=None):
-    return lib().hip_transpose_010_bs_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), A,
+    es = _es(buf, elem_bytes)
+    return lib().hip_transpose_010_bs_es(_stream(stream), _ptr(buf, _need(es, A, a, B)), es, A,
                                          a, B, _et(elapsed))
 
 
 def transpose_010_pttwac(buf, A, a, B, elapsed=None, R=1, P=1, elem_bytes=None, stream=None):
-    return lib().hip_transpose_010_pttwac_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes),
+    es = _es(buf, elem_bytes)
+    return lib().hip_transpose_010_pttwac_es(_stream(stream), _ptr(buf, _need(es, A, a, B)), es,
                                              A, a, B, _et(elapsed), R, P)
 
 
 def transpose_100(buf, A, B, b, elapsed=None, elem_bytes=None, stream=None):
-    return lib().hip_transpose_100_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), A, B,
+    es = _es(buf, elem_bytes)
+    return lib().hip_transpose_100_es(_stream(stream), _ptr(buf, _need(es, A, B, b)), es, A, B,
                                       b, _et(elapsed))
 
 
 def transpose_0100(buf, A, a, B, b, elapsed=None, elem_bytes=None, stream=None):
-    return lib().hip_transpose_0100_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), A, a,
+    es = _es(buf, elem_bytes)
+    return lib().hip_transpose_0100_es(_stream(stream), _ptr(buf, _need(es, A, a, B, b)), es, A, a,
                                        B, b, _et(elapsed))
 
 
 def aos_asta(buf, height, width, tile_size, method=0, elapsed=None, elem_bytes=None,
              stream=None):
     """AoS [h][w] -> ASTA [h/t][w][t]; method 0 = BS then PTTWAC, 1 = BS, 2 = PTTWAC."""
-    return lib().hip_aos_asta_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), height,
+    es = _es(buf, elem_bytes)
+    return lib().hip_aos_asta_es(_stream(stream), _ptr(buf, _need(es, height, width)), es, height,
                                  width, tile_size, method, _et(elapsed))
 
 
 def asta_aos(buf, height, width, tile_size, method=0, elapsed=None, elem_bytes=None,
              stream=None):
-    return lib().hip_asta_aos_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), height,
+    es = _es(buf, elem_bytes)
+    return lib().hip_asta_aos_es(_stream(stream), _ptr(buf, _need(es, height, width)), es, height,
                                  width, tile_size, method, _et(elapsed))
 
 
 def soa_asta(buf, height, width, tile_size, elapsed=None, elem_bytes=None, stream=None):
     """SoA [w][h] -> ASTA [h/t][w][t] (100(w, h/t, t))."""
-    return lib().hip_soa_asta_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), height,
+    es = _es(buf, elem_bytes)
+    return lib().hip_soa_asta_es(_stream(stream), _ptr(buf, _need(es, height, width)), es, height,
                                  width, tile_size, _et(elapsed))
 
 
 def asta_soa(buf, height, width, tile_size, elapsed=None, elem_bytes=None, stream=None):
-    return lib().hip_asta_soa_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), height,
+    es = _es(buf, elem_bytes)
+    return lib().hip_asta_soa_es(_stream(stream), _ptr(buf, _need(es, height, width)), es, height,
                                  width, tile_size, _et(elapsed))
 
 
 def padding(buf, x_size, y_size, pad_size, elapsed=None, elem_bytes=None, stream=None):
-    return lib().hip_padding_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), x_size,
+    es = _es(buf, elem_bytes)
+    return lib().hip_padding_es(_stream(stream), _ptr(buf, _need(es, y_size, pad_size)), es, x_size,
                                 y_size, pad_size, _et(elapsed))
 
 
 def unpadding(buf, x_size, y_size, pad_size, elapsed=None, elem_bytes=None, stream=None):
-    return lib().hip_unpadding_es(_stream(stream), _ptr(buf), _es(buf, elem_bytes), x_size,
-                                  y_size, pad_size, _et(elapsed))
+    es = _es(buf, elem_bytes)
+    return lib().hip_unpadding_es(_stream(stream), _ptr(buf, _need(es, y_size, pad_size)), es,
+                                  x_size, y_size, pad_size, _et(elapsed))
 
 
 def set_lds_budget(nbytes):
@@ -286,7 +309,8 @@ def autotune(buf, M, N, elem_bytes=None, stream=None):
     """Times the planner's candidate routes for an M x N transpose on `buf` (left unchanged)
     and makes the fastest the plan for this shape. Returns its stage code."""
     st = ctypes.c_int(0)
-    if lib().hip_marshal_autotune(_stream(stream), _ptr(buf), _es(buf, elem_bytes), M, N,
+    es = _es(buf, elem_bytes)
+    if lib().hip_marshal_autotune(_stream(stream), _ptr(buf, _need(es, M, N)), es, M, N,
                                   ctypes.byref(st)):
         raise RuntimeError("hip_marshal_autotune failed")
     return st.value
@@ -329,7 +353,7 @@ class Batch:
         arr = (Desc * len(items))()
         self._keep = []
         for i, (t, op, es, rows, cols, tile) in enumerate(items):
-            arr[i] = Desc(_ptr(t).value, op, es, rows, cols, tile)
+            arr[i] = Desc(_ptr(t, _need(es, rows, cols)).value, op, es, rows, cols, tile)
             self._keep.append(t)
         self._h = lib().hip_marshal_batch_create(arr, len(items), 1 if use_graph else 0)
         if not self._h:
@@ -370,7 +394,7 @@ class Server:
 
     def submit(self, tensor, op, elem_bytes, rows, cols, tile=0, arrival=0.0, ptr=None):
         """Submits a conversion of `tensor` (or of the raw device pointer `ptr`); returns its id."""
-        p = ptr if ptr is not None else _ptr(tensor).value
+        p = ptr if ptr is not None else _ptr(tensor, _need(elem_bytes, rows, cols)).value
         d = Desc(p, op, elem_bytes, rows, cols, tile)
         rid = lib().hip_marshal_server_submit_at(self._h, ctypes.byref(d), arrival)
         if rid < 0:

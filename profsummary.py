@@ -1,7 +1,7 @@
 """Turns rocprofv3 CSV output into the short summaries kept under profiles/.
 
-    python -m libmarshal_amd.profsummary stats  <kernel_stats.csv> <out.txt> <command...>
-    python -m libmarshal_amd.profsummary pmc    <fetch_counter_collection.csv>
+    python profsummary.py stats  <kernel_stats.csv> <out.txt> <command...>
+    python profsummary.py pmc    <fetch_counter_collection.csv>
                                                 <write_counter_collection.csv>
                                                 <kernel-substring> <copy-bytes> <out.json>
 
