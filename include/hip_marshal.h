@@ -148,6 +148,24 @@ bool hip_unpadding_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t 
                       int64_t y_size, int64_t pad_size, uint64_t *elapsed_ns);
 
 /* ---------------------------------------------------------------------------------------
+ * No-allocation variant. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same result as hip_transpose_0100_es (A instances of [a][B][b] ->
+ * [B][a][b], elem_bytes in {1,2,4,8,16}) but restricted to the in-place forms (LDS-staged,
+ * row/column shuffles, LDS bits, global bits) and with the visited bits of the global-bit form
+ * in a fixed device array of HIP_MARSHAL_NOALLOC_SCRATCH_BYTES (32 MiB) that each device holds
+ * from the first call on it until the process ends. A step whose bits exceed that is rejected
+ * with an error (true) before anything runs. The call allocates no device memory, creates no
+ * events and does not time itself (no elapsed argument); it is stream-ordered and never
+ * synchronizes. Every call on one device shares that device's array and clears the bits it
+ * uses before its kernels run: calls that overlap on one device (concurrent streams, or host
+ * threads) race and must be serialized by the caller.
+ * ------------------------------------------------------------------------------------- */
+#define HIP_MARSHAL_NOALLOC_SCRATCH_BYTES (32u << 20)
+bool hip_transpose_0100_noalloc_es(hipStream_t stream, void *src, size_t elem_bytes, int64_t A,
+                                   int64_t a, int64_t B, int64_t b);
+
+/* ---------------------------------------------------------------------------------------
  * Batched form: many conversions of uneven shapes and element sizes in one graph launch.
  * No counterpart in the reference (it launches one kernel per stage per matrix).
  * hip_marshal_batch_create plans every descriptor (the planner picks tile factors, routes and

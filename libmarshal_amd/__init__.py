@@ -75,6 +75,7 @@ def lib():
         "hip_transpose_010_pttwac_es": (b, [vp, vp, sz, i64, i64, i64, u64p, i32, i32]),
         "hip_transpose_100_es": (b, [vp, vp, sz, i64, i64, i64, u64p]),
         "hip_transpose_0100_es": (b, [vp, vp, sz, i64, i64, i64, i64, u64p]),
+        "hip_transpose_0100_noalloc_es": (b, [vp, vp, sz, i64, i64, i64, i64]),
         "hip_transpose_es": (b, [vp, vp, sz, i64, i64, i64, i64, i32, i32, u64p]),
         "hip_transpose_full_es": (b, [vp, vp, sz, i64, i64, u64p]),
         "hip_aos_asta_es": (b, [vp, vp, sz, i64, i64, i64, i32, u64p]),
@@ -135,7 +136,7 @@ EXPORTED = [
     "hip_transpose_010_bs", "hip_transpose_010_pttwac", "hip_transpose_100",
     "hip_transpose_0100", "hip_padding", "hip_unpadding", "hip_marshal_finalize",
     "hip_transpose_010_bs_es", "hip_transpose_010_pttwac_es", "hip_transpose_100_es",
-    "hip_transpose_0100_es", "hip_transpose_es", "hip_transpose_full_es", "hip_aos_asta_es",
+    "hip_transpose_0100_es", "hip_transpose_0100_noalloc_es", "hip_transpose_es", "hip_transpose_full_es", "hip_aos_asta_es",
     "hip_asta_aos_es", "hip_soa_asta_es", "hip_asta_soa_es", "hip_padding_es",
     "hip_unpadding_es", "hip_marshal_set_lds_budget", "hip_marshal_get_lds_budget",
     "hip_marshal_set_profiling", "hip_marshal_last_route", "hip_marshal_reserve",
@@ -242,6 +243,14 @@ def transpose_0100(buf, A, a, B, b, elapsed=None, elem_bytes=None, stream=None):
     es = _es(buf, elem_bytes)
     return lib().hip_transpose_0100_es(_stream(stream), _ptr(buf, _need(es, A, a, B, b)), es, A, a,
                                        B, b, _et(elapsed))
+
+
+def transpose_0100_noalloc(buf, A, a, B, b, elem_bytes=None, stream=None):
+    """0100 step that allocates no device memory (hip_transpose_0100_noalloc_es): in-place forms
+    only, visited bits in the library's fixed 32 MiB device array; not timed."""
+    es = _es(buf, elem_bytes)
+    return lib().hip_transpose_0100_noalloc_es(_stream(stream), _ptr(buf, _need(es, A, a, B, b)),
+                                               es, A, a, B, b)
 
 
 def aos_asta(buf, height, width, tile_size, method=0, elapsed=None, elem_bytes=None,

@@ -15,7 +15,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["kernels_010.hip", "kernels_100.hip", "kernels_rc.hip", "kernels_pad.hip", "kernels_st.hip", "batch.hip",
-           "serve.hip", "api.hip"]
+           "serve.hip", "api.hip", "noalloc.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

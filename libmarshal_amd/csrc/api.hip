@@ -335,6 +335,8 @@ uint32_t grid_cap(uint32_t per_cu) { return uint32_t(device_info().cus) * per_cu
 // ---------------------------------------------------------------------------------------
 size_t current_lds_budget() { return lds_budget(); }
 
+void set_last_route(const std::string &route) { t_route = route; }
+
 // Which form a 010 step can use under the current LDS budget.
 M010 choose_010(size_t es, uint64_t A, uint64_t a, uint64_t B, int R, int P,
                 uint64_t base_addr) {

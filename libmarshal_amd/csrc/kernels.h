@@ -268,6 +268,8 @@ bool run_stages(const std::vector<StageRec> &stages, hipStream_t st);
 bool run_stages_on(const std::vector<StageRec> &stages, hipStream_t st, unsigned char *scratch,
                    bool alone = false);
 size_t stages_scratch_bytes(const std::vector<StageRec> &stages);
+// Sets what hip_marshal_last_route reports on this thread.
+void set_last_route(const std::string &route);
 // Appends the row / column decomposition of an M x N transpose of S-byte elements; false
 // when it does not fit the LDS budget.
 bool push_rc(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t M,
