@@ -287,12 +287,20 @@ void hip_marshal_set_profiling(int on);
  * (0/1: measure each new full-transpose shape on its first call) and "batch_solo_kb"
  * (matrices of at least this many KiB whose route is the square or through-scratch one run
  * on the batch's side streams instead of in a grouped kernel; below 1 MiB only in batches
- * of at least 1 GiB in all), "batch_solo_lanes" (how many side streams, 1-8, default 4).
+ * of at least 1 GiB in all), "batch_solo_lanes" (how many side streams, 1-8, default 4),
+ * and for the segmented cycle walk of 100 / 0100 steps (route tag "0100_SEG-": every cycle cut
+ * into segments walked in parallel, one pass, a side buffer of each segment's start)
+ * "seg_len" (slots per segment, default 16), "seg_min_bytes" (super-elements of at least this
+ * many bytes, default 64), "seg_max_slots" (grids of at most this many slots, default 2^24),
+ * "seg_blocks_per_cu", "seg_nt" (bit 0 / 1: non-temporal loads / stores, default 0),
+ * "seg_xcd" (0/1: keep each XCD on a contiguous run of segments, default 1) and "seg_force"
+ * (0/1: take it wherever it can run, default 0).
  * Replaces the reference's hard-coded work-group tables (src/cl_marshal.cc:165-231,
  * 388-465). The setter returns true on an unknown key or a value outside [min, 2^30] (min is
  * 0 for "st_scratch_mb", "st_rotate", "st_wstore", "st_skew", "copy_nt", "oop_nt", "pad_nt",
  * "oop_elem_tile_kb", "oop_pitched", "oop_min_rows", "oop_min_tiles", "copy_nt_store_mb",
- * "gather_row_bytes", "st_tile_rows", "pad_shape" and "autotune", 1 otherwise); the getter
+ * "gather_row_bytes", "st_tile_rows", "pad_shape", "autotune", "seg_nt", "seg_xcd" and
+ * "seg_force", 1 otherwise); the getter
  * returns 0 for an unknown key. */
 bool hip_marshal_set_tuning(const char *key, int64_t value);
 /* Measured autotuner for an M x N transpose of elem_bytes elements: the planner's cheapest

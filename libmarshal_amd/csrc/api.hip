@@ -31,7 +31,12 @@ DeviceInfo g_dev[64];
 std::atomic<size_t> g_lds_budget{0};  // 0 = device maximum
 std::atomic<int> g_profiling{1};
 thread_local std::string t_route;
+thread_local int t_restricted = 0;
 }  // namespace
+
+RestrictedPlanning::RestrictedPlanning() { t_restricted++; }
+RestrictedPlanning::~RestrictedPlanning() { t_restricted--; }
+bool planning_restricted() { return t_restricted > 0; }
 
 const DeviceInfo &device_info() {
   int dev = 0;
@@ -108,6 +113,13 @@ Knob g_knobs[] = {
     {"pl_min_walkers", {16384}},
     {"batch_solo_kb", {64}},
     {"batch_solo_lanes", {4}},
+    {"seg_len", {16}},           // slots per segment of the segmented cycle walk
+    {"seg_min_bytes", {64}},     // super-elements of at least this many bytes take it
+    {"seg_max_slots", {1 << 24}},  // grids of at most this many slots (host-built tables)
+    {"seg_blocks_per_cu", {64}},
+    {"seg_nt", {0}, 0},          // bit 0: non-temporal loads, bit 1: stores, in the walk
+    {"seg_xcd", {1}, 0},         // 1: each XCD takes a contiguous run of walkers
+    {"seg_force", {0}, 0},       // 1: take the segmented walk wherever it can run
 };
 Knob *find_knob(const char *key) {
   if (!key) return nullptr;
@@ -325,7 +337,12 @@ bool aligned_ok(const char *who, const void *p, size_t es) {
   return true;
 }
 
-uint32_t grid_cap(uint32_t per_cu) { return uint32_t(device_info().cus) * per_cu; }
+// Workgroups for a grid-stride launch: CUs x a per-CU knob, computed in 64 bits and clamped
+// to [1, 2^24] (knobs accept up to 2^30, so the product could wrap in 32 bits).
+uint32_t grid_cap(int64_t per_cu) {
+  const uint64_t v = uint64_t(std::max(device_info().cus, 1)) * uint64_t(std::max<int64_t>(per_cu, 1));
+  return uint32_t(std::min<uint64_t>(v, uint64_t(1) << 24));
+}
 
 }  // namespace
 
@@ -426,17 +443,38 @@ void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64
       return;
     }
   }
+  // Long cycles over super-elements of some size: the segmented walk, one pass with a side
+  // buffer of each segment's start piece (worth it while that buffer stays under a quarter of
+  // the data; grids of many short cycles have better forms).
+  StageRec sg = r;
+  sg.kind = StageKind::SEG;
+  bool seg_ok = false;
+  // (buf == nullptr: pricing a candidate route only; the table is not built, its segment
+  // count is estimated.)
+  if (!planning_restricted() && b * es >= uint64_t(tuning("seg_min_bytes")) &&
+      plan_100_segments(inst, R, C, b * es, reinterpret_cast<uintptr_t>(buf),
+                        uint32_t(tuning("seg_len")), uint64_t(tuning("seg_max_slots")),
+                        buf != nullptr, &sg.sg))
+    seg_ok = 4.0 * double(seg_save_bytes(inst, b * es, sg.sg)) <=
+             double(inst) * double(R) * double(C) * double(b * es);
+  StageRec l = r;
+  l.kind = lds_form ? StageKind::PT0100L : StageKind::PT100G;
   if (allow_oop) {
     std::vector<StageRec> o;
     if (push_oop(o, buf, es, inst, R, C, b * es)) {
-      StageRec l = r;
-      l.kind = StageKind::PT0100L;
-      if (!lds_form || stage_bw_tbs(l) < stage_bw_tbs(o[0])) {
+      double inplace = lds_form ? stage_bw_tbs(l) : 0.0;
+      if (seg_ok) inplace = std::max(inplace, tuning("seg_force") ? 1e9 : stage_bw_tbs(sg));
+      if (inplace < stage_bw_tbs(o[0])) {
         out.push_back(o[0]);
         if (tag) *tag = "0100_OOP-";
         return;
       }
     }
+  }
+  if (seg_ok && (stage_bw_tbs(sg) > stage_bw_tbs(l) || tuning("seg_force"))) {
+    if (tag) *tag = "0100_SEG-";
+    out.push_back(sg);
+    return;
   }
   // A single 100 over short super-elements is a full R x C transpose of (b*es)-byte
   // elements: the LDS-staged row / column decomposition beats cycle following there.
@@ -714,6 +752,7 @@ size_t stages_scratch_bytes(const std::vector<StageRec> &stages) {
     if (r.kind == StageKind::PT100C) m = std::max<size_t>(m, size_t(r.pc.cycles + 1) * 4);
     if (r.kind == StageKind::ST) m = std::max<size_t>(m, r.stp.scratch_bytes);
     if (r.kind == StageKind::OOP) m = std::max<size_t>(m, size_t(oop_scratch_bytes(r)));
+    if (r.kind == StageKind::SEG) m = std::max<size_t>(m, seg_save_bytes(r.d0, r.d3 * r.es, r.sg));
   }
   return m;
 }
@@ -729,11 +768,11 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
     switch (r.kind) {
       case StageKind::BS010:
         LM_CHECK_HIP(launch_010_bs(r.buf, r.es, r.d0, r.d1, r.d2, r.bs,
-                                   grid_cap(uint32_t(tuning("bs_blocks_per_cu"))), st));
+                                   grid_cap(tuning("bs_blocks_per_cu")), st));
         break;
       case StageKind::PT010:
         LM_CHECK_HIP(launch_010_pttwac(r.buf, r.es, r.d0, r.d1, r.d2, r.R, r.P,
-                                       grid_cap(uint32_t(tuning("pt010_blocks_per_cu"))), st));
+                                       grid_cap(tuning("pt010_blocks_per_cu")), st));
         break;
       case StageKind::PT0100L:
         LM_CHECK_HIP(launch_0100_lds(r.buf, r.d0, r.d1, r.d2, r.d3 * r.es, r.pt, grid_cap(8),
@@ -821,11 +860,11 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
         }
         if (!buf) return true;
         LM_CHECK_HIP(run_st(r.buf, r.stp, buf, size_t(tuning("st_tile_kb")) << 10,
-                            grid_cap(uint32_t(tuning("pt100_blocks_per_cu"))), st));
+                            grid_cap(tuning("pt100_blocks_per_cu")), st));
         break;
       }
       case StageKind::PT100C: {
-        const uint32_t cap = grid_cap(uint32_t(tuning("pt100_blocks_per_cu")));
+        const uint32_t cap = grid_cap(tuning("pt100_blocks_per_cu"));
         const uint32_t *cached = cached_leaders(r.d1, r.d2, r.pc, st);
         if (cached) {
           LM_CHECK_HIP(launch_leader_walk(r.buf, cached, r.d0, r.d1, r.d2, r.d3 * r.es, r.pt,
@@ -843,6 +882,21 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
                                         cap, st));
         break;
       }
+      case StageKind::SEG: {
+        std::unique_ptr<Lease> lease;
+        unsigned char *save = scr;
+        if (!save) {
+          lease.reset(new Lease(pool_here(), st));
+          save = reinterpret_cast<unsigned char *>(
+              lease->get(seg_save_bytes(r.d0, r.d3 * r.es, r.sg)));
+        }
+        if (!save) return true;
+        LM_CHECK_HIP(launch_100_segments(r.buf, save, r.d0, r.d1, r.d2, r.d3 * r.es, r.sg,
+                                         grid_cap(tuning("seg_blocks_per_cu")),
+                                         uint32_t(tuning("seg_nt")), uint32_t(tuning("seg_xcd")),
+                                         st));
+        break;
+      }
       case StageKind::PT100G: {
         std::unique_ptr<Lease> lease;
         uint32_t *flags = reinterpret_cast<uint32_t *>(scr);
@@ -853,7 +907,7 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
         if (!flags) return true;
         LM_CHECK_HIP(launch_fill_u32(flags, r.pt.flag_words, 0, st));
         LM_CHECK_HIP(launch_100_pttwac(r.buf, flags, r.d0, r.d1, r.d2, r.d3 * r.es, r.pt,
-                                       grid_cap(uint32_t(tuning("pt100_blocks_per_cu"))), st));
+                                       grid_cap(tuning("pt100_blocks_per_cu")), st));
         break;
       }
     }
@@ -945,6 +999,22 @@ double stage_bw_tbs(const StageRec &r) {
       return std::min(3.0, 0.12 + 0.011 * S);
     }
     case StageKind::PT100C: return std::min(5.0, 0.02 * double(r.d3 * es));
+    case StageKind::SEG: {
+      // One pass of S-byte pieces at scattered slots: a piece at an offset aligned to
+      // a = gcd(S, 128) touches (S + 128 - a) / 128 cache lines on average, so the useful share
+      // of the lines moved is S / (S + 128 - a). Measured on MI355X (GB/s): 64 B 3100-3900,
+      // 72 B 2100-2500, 128 B 5050, 256 B 4870, 288 B 3300, 1 KiB 5440, 4 KiB 5700. Narrow
+      // units (pieces aligned to < 8 bytes) cost more instructions per byte. The side buffer
+      // adds a read and a write of every segment start.
+      const uint64_t S = r.d3 * es;
+      uint64_t a = 128;
+      while (a > 1 && S % a) a >>= 1;
+      const double useful = double(S) / double(S + 128 - a);
+      const int ub = r.sg.ub;
+      const double unit = ub >= 8 ? 1.0 : ub == 4 ? 0.6 : ub == 2 ? 0.35 : 0.2;
+      const double extra = 2.0 * double(r.sg.nseg) / double(std::max<uint64_t>(r.d1 * r.d2, 1));
+      return std::min(5.5, 6.5 * useful) * unit / (1.0 + extra);
+    }
     case StageKind::RC_ROW: return r.rc.ub >= 4 ? 3.0 : r.rc.ub == 2 ? 2.0 : 1.2;
     case StageKind::RC_COL1:
     case StageKind::RC_COL2: return r.rc.ub >= 4 ? 0.55 : r.rc.ub == 2 ? 0.4 : 0.3;
@@ -980,6 +1050,7 @@ double stage_bytes(const StageRec &r) {
     case StageKind::PT100G:
     case StageKind::PT100C:
     case StageKind::PT0100L:
+    case StageKind::SEG:
       return double(r.d0) * double(r.d1) * double(r.d2) * double(r.d3) * double(r.es);
     case StageKind::RC_ROW:
     case StageKind::RC_COL1:
@@ -1008,8 +1079,8 @@ bool plan_debug() {
 }
 
 std::mutex g_plan_mu;
-std::map<std::tuple<uint64_t, uint64_t, size_t, size_t, bool>, Choice> g_plan_cache;
-std::map<std::tuple<uint64_t, uint64_t, size_t, size_t, bool>, Choice> g_tuned;  // measured
+std::map<std::tuple<uint64_t, uint64_t, size_t, size_t, bool, bool>, Choice> g_plan_cache;
+std::map<std::tuple<uint64_t, uint64_t, size_t, size_t, bool, bool>, Choice> g_tuned;  // measured
 }  // namespace
 
 // Every candidate route the model considers, cheapest first (one entry per stage code and
@@ -1067,7 +1138,7 @@ Choice plan_full(size_t es, uint64_t M, uint64_t N, bool allow_st) {
   best.b = N;
   best.stages = -1;
   if (M <= 1 || N <= 1) return best;
-  auto key = std::make_tuple(M, N, es, lds_budget(), allow_st);
+  auto key = std::make_tuple(M, N, es, lds_budget(), allow_st, planning_restricted());
   {
     std::lock_guard<std::mutex> lk(g_plan_mu);
     auto it = g_tuned.find(key);
@@ -1173,7 +1244,7 @@ static bool autotune_full(hipStream_t st, unsigned char *buf, size_t es, uint64_
   Choice c = cand[best];
   c.cost = double(best_ms);
   std::lock_guard<std::mutex> lk(g_plan_mu);
-  g_tuned[std::make_tuple(M, N, es, lds_budget(), allow_st)] = c;
+  g_tuned[std::make_tuple(M, N, es, lds_budget(), allow_st, planning_restricted())] = c;
   *out = c;
   return false;
 }
@@ -1261,6 +1332,7 @@ size_t hip_marshal_flag_bytes_0100(int64_t A, int64_t a, int64_t B, int64_t b,
 }
 
 void hip_marshal_finalize(void) {
+  free_segment_tables();
   {
     std::lock_guard<std::mutex> lk(g_lc_mu);
     int cur = 0;
@@ -1355,7 +1427,7 @@ bool hip_transpose_es(hipStream_t st, void *src, size_t es, int64_t A, int64_t a
     bool tuned;
     {
       std::lock_guard<std::mutex> lk(g_plan_mu);
-      tuned = g_tuned.count(std::make_tuple(M, N, es, lds_budget(), true)) != 0;
+      tuned = g_tuned.count(std::make_tuple(M, N, es, lds_budget(), true, planning_restricted())) != 0;
     }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     hipStreamIsCapturing(st, &cs);

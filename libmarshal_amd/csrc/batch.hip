@@ -423,7 +423,12 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
         continue;
       }
     }
-    if (!plan_desc(d, per[i], route, false)) {
+    bool planned;
+    {
+      RestrictedPlanning restricted;  // grouped kernels: no side buffers or device tables
+      planned = plan_desc(d, per[i], route, false);
+    }
+    if (!planned) {
       fprintf(stderr, "[libmarshal] %s: invalid descriptor %lld\n", who, (long long)i);
       return false;
     }
@@ -525,6 +530,7 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
         case StageKind::SQ:
         case StageKind::OOP:
         case StageKind::PT100C:
+        case StageKind::SEG:
           fprintf(stderr, "[libmarshal] %s: stripe route in a batch\n", who);
           return false;
       }

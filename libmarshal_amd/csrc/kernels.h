@@ -213,10 +213,34 @@ hipError_t launch_padding(unsigned char *base, size_t es, uint64_t x, uint64_t y
 // "pt010_blocks_per_cu", "pt100_blocks_per_cu".
 int64_t tuning(const char *key);
 
+// ---- 100 / 0100 by cycle segments: one pass, no bits, no claims -------------------------
+// Every cycle of the grid's 100 map is cut into segments of at most seg_len slots; a first
+// launch saves each segment's start piece to a side buffer, a second walks the segments, each
+// carrying its pieces one slot on. The table of segments (start, length) is built on the host
+// once per (R, C, seg_len) and uploaded once per device (false under stream capture, or when
+// R*C exceeds max_slots).
+struct SegPlan {
+  const uint32_t *table = nullptr;  // device copy: start slot and length of each segment
+  uint64_t nseg = 0;                // segments per instance
+  uint32_t seg_len = 0;
+  int ub = 16, upl = 1;             // unit bytes, units per lane
+  uint32_t vw = 1;                  // lanes per walker (1-64)
+  uint64_t nsub = 1, sub_bytes = 0; // pieces per super-element and their size
+};
+// build = false: no table (for pricing a candidate); the segment count is estimated.
+bool plan_100_segments(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64_t base_addr,
+                       uint32_t seg_len, uint64_t max_slots, bool build, SegPlan *out);
+// Bytes of the side buffer (every segment's start piece, per instance).
+size_t seg_save_bytes(uint64_t inst, uint64_t S, const SegPlan &sp);
+hipError_t launch_100_segments(unsigned char *base, unsigned char *save, uint64_t inst,
+                               uint64_t R, uint64_t C, uint64_t S, const SegPlan &sp,
+                               uint32_t max_blocks, uint32_t nt, uint32_t xcd, hipStream_t st);
+void free_segment_tables();
+
 // ---- stage records (a route as a list of elementary stages) -----------------------------
 enum class M010 { Auto, BS, PTTWAC, Global };
 enum class StageKind : int {
-  BS010, PT010, PT100G, PT0100L, PT100C, RC_ROW, RC_COL1, RC_COL2, ST, SQ, OOP
+  BS010, PT010, PT100G, PT0100L, PT100C, RC_ROW, RC_COL1, RC_COL2, ST, SQ, OOP, SEG
 };
 struct StageRec {
   StageKind kind;
@@ -230,7 +254,16 @@ struct StageRec {
   StPlan stp;               // ST: the whole stripe route (M, N in d1, d2)
   uint64_t chunk;           // OOP: blocks per scratch fill (d0 blocks of d1 x d2, d3 bytes each)
   PlPlan pc;                // PT100C: cycle count and order (pt: the walker shape)
+  SegPlan sg;               // SEG: segment table and walker shape
 };
+// While one of these is alive on a thread, planning on that thread uses only the forms whose
+// scratch is visited bits or nothing (no side buffers, no device tables): the batch's grouped
+// kernels and the no-alloc entry point plan this way.
+struct RestrictedPlanning {
+  RestrictedPlanning();
+  ~RestrictedPlanning();
+};
+bool planning_restricted();
 struct Choice {
   double cost = 1e300;
   uint64_t a = 1, b = 1;

@@ -81,11 +81,13 @@ bool hip_transpose_0100_noalloc_es(hipStream_t st, void *src, size_t es, int64_t
   // (whose cache allocates); the only scratch is the global-bit form's visited bits.
   std::vector<StageRec> list;
   std::string tag;
+  RestrictedPlanning restricted;  // no side buffers or device tables: visited bits only
   if (!plan_step_0100(list, static_cast<unsigned char *>(src), es, uint64_t(A), uint64_t(a),
                       uint64_t(B), uint64_t(b), false, tag))
     return true;
   for (const StageRec &r : list) {
-    if (r.kind == StageKind::OOP || r.kind == StageKind::ST || r.kind == StageKind::PT100C) {
+    if (r.kind == StageKind::OOP || r.kind == StageKind::ST || r.kind == StageKind::PT100C ||
+        r.kind == StageKind::SEG) {
       fprintf(stderr, "[libmarshal] %s: internal error: planned a form that needs a pool\n", who);
       return true;
     }
