@@ -1,15 +1,17 @@
-"""Benchmark: in-place AoS -> ASTA -> AoS on one MI355X.
+"""Benchmark: in-place full transpose of a 98304 x 131071 float64 matrix (103 GB) on one MI355X.
 
-Workload: 2^24 structs of 16 f32 fields (1 GiB), ASTA tile width 64 (one wave64), converted
-AoS -> ASTA (010(2^18, 64, 16)) and back ASTA -> AoS (010(2^18, 16, 64)) by the LDS-staged
-kernel behind hip_aos_asta_es / hip_asta_aos_es. One step = both conversions; inputs are
-resident in HBM before timing starts. Effective bandwidth = 2 x bytes / time per conversion.
+Workload (the largest single-GPU configuration): a 98304 x 131071 row-major float64 matrix,
+U[0,1) values from a seeded generator, resident in HBM, transposed in place by the library's
+planner (hip_transpose_full_es). One step = one full transpose; steps alternate direction
+(98304 x 131071 -> 131071 x 98304 and back), each moving the whole 103 GB. Effective bandwidth
+= 2 x bytes / time per transpose. A million sampled elements are checked after the timed steps.
 
-Printed: one JSON line with the metric, a roofline object for the dominant kernel (its
-launches timed with HIP events on the stream they run on) and a CPU baseline (the oracle's
-multi-threaded restatement of the reference's barrier-sync kernel on a bounded sample of the
-same workload, on this host's cores). Extra measurements of the other forms ride along in
-"extra".
+Printed: one JSON line with the metric, a roofline object for the route's kernels (every
+library launch of the timed steps bracketed by HIP events on the stream it runs on: algorithmic
+bytes per step / summed kernel time per step), the HBM traffic per transpose from the committed
+rocprofv3 counter summary, and a CPU baseline: the reference test's own CPU golden for a full
+transpose (test/cl_unittest.cc:547-551, single-threaded) on the first 768 rows of the same
+matrix. Extra measurements of the other configurations ride along in "extra".
 """
 import argparse
 import json
@@ -22,24 +24,23 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "in-place GB/s (2x bytes/time) and % HBM peak; batched/served p50/p99 latency"
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_010_bs_aos_asta_2e24x16_f32_pmc.json")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "route_st_98304x131071_f64_pmc.json")
+M3A, N3A, ES3A = 98304, 131071, 8
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=100)
-    # The first ~20 round trips after start-up run up to 30 % slower while the GPU ramps up
-    # (measured: 4.6 TB/s on the first, 6.3 TB/s from about the 20th on).
-    p.add_argument("--warmup", type=int, default=30)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=4)
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="approximate CPU work for the baseline sample")
     p.add_argument("--no-extra", action="store_true")
     p.add_argument("--profile-batch", action="store_true",
                    help="only run the 4096-matrix batch (and the same matrices as plain calls)")
     p.add_argument("--profile-st", action="store_true",
-                   help="only a 1 GiB calibration copy and one 98304 x 131071 f64 transpose "
-                        "(for HBM byte counters of the stripe-route kernels)")
+                   help="only a 1 GiB calibration copy and one 98304 x 131071 f64 transpose and "
+                        "its inverse (for HBM byte counters of the route's kernels)")
     p.add_argument("--profile-large", action="store_true",
                    help="only run the two large prime-dimension transposes (for kernel traces)")
     p.add_argument("--profile-soa", type=int, default=0,
@@ -80,25 +81,94 @@ def time_calls(fn, reps, stream, warm=True):
     return e0.elapsed_time(e1) / reps / 1e3  # seconds
 
 
-def cpu_baseline(n_structs, w, t, seconds):
+def cpu_baseline(rows=768):
+    """The reference test's CPU golden for a full transpose (test/cl_unittest.cc:547-551: a 010
+    with unit tiles, then a 100), single-threaded as the test runs it, on a bounded sample of
+    the headline workload: the first `rows` rows of the 98304 x 131071 f64 matrix (a
+    rows x 131071 transpose, same row length, same prime column count). Also the oracle's
+    16-thread restatement of the reference's own device route for such a matrix (the
+    element-granular cycle-following fallback, src/cl_marshal.cc:769-775) on the same sample."""
     import numpy as np
     import oracle
+    x = np.random.default_rng(11).random(rows * N3A)
+    want = x.reshape(rows, N3A).T.copy()
+    t0 = time.perf_counter()
+    oracle.golden_transpose(x, rows, N3A, es=8)
+    dt = time.perf_counter() - t0
+    if not np.array_equal(x.reshape(N3A, rows), want):
+        raise RuntimeError("CPU golden transpose is wrong")
     cores = oracle.num_cpus()
-    # Bounded sample of the same workload: a slice of the structs, converted repeatedly.
-    sample = min(n_structs, 1 << 22)
-    x = oracle.tagged(sample * w, 4, seed=11).view(np.uint32)
+    x = want.reshape(-1).copy()
     t0 = time.perf_counter()
-    oracle.par_010_bs(x, sample // t, t, w, nthreads=cores)
-    one = time.perf_counter() - t0
-    reps = max(1, int(seconds / max(one, 1e-3)))
+    if oracle.par_0100_pttwac(x, 1, N3A, rows, 1, nthreads=cores):
+        raise RuntimeError("CPU fallback restatement failed")
+    dtp = time.perf_counter() - t0
+    nb = rows * N3A * 8
+    base = {"value": round(2 * nb / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"{rows} x {N3A} f64 ({nb / 1e9:.2f} GB: the first {rows} of the "
+                      f"98304 rows), the reference test's CPU golden (010 with unit tiles, then "
+                      f"100; test/cl_unittest.cc:547-551) restated in oracle/oracle.c "
+                      f"orc_golden_transpose, single-threaded, {dt:.1f} s"}
+    par = {"value": round(2 * nb / dtp / 1e9, 3), "unit": "GB/s", "cores": cores,
+           "kind": "port",
+           "sample": f"same {rows} x {N3A} f64 back, by the reference's element-granular "
+                     f"fallback route (cycle following with atomic bit flags) restated in "
+                     f"oracle/oracle.c orc_par_0100_pttwac on {cores} threads, {dtp:.1f} s"}
+    return base, par
+
+
+def headline_3a(lm, steps, warmup):
+    """Full in-place transposes of the 98304 x 131071 f64 matrix, alternating direction; the
+    timed steps run with per-launch kernel timing on."""
+    import torch
+    M, N, es = M3A, N3A, ES3A
+    n = M * N
+    free, _ = torch.cuda.mem_get_info()
+    if free < n * es + (16 << 30):
+        raise RuntimeError(f"needs {n * es / 1e9:.0f} GB of free device memory")
+    x = torch.empty(n, dtype=torch.float64, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(2024)
+    step = 1 << 30
+    for s in range(0, n, step):
+        e = min(n, s + step)
+        x[s:e] = torch.rand(e - s, dtype=torch.float64, device="cuda", generator=g)
+    idx = torch.randint(0, n, (1 << 20,), device="cuda", generator=g)
+    ref = x[idx].clone()
+    shape = [M, N]
+    routes = {}
+    stream = torch.cuda.current_stream()
+
+    def one():
+        rows, cols = shape
+        if lm.transpose_full(x, rows, cols):
+            raise RuntimeError("transpose failed")
+        routes[f"{rows}x{cols}"] = lm.last_route()
+        shape[0], shape[1] = cols, rows
+
+    for _ in range(warmup):
+        one()
+    torch.cuda.synchronize()
+    lm.set_kernel_timing(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(reps):
-        oracle.par_010_bs(x, sample // t, t, w, nthreads=cores)
-    dt = (time.perf_counter() - t0) / reps
-    return {"value": round(2 * x.nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": cores,
-            "kind": "port",
-            "sample": f"AoS->ASTA of {sample} of the 2^24 structs (16 x f32, tile 64), "
-                      f"{reps} repetitions, oracle/oracle.c orc_par_010_bs on {cores} threads"}
+    e0.record(stream)
+    for _ in range(steps):
+        one()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = lm.kernel_times()
+    lm.set_kernel_timing(False)
+    step_s = e0.elapsed_time(e1) / 1e3 / steps
+    r, c = idx // N, idx % N
+    ok = torch.equal(x[idx] if shape == [M, N] else x[c * M + r], ref)
+    del x, idx, ref, r, c
+    torch.cuda.empty_cache()
+    if not ok:
+        raise RuntimeError("sampled elements are not where the transpose puts them")
+    return {"step_s": step_s, "wall_s": wall, "kernels": kt, "routes": routes,
+            "bytes": 2 * n * es, "checked": 1 << 20}
 
 
 def extra_batch(lm, reps=3):
@@ -230,18 +300,36 @@ def extra_large(lm, M, N, es):
 
 
 def cpu_config1():
-    """The reference's 3-stage route restated on the host: 1800 x 7200 int32, a=18, b=72."""
+    """Config 1 on the host: the reference test's CPU golden (single-threaded) and the oracle's
+    restatement of the reference's 3-stage route (100 -> 010 -> 0100) on every host thread,
+    1800 x 7200 int32, a=18, b=72."""
     import numpy as np
     import oracle
     M, N = 1800, 7200
     x = oracle.tagged(M * N, 4, seed=3).view(np.uint32)
+    want = x.reshape(M, N).T.copy()
+    y = x.copy()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        oracle.golden_transpose(y, M, N, es=4)
+        oracle.golden_transpose(y, N, M, es=4)
+    dt1 = (time.perf_counter() - t0) / (2 * reps)
     cores = oracle.num_cpus()
     t0 = time.perf_counter()
     oracle.par_transpose_3(x, M // 18, 18, N // 72, 72, nthreads=cores)
     dt = time.perf_counter() - t0
-    return {"value": round(2 * x.nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": cores,
-            "kind": "port", "sample": "1800 x 7200 int32, a=18 b=72, stages 100 -> 010 -> 0100 "
-                                      "(oracle/oracle.c orc_par_transpose_3)"}
+    if not np.array_equal(x.reshape(N, M), want):
+        raise RuntimeError("CPU 3-stage restatement is wrong")
+    return {"golden_1thread": {"value": round(2 * x.nbytes / dt1 / 1e9, 3), "unit": "GB/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "1800 x 7200 int32 and back, 5 round trips, the "
+                                         "reference test's CPU golden (oracle/oracle.c "
+                                         "orc_golden_transpose)"},
+            "route3": {"value": round(2 * x.nbytes / dt / 1e9, 3), "unit": "GB/s",
+                       "cores": cores, "kind": "port",
+                       "sample": "1800 x 7200 int32, a=18 b=72, stages 100 -> 010 -> 0100 "
+                                 "(oracle/oracle.c orc_par_transpose_3)"}}
 
 
 def main():
@@ -267,8 +355,12 @@ def main():
             torch.rand(min(M * N, s + (1 << 30)) - s, dtype=torch.float64, out=x[s:s + (1 << 30)])
         if lm.transpose_full(x, M, N):
             raise RuntimeError("transpose failed")
+        r1 = lm.last_route()
+        if lm.transpose_full(x, N, M):
+            raise RuntimeError("transpose failed")
         torch.cuda.synchronize()
-        print(f"profile run: 1 GiB copy, {M} x {N} f64 transpose ({lm.last_route()})", flush=True)
+        print(f"profile run: 1 GiB copy, {M} x {N} f64 transpose ({r1}) and back "
+              f"({lm.last_route()})", flush=True)
         return
     if args.profile_batch:
         print(json.dumps({"profile_batch": extra_batch(lm)}), flush=True)
@@ -336,61 +428,65 @@ def main():
               flush=True)
         return
 
-    def step():
-        return lm.aos_asta(d, n, w, t, method=1, elem_bytes=4) or \
-            lm.asta_aos(d, n, w, t, method=1, elem_bytes=4)
-
-    for _ in range(args.warmup):
-        if step():
-            raise RuntimeError("warm-up failed")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        if step():
-            raise RuntimeError("step failed")
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    step_s = e0.elapsed_time(e1) / 1e3 / args.steps
-    if not torch.equal(d.view(torch.int32), ref.view(torch.int32)):
-        raise RuntimeError("round trip is not bit-exact")
-
-    # Dominant kernel: one LDS-staged 010 launch per conversion call, so a step of the timed
-    # region is exactly two launches of it, back to back on `stream`. (Re-timing it after the
-    # check below would start from an idle GPU, which reads up to 5 % slow.)
-    launch_s = step_s / 2
-    achieved = 2 * nbytes / launch_s / 1e9
+    # The headline: the 103 GB matrix, with the device to itself.
+    head = headline_3a(lm, args.steps, args.warmup)
+    step_s = head["step_s"]
+    value = head["bytes"] / step_s / 1e9
+    kern_ms = sum(k["ms"] for k in head["kernels"]) / args.steps
+    achieved = head["bytes"] / (kern_ms / 1e3) / 1e9
     traffic = None
     if os.path.exists(PMC_SUMMARY):
         try:
-            traffic = json.load(open(PMC_SUMMARY)).get("hbm_bytes_per_launch")
+            traffic = json.load(open(PMC_SUMMARY)).get("hbm_bytes_per_transpose")
         except (OSError, ValueError):
             traffic = None
-    value = 2 * 2 * nbytes / step_s / 1e9
+    kernels = [{"name": k["name"], "launches_per_step": round(k["launches"] / args.steps, 2),
+                "us_per_launch": round(1e3 * k["ms"] / max(k["launches"], 1), 2),
+                "ms_per_step": round(k["ms"] / args.steps, 3),
+                "GBs": round(k["bytes"] / (k["ms"] / 1e3) / 1e9, 1) if k["ms"] > 0 else None}
+               for k in head["kernels"]]
 
     extra = {}
     if not args.no_extra:
-        reps = 5
+        # Config 2: AoS -> ASTA -> AoS of 2^24 structs x 16 f32, LDS-staged and cycle
+        # following, and SoA -> ASTA -> SoA. The first ~20 round trips after start-up run up to
+        # 30 % slower while the GPU ramps up, so 30 untimed round trips come first.
+        def bs_round():
+            return lm.aos_asta(d, n, w, t, method=1, elem_bytes=4) or \
+                lm.asta_aos(d, n, w, t, method=1, elem_bytes=4)
+        for _ in range(30):
+            if bs_round():
+                raise RuntimeError("conversion failed")
+        c2 = {}
+        s = time_calls(bs_round, 50, stream, warm=False) / 2
+        c2["aos_asta_bs_GBs"] = round(2 * nbytes / s / 1e9, 1)
+        c2["aos_asta_bs_pct_hbm_peak"] = round(100 * 2 * nbytes / s / 1e9 / HBM_PEAK_GBS, 2)
         s = time_calls(lambda: lm.aos_asta(d, n, w, t, method=2, elem_bytes=4) or
-                       lm.asta_aos(d, n, w, t, method=2, elem_bytes=4), reps, stream) / 2
-        extra["aos_asta_cycle_following_GBs"] = round(2 * nbytes / s / 1e9, 1)
+                       lm.asta_aos(d, n, w, t, method=2, elem_bytes=4), 5, stream) / 2
+        c2["aos_asta_cycle_following_GBs"] = round(2 * nbytes / s / 1e9, 1)
         s = time_calls(lambda: lm.soa_asta(d, n, w, t, elem_bytes=4) or
-                       lm.asta_soa(d, n, w, t, elem_bytes=4), reps, stream) / 2
-        extra["soa_asta_GBs"] = round(2 * nbytes / s / 1e9, 1)
+                       lm.asta_soa(d, n, w, t, elem_bytes=4), 5, stream) / 2
+        c2["soa_asta_GBs"] = round(2 * nbytes / s / 1e9, 1)
+        c2["soa_route"] = lm.last_route()
         if not torch.equal(d.view(torch.int32), ref.view(torch.int32)):
-            raise RuntimeError("cycle-following round trips are not bit-exact")
-        # Full 1800 x 7200 int32 transpose, a = 18, b = 72, 3-stage route (paper shape).
+            raise RuntimeError("config-2 round trips are not bit-exact")
+        extra["aos_soa_asta_2e24x16_f32"] = c2
+        # Config 1: 1800 x 7200 int32, a = 18, b = 72, the reference's 3-stage route, and the
+        # planner's route (~40 us per call: 50 calls so that launch gaps average out).
         M, N = 1800, 7200
         x = torch.randint(-2**31, 2**31 - 1, (M * N,), dtype=torch.int32, device="cuda")
-        s = time_calls(lambda: lm.transpose(x, M // 18, 18, N // 72, 72, 1, 3), reps, stream)
+        x0 = x.clone()
+        tri = lambda: lm.transpose(x, M // 18, 18, N // 72, 72, 1, 3) or \
+            lm.transpose(x, N // 72, 72, M // 18, 18, 1, 3)
+        s = time_calls(tri, 25, stream) / 2
         extra["transpose_1800x7200_i32_3stage_GBs"] = round(2 * M * N * 4 / s / 1e9, 1)
-        # ~45 us per call: 50 calls so that launch gaps and clock noise average out
-        s = time_calls(lambda: lm.transpose_full(x, M, N), 50, stream)
+        s = time_calls(lambda: lm.transpose_full(x, M, N) or lm.transpose_full(x, N, M), 25,
+                       stream) / 2
         extra["transpose_1800x7200_i32_planner_GBs"] = round(2 * M * N * 4 / s / 1e9, 1)
         extra["planner_route"] = lm.last_route()
-        del x
+        if not torch.equal(x, x0):
+            raise RuntimeError("config-1 round trips are not bit-exact")
+        del x, x0
         # the other sizes of the reference's paper run (test/cl_unittest.cc:386-387), float
         sizes = {}
         for (M, N) in ((2500, 5100), (3200, 4000), (3900, 3300), (5100, 2500), (7200, 1800)):
@@ -400,13 +496,9 @@ def main():
             del x
         extra["transpose_paper_sizes_f32_planner"] = sizes
         torch.cuda.empty_cache()
-        extra["cpu_transpose_1800x7200_i32_3stage"] = cpu_config1()
+        extra["cpu_transpose_1800x7200_i32"] = cpu_config1()
         if not args.no_large:
-            del ref, d
-            torch.cuda.empty_cache()
-            extra["transpose_98304x131071_f64"] = extra_large(lm, 98304, 131071, 8)
             extra["transpose_250007x199999_u8"] = extra_large(lm, 250007, 199999, 1)
-            d = ref = None
         extra["batch_4096_mixed"] = extra_batch(lm)
         extra["serve_poisson_2000"] = extra_serve(lm)
         # three times the load for half a second: requests queue, so admission matters
@@ -414,6 +506,8 @@ def main():
     del ref, d
     torch.cuda.empty_cache()
 
+    base, par = cpu_baseline()
+    extra["cpu_3a_sample_fallback_route"] = par
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -425,22 +519,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic: U[-1,1) f32 from a seeded generator, resident in HBM",
-        "config": {"workload": "AoS->ASTA->AoS in place, 2^24 structs x 16 f32 (1 GiB), "
-                               "ASTA width 64, LDS-staged 010 kernel",
-                   "structs": n, "fields": w, "tile": t, "elem_bytes": 4,
-                   "parallelism": "single GPU"},
+        "dtype": "f64",
+        "data": "synthetic: U[0,1) f64 from a seeded generator, resident in HBM",
+        "config": {"workload": "98304 x 131071 f64 full in-place transpose (103 GB), steps "
+                               "alternating direction",
+                   "rows": M3A, "cols": N3A, "elem_bytes": ES3A,
+                   "routes": head["routes"], "parallelism": "single GPU"},
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_010_bs<4,1,256,16,0>",
-                     "launch_us": round(launch_s * 1e6, 2),
-                     "algorithmic_bytes_per_launch": 2 * nbytes},
-        "wall_s": round(wall, 3),
+                     "traffic": traffic,
+                     "kernel": "the route's kernels together: " +
+                               " + ".join(k["name"] for k in head["kernels"]),
+                     "algorithmic_bytes_per_step": head["bytes"],
+                     "kernel_ms_per_step": round(kern_ms, 3),
+                     "kernels": kernels},
+        "wall_s": round(head["wall_s"], 3),
+        "checked_elements": head["checked"],
         "extra": extra,
+        "cpu_baseline": base,
     }
-    out["cpu_baseline"] = cpu_baseline(n, w, t, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

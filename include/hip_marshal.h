@@ -31,7 +31,11 @@
  * (hipStreamWaitEvent) for everything the previous user had enqueued, so two calls on two
  * streams never use the pool at the same time on the GPU - their pool-using stages run one
  * after the other. Under stream capture the cross-stream wait is not recorded, so a
- * captured graph must not run concurrently with other pool users on the same device.
+ * captured graph must not run concurrently with other pool users on the same device; the
+ * pool does not grow under capture (reserve it first), and a pool buffer handed out under
+ * capture is kept, not freed, when the pool later grows. hip_marshal_finalize frees the pool,
+ * the cycle-leader lists and the segment tables: graphs that captured library calls must not
+ * be replayed after it.
  */
 #ifndef LIBMARSHAL_AMD_HIP_MARSHAL_H_
 #define LIBMARSHAL_AMD_HIP_MARSHAL_H_
@@ -208,12 +212,17 @@ void hip_marshal_batch_destroy(hip_marshal_batch *batch);
 /* ---------------------------------------------------------------------------------------
  * Serving: a request loop with continuous batching. No counterpart in the reference.
  * hip_marshal_server_create starts a scheduler thread that owns `streams` HIP streams.
- * Submitted requests convert device-resident buffers in place; they are admitted in arrival
- * order while the bytes of admitted, unfinished requests (buffer + visited bits) fit
- * hbm_budget_bytes, packed into dynamic batches of at most max_batch requests and launched
+ * Submitted requests convert device-resident buffers in place. Each is planned when it is
+ * submitted, its scratch capped by the knob "serve_scratch_mb" (8192 by default; 0 keeps every
+ * request on the in-place routes, whose scratch is visited bits or one stripe). Requests are
+ * admitted in arrival order while the device memory they would hold fits hbm_budget_bytes:
+ * the buffers of admitted, unfinished requests plus a bound on their visited bits, plus every
+ * scratch buffer the server holds (the one shared by the large request that runs at a time,
+ * each stream's own), counting the growth an admission needs. They are packed into dynamic
+ * batches of at most max_batch requests and launched
  * on an idle stream, so requests arriving while others run are admitted as budget and
- * streams free up and run concurrently with them. A request larger than the budget is
- * rejected (-1). The caller keeps each buffer alive and untouched until its request is done
+ * streams free up and run concurrently with them. A request whose buffer and scratch exceed
+ * the budget is rejected (-1). The caller keeps each buffer alive and untouched until its request is done
  * and must not submit overlapping buffers concurrently. Results are bit-identical to the
  * plain entry points run on each buffer alone.
  * ------------------------------------------------------------------------------------- */
@@ -338,6 +347,23 @@ uint64_t hip_marshal_cycle_count(int64_t R, int64_t C, uint32_t max_order, uint3
 /* Bytes of visited-bit flags the given stage needs (0 for LDS-staged stages). */
 size_t hip_marshal_flag_bytes_0100(int64_t A, int64_t a, int64_t B, int64_t b,
                                    size_t elem_bytes);
+/* Per-launch kernel timing. With timing on, every kernel launch of the library is bracketed by
+ * a pair of HIP events on the stream it runs on (nothing is recorded under stream capture).
+ * hip_marshal_kernel_times waits for the recorded launches, writes one entry per kernel family
+ * (in order of first launch, at most max_entries) with the launch count, the summed device
+ * time and the summed algorithmic bytes (2 x the bytes each launch moves into place), clears
+ * the record and returns the number of families. Turning timing on or off also clears it.
+ * No counterpart in the reference (its Profiling class times one kernel per call,
+ * include/cl_profile.h:23-56). */
+typedef struct hip_marshal_kernel_time {
+  char name[48];
+  int64_t launches;
+  double ms;
+  double bytes;
+} hip_marshal_kernel_time;
+void hip_marshal_set_kernel_timing(int on);
+int hip_marshal_kernel_times(hip_marshal_kernel_time *out, int max_entries);
+
 /* Library version string. */
 const char *hip_marshal_version(void);
 

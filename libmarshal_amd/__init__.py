@@ -39,6 +39,12 @@ class ServerStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class KernelTime(ctypes.Structure):
+    """hip_marshal_kernel_time of include/hip_marshal.h."""
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_int64),
+                ("ms", ctypes.c_double), ("bytes", ctypes.c_double)]
+
+
 def lib():
     """Loads the HIP library (raises if it has not been built)."""
     global _lib
@@ -117,6 +123,8 @@ def lib():
         "hip_marshal_server_reset_stats": (None, [vp]),
         "hip_marshal_server_destroy": (None, [vp]),
         "hip_marshal_get_tuning": (i64, [ctypes.c_char_p]),
+        "hip_marshal_set_kernel_timing": (None, [i32]),
+        "hip_marshal_kernel_times": (i32, [ctypes.POINTER(KernelTime), i32]),
         "gpu_aos_asta_bs_float": (b, [vp, i32, i32, i32, vp]),
         "gpu_aos_asta_bs_double": (b, [vp, i32, i32, i32, vp]),
         "gpu_aos_asta_pttwac": (b, [vp, i32, i32, i32, vp]),
@@ -148,7 +156,7 @@ EXPORTED = [
     "hip_marshal_server_now", "hip_marshal_server_status", "hip_marshal_server_wait",
     "hip_marshal_server_drain", "hip_marshal_server_request_times",
     "hip_marshal_server_get_stats", "hip_marshal_server_reset_stats",
-    "hip_marshal_server_destroy",
+    "hip_marshal_server_destroy", "hip_marshal_set_kernel_timing", "hip_marshal_kernel_times",
     "gpu_aos_asta_bs_float", "gpu_aos_asta_bs_double", "gpu_aos_asta_pttwac",
     "gpu_soa_asta_pttwac",
 ]
@@ -340,6 +348,20 @@ def plan_transpose(M, N, elem_bytes):
                                         ctypes.byref(st)):
         raise ValueError("bad arguments")
     return a.value, b.value, st.value
+
+
+def set_kernel_timing(on):
+    """Starts (clearing the record) or stops per-launch timing of the library's kernels."""
+    lib().hip_marshal_set_kernel_timing(1 if on else 0)
+
+
+def kernel_times(max_entries=64):
+    """Per kernel family since timing was turned on: launches, device ms and algorithmic bytes
+    (waits for the recorded launches; clears the record)."""
+    arr = (KernelTime * max_entries)()
+    n = lib().hip_marshal_kernel_times(arr, max_entries)
+    return [{"name": arr[i].name.decode(), "launches": arr[i].launches, "ms": arr[i].ms,
+             "bytes": arr[i].bytes} for i in range(n)]
 
 
 def last_route():

@@ -15,7 +15,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["kernels_010.hip", "kernels_100.hip", "kernels_rc.hip", "kernels_pad.hip", "kernels_st.hip", "kernels_seg.hip", "batch.hip",
-           "serve.hip", "api.hip", "noalloc.hip"]
+           "serve.hip", "api.hip", "noalloc.hip", "ktiming.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -32,7 +32,17 @@ std::atomic<size_t> g_lds_budget{0};  // 0 = device maximum
 std::atomic<int> g_profiling{1};
 thread_local std::string t_route;
 thread_local int t_restricted = 0;
+thread_local uint64_t t_scratch_cap = UINT64_MAX;  // ScratchCapScope override
 }  // namespace
+
+ScratchCapScope::ScratchCapScope(uint64_t bytes) : prev_(t_scratch_cap) {
+  t_scratch_cap = std::min(t_scratch_cap, bytes);
+}
+ScratchCapScope::~ScratchCapScope() { t_scratch_cap = prev_; }
+// Scratch cap of the through-scratch and stripe forms for stages planned on this thread.
+static uint64_t scratch_cap_bytes() {
+  return std::min<uint64_t>(uint64_t(tuning("st_scratch_mb")) << 20, t_scratch_cap);
+}
 
 RestrictedPlanning::RestrictedPlanning() { t_restricted++; }
 RestrictedPlanning::~RestrictedPlanning() { t_restricted--; }
@@ -113,6 +123,7 @@ Knob g_knobs[] = {
     {"pl_min_walkers", {16384}},
     {"batch_solo_kb", {64}},
     {"batch_solo_lanes", {4}},
+    {"serve_scratch_mb", {8192}, 0},  // per-request scratch cap of server requests
     {"seg_len", {16}},           // slots per segment of the segmented cycle walk
     {"seg_min_bytes", {64}},     // super-elements of at least this many bytes take it
     {"seg_max_slots", {1 << 24}},  // grids of at most this many slots (host-built tables)
@@ -120,6 +131,7 @@ Knob g_knobs[] = {
     {"seg_nt", {0}, 0},          // bit 0: non-temporal loads, bit 1: stores, in the walk
     {"seg_xcd", {1}, 0},         // 1: each XCD takes a contiguous run of walkers
     {"seg_force", {0}, 0},       // 1: take the segmented walk wherever it can run
+    {"seg_sync_slots", {1 << 20}},  // larger grids get their tables built in the background
 };
 Knob *find_knob(const char *key) {
   if (!key) return nullptr;
@@ -147,6 +159,10 @@ struct Pool {
   hipStream_t last = nullptr;
   hipEvent_t ev = nullptr;
   bool has_ev = false;
+  // Handed out under stream capture: a captured graph may hold the address, so growth keeps
+  // the old buffer (in `retired`) until hip_marshal_finalize instead of freeing it.
+  bool captured = false;
+  std::vector<void *> retired;
 };
 Pool g_pool[64];
 
@@ -160,7 +176,12 @@ bool pool_grow(Pool &p, size_t bytes) {
   if (p.cap >= bytes) return false;
   if (p.ptr) {
     if (p.has_ev) LM_CHECK_HIP(hipEventSynchronize(p.ev));
-    LM_CHECK_HIP(hipFree(p.ptr));
+    if (p.captured) {
+      p.retired.push_back(p.ptr);
+      p.captured = false;
+    } else {
+      LM_CHECK_HIP(hipFree(p.ptr));
+    }
     p.ptr = nullptr;
     p.cap = 0;
   }
@@ -225,12 +246,18 @@ struct Lease {
   Lease(Pool &pool, hipStream_t s) : p(pool), lk(pool.mu), st(s) {}
   // Returns a pointer to at least `bytes` of flag space, ordered after earlier users.
   uint32_t *get(size_t bytes) {
-    if (pool_grow(p, bytes)) return nullptr;
-    if (p.has_ev && p.last != st) {
-      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-      hipStreamIsCapturing(st, &cs);
-      if (cs == hipStreamCaptureStatusNone) hipStreamWaitEvent(st, p.ev, 0);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hipStreamIsCapturing(st, &cs);
+    if (cs != hipStreamCaptureStatusNone && p.cap < bytes) {
+      // Growing would allocate during capture; reserve first (hip_marshal_reserve).
+      fprintf(stderr, "[libmarshal] scratch pool too small under stream capture (need %zu "
+                      "bytes; call hip_marshal_reserve first)\n", bytes);
+      return nullptr;
     }
+    if (pool_grow(p, bytes)) return nullptr;
+    if (cs != hipStreamCaptureStatusNone) p.captured = true;
+    if (p.has_ev && p.last != st && cs == hipStreamCaptureStatusNone)
+      hipStreamWaitEvent(st, p.ev, 0);
     ok = true;
     return static_cast<uint32_t *>(p.ptr);
   }
@@ -386,8 +413,10 @@ static uint64_t oop_pitch(const StageRec &r) {
   return (row_bytes + 15) & ~uint64_t(15);
 }
 
+// Uses the pitch fixed when the stage was planned (StageRec::pitch), so scratch sized from a
+// plan covers the run even if the knobs change in between.
 static uint64_t oop_scratch_bytes(const StageRec &r) {
-  const uint64_t P = oop_pitch(r);
+  const uint64_t P = r.pitch;
   return P ? r.chunk * r.d2 * P : r.chunk * r.d1 * r.d2 * r.d3;
 }
 
@@ -398,7 +427,7 @@ static uint64_t oop_scratch_bytes(const StageRec &r) {
 static bool push_oop(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t inst,
                      uint64_t R, uint64_t C, uint64_t S) {
   if (inst == 0 || R <= 1 || C <= 1) return false;
-  const uint64_t cap = uint64_t(tuning("st_scratch_mb")) << 20;
+  const uint64_t cap = scratch_cap_bytes();
   const double block = double(R) * double(C) * double(S);
   if (block > double(cap) || !oop_gather_fits(S)) return false;
   StageRec r{};
@@ -410,6 +439,7 @@ static bool push_oop(std::vector<StageRec> &out, unsigned char *buf, size_t es, 
   r.d2 = C;
   r.d3 = S;
   r.chunk = std::max<uint64_t>(1, std::min<uint64_t>(inst, cap / uint64_t(block)));
+  r.pitch = oop_pitch(r);
   out.push_back(r);
   return true;
 }
@@ -454,7 +484,7 @@ void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64
   if (!planning_restricted() && b * es >= uint64_t(tuning("seg_min_bytes")) &&
       plan_100_segments(inst, R, C, b * es, reinterpret_cast<uintptr_t>(buf),
                         uint32_t(tuning("seg_len")), uint64_t(tuning("seg_max_slots")),
-                        buf != nullptr, &sg.sg))
+                        uint64_t(tuning("seg_sync_slots")), buf != nullptr, &sg.sg))
     seg_ok = 4.0 * double(seg_save_bytes(inst, b * es, sg.sg)) <=
              double(inst) * double(R) * double(C) * double(b * es);
   StageRec l = r;
@@ -641,7 +671,7 @@ bool push_st(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t
   if (M <= 1 || N <= 1) return true;
   StageRec r{};
   if (!plan_st(es, M, N, variant, uint64_t(tuning("st_super_bytes")),
-               uint64_t(tuning("st_scratch_mb")) << 20, tuning("st_pt_vw"),
+               scratch_cap_bytes(), tuning("st_pt_vw"),
                tuning("st_rotate") != 0, &r.stp, reinterpret_cast<uintptr_t>(buf)))
     return false;
   r.kind = StageKind::ST;
@@ -795,7 +825,7 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
         // groups of `chunk` blocks: copy to scratch, then out of place back into their place
         // (or, pitched: out of place into scratch, then copied back)
         const uint64_t S = r.d3, block = r.d1 * r.d2 * S;
-        const uint64_t P = oop_pitch(r);
+        const uint64_t P = r.pitch;  // decided when planned: the scratch was sized with it
         std::unique_ptr<Lease> lease;
         unsigned char *tmp = scr;
         if (!tmp) {
@@ -1350,12 +1380,15 @@ void hip_marshal_finalize(void) {
   for (int d = 0; d < 64; d++) {
     Pool &p = g_pool[d];
     std::lock_guard<std::mutex> lk(p.mu);
-    if (!p.ptr && !p.ev) continue;
+    if (!p.ptr && !p.ev && p.retired.empty()) continue;
     int cur = 0;
     hipGetDevice(&cur);
     hipSetDevice(d);
     if (p.has_ev) hipEventSynchronize(p.ev);
     if (p.ptr) hipFree(p.ptr);
+    for (void *q : p.retired) hipFree(q);
+    p.retired.clear();
+    p.captured = false;
     if (p.ev) hipEventDestroy(p.ev);
     p.ptr = nullptr;
     p.cap = 0;

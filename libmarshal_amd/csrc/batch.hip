@@ -407,14 +407,21 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
     if (allow_solo) {
       std::vector<StageRec> list;
       std::string r2;
-      // The stripe route has no grouped form; the square and through-scratch routes beat the
-      // grouped ones from `solo_min` on (smaller matrices gain more from sharing launches).
+      // The stripe route has no grouped form; the square, through-scratch, cycle-leader and
+      // segmented forms beat the grouped ones from `solo_min` on (smaller matrices gain more
+      // from sharing launches).
       const double mbytes = double(d.rows) * double(d.cols) * double(d.elem_bytes);
+      auto has = [&](std::initializer_list<StageKind> ks) {
+        for (const StageRec &r : list)
+          for (StageKind k : ks)
+            if (r.kind == k) return true;
+        return false;
+      };
+      // (stage kinds, not route tags: a composer route's tag does not name its stage forms)
       if (plan_desc(d, list, r2, true) &&
-          (r2.find("ST_") != std::string::npos ||
-           (mbytes >= solo_min &&
-            (r2.find("SQ-") != std::string::npos || r2.find("OOP-") != std::string::npos ||
-             r2.find("CYC-") != std::string::npos)))) {
+          (has({StageKind::ST}) ||
+           (mbytes >= solo_min && has({StageKind::SQ, StageKind::OOP, StageKind::PT100C,
+                                       StageKind::SEG})))) {
         b.solo.push_back(std::move(list));
         b.solo_bytes.push_back(double(d.rows) * double(d.cols) * double(d.elem_bytes));
         b.routes[i] = r2;

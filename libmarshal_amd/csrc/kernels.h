@@ -227,9 +227,12 @@ struct SegPlan {
   uint32_t vw = 1;                  // lanes per walker (1-64)
   uint64_t nsub = 1, sub_bytes = 0; // pieces per super-element and their size
 };
-// build = false: no table (for pricing a candidate); the segment count is estimated.
+// build = false: no table (for pricing a candidate); the segment count is estimated. A grid
+// of more than sync_slots slots whose table does not exist yet gets it built on a background
+// thread, and this call returns false (the caller plans another form).
 bool plan_100_segments(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64_t base_addr,
-                       uint32_t seg_len, uint64_t max_slots, bool build, SegPlan *out);
+                       uint32_t seg_len, uint64_t max_slots, uint64_t sync_slots, bool build,
+                       SegPlan *out);
 // Bytes of the side buffer (every segment's start piece, per instance).
 size_t seg_save_bytes(uint64_t inst, uint64_t S, const SegPlan &sp);
 hipError_t launch_100_segments(unsigned char *base, unsigned char *save, uint64_t inst,
@@ -253,6 +256,7 @@ struct StageRec {
   RcPlan rc;                // RC_*: instances in d0, (M, N) in d1, d2, element bytes in d3
   StPlan stp;               // ST: the whole stripe route (M, N in d1, d2)
   uint64_t chunk;           // OOP: blocks per scratch fill (d0 blocks of d1 x d2, d3 bytes each)
+  uint64_t pitch;           // OOP: scratch row pitch of the pitched form (0: plain form)
   PlPlan pc;                // PT100C: cycle count and order (pt: the walker shape)
   SegPlan sg;               // SEG: segment table and walker shape
 };
@@ -264,6 +268,15 @@ struct RestrictedPlanning {
   ~RestrictedPlanning();
 };
 bool planning_restricted();
+// While one of these is alive on a thread, the through-scratch and stripe forms planned on that
+// thread use at most `bytes` of scratch (below the "st_scratch_mb" knob when that is lower).
+struct ScratchCapScope {
+  explicit ScratchCapScope(uint64_t bytes);
+  ~ScratchCapScope();
+
+ private:
+  uint64_t prev_;
+};
 struct Choice {
   double cost = 1e300;
   uint64_t a = 1, b = 1;
@@ -301,6 +314,20 @@ bool run_stages(const std::vector<StageRec> &stages, hipStream_t st);
 bool run_stages_on(const std::vector<StageRec> &stages, hipStream_t st, unsigned char *scratch,
                    bool alone = false);
 size_t stages_scratch_bytes(const std::vector<StageRec> &stages);
+// Per-launch timing (hip_marshal_set_kernel_timing): a launcher opens one of these around its
+// launch; `bytes` is the launch's algorithmic traffic (2 x the bytes it moves into place).
+class KernelSpan {
+ public:
+  KernelSpan(hipStream_t st, const char *name, double bytes);
+  ~KernelSpan();
+  KernelSpan(const KernelSpan &) = delete;
+  KernelSpan &operator=(const KernelSpan &) = delete;
+
+ private:
+  hipStream_t st_;
+  int64_t idx_ = -1;
+};
+
 // Sets what hip_marshal_last_route reports on this thread.
 void set_last_route(const std::string &route);
 // Appends the row / column decomposition of an M x N transpose of S-byte elements; false

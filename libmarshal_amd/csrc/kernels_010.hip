@@ -222,6 +222,7 @@ hipError_t launch_010_bs(unsigned char *base, size_t es, uint64_t A, uint64_t a,
                          const Bs010Plan &pl, uint32_t max_blocks, hipStream_t st) {
   const Bs010Params p = make_bs010_params(base, A, a, B, pl);
   uint32_t blocks = uint32_t(std::min<uint64_t>(pl.nchunks, max_blocks));
+  KernelSpan span(st, "k_010_bs", 2.0 * double(A) * double(a) * double(B) * double(es));
   switch (es) {
     case 1: return pl.G == 4 ? launch_bs_g<1, 4>(p, pl, blocks, st)
                              : launch_bs_g<1, 1>(p, pl, blocks, st);
@@ -243,6 +244,7 @@ uint64_t pt010_flag_words(uint64_t a, uint64_t B, int R, int P) {
 hipError_t launch_010_pttwac(unsigned char *base, size_t es, uint64_t A, uint64_t a,
                              uint64_t B, int R, int P, uint32_t max_blocks, hipStream_t st) {
   constexpr int NT = 256;
+  KernelSpan span(st, "k_010_pttwac", 2.0 * double(A) * double(a) * double(B) * double(es));
   Pt010Params p{};
   p.base = base;
   p.A = A;

@@ -68,6 +68,7 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t words, uint32_t value) {
 hipError_t launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st) {
   if (words == 0) return hipSuccess;
   uint64_t blocks = std::min<uint64_t>((words + 255) / 256, uint64_t(device_info().cus) * 16);
+  KernelSpan span(st, "k_fill_u32", 0.0);
   hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t(blocks)), dim3(256), 0, st, p, words, value);
   return hipGetLastError();
 }
@@ -196,6 +197,7 @@ hipError_t launch_0100_lds(unsigned char *base, uint64_t inst, uint64_t R, uint6
   p.div_nsub = FastDiv64(pl.nsub);
   p.div_C32 = FastDiv32(uint32_t(C));
   size_t lds = lds0100_bytes(R, C, pl);
+  KernelSpan span(st, "k_0100_lds", 2.0 * double(inst) * double(R) * double(C) * double(S));
   // Few instances: make each workgroup wide so that the machine still has enough walkers.
   int nt = inst < uint64_t(device_info().cus) * 2 ? 1024 : 256;
   uint32_t blocks = uint32_t(std::min<uint64_t>(inst, max_blocks));
@@ -229,9 +231,11 @@ hipError_t launch_100_pttwac(unsigned char *base, uint32_t *flags, uint64_t inst
   p.div_RC = FastDiv64(p.RC);
   p.div_nsub = FastDiv64(pl.nsub);
   p.div_C32 = FastDiv32(uint32_t(std::min<uint64_t>(C, 0xffffffffu)));
+  if (p.walkers == 0) return hipSuccess;
   uint64_t per_block = 4ull * (64u >> pl.vw_log2);
   uint64_t blocks = std::min<uint64_t>((p.walkers + per_block - 1) / per_block, max_blocks);
-  if (blocks == 0) return hipSuccess;
+  if (blocks == 0) return hipErrorInvalidValue;  // walkers but no workgroups: never skip work
+  KernelSpan span(st, "k_100_pttwac", 2.0 * double(inst) * double(R) * double(C) * double(S));
   switch (pl.ub) {
     case 1: return launch_100_ub<1>(p, pl.upl, uint32_t(blocks), st);
     case 2: return launch_100_ub<2>(p, pl.upl, uint32_t(blocks), st);
@@ -468,6 +472,7 @@ hipError_t launch_find_leaders(uint32_t *list, uint64_t R, uint64_t C, const PlP
   if (e != hipSuccess) return e;
   const uint64_t chunks = (uint64_t(p.m) + kLeadChunk - 1) / kLeadChunk;
   const uint32_t fblocks = uint32_t(std::min<uint64_t>(chunks, uint64_t(device_info().cus) * 8));
+  KernelSpan span(st, "k_find_leaders", 0.0);
   hipLaunchKernelGGL(k_find_leaders, dim3(fblocks), dim3(256), 0, st, p);
   return hipGetLastError();
 }
@@ -481,6 +486,8 @@ hipError_t launch_leader_walk(unsigned char *base, const uint32_t *list, uint64_
   const uint64_t per_block = 4ull * (64u >> pl.vw_log2);
   const uint32_t blocks =
       uint32_t(std::min<uint64_t>((p.walkers + per_block - 1) / per_block, max_blocks));
+  if (blocks == 0) return hipErrorInvalidValue;
+  KernelSpan span(st, "k_100_leaders", 2.0 * double(inst) * double(R) * double(C) * double(S));
   switch (pl.ub) {
     case 1: return launch_pl_ub<1>(p, pl.upl, blocks, st);
     case 2: return launch_pl_ub<2>(p, pl.upl, blocks, st);

@@ -296,6 +296,7 @@ hipError_t launch_padding(unsigned char *base, size_t es, uint64_t x, uint64_t y
   if (p.ntiles == 0) return hipSuccess;
   hipError_t e = launch_fill_u32(flags, pad_flag_words(p.total, es), 0, st);
   if (e != hipSuccess) return e;
+  KernelSpan span(st, unpad ? "k_unpad" : "k_pad", 2.0 * double(x) * double(y) * double(es));
   if ((reinterpret_cast<uintptr_t>(base) & 15u) == 0) {
     const uint64_t nvec = (p.total * es + 15) / 16;
     // lanes x vectors per lane: 0: 512 x 4 (default), 1: 256 x 4, 2: 256 x 8,

@@ -151,6 +151,8 @@ hipError_t launch_rc_ub(const RcParams &p, const RcPlan &pl, int pass, uint32_t 
 hipError_t launch_rc(unsigned char *base, uint64_t M, uint64_t N, const RcPlan &pl, int pass,
                      uint32_t max_blocks, hipStream_t st) {
   RcParams p = make_rc(base, M, N, pl);
+  KernelSpan span(st, pass == 0 ? "k_rc_row" : "k_rc_col",
+                  2.0 * double(pl.inst) * double(M) * double(N) * double(pl.m) * double(pl.ub));
   switch (pl.ub) {
     case 1: return launch_rc_ub<1>(p, pl, pass, max_blocks, st);
     case 2: return launch_rc_ub<2>(p, pl, pass, max_blocks, st);

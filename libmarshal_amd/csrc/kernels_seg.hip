@@ -29,6 +29,8 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -219,6 +221,17 @@ std::mutex g_seg_mu;
 std::map<std::tuple<uint64_t, uint64_t, uint32_t>, std::shared_ptr<SegTable>> g_seg;
 size_t g_seg_bytes = 0;
 constexpr size_t kSegCacheBytes = size_t(256) << 20;
+// Tables of grids above the synchronous limit are built on background threads: the call that
+// first meets such a grid plans another form instead of waiting for the walk.
+std::set<std::tuple<uint64_t, uint64_t, uint32_t>> g_building;
+struct Builders {
+  std::vector<std::thread> th;
+  ~Builders() {
+    for (std::thread &t : th)
+      if (t.joinable()) t.join();
+  }
+};
+Builders g_builders;
 
 std::shared_ptr<SegTable> build_table(uint64_t R, uint64_t C, uint32_t L) {
   auto tb = std::make_shared<SegTable>();
@@ -269,8 +282,22 @@ static void seg_shape(uint64_t S, uint64_t base_addr, SegPlan &sp) {
   }
 }
 
+static std::shared_ptr<SegTable> insert_table(std::tuple<uint64_t, uint64_t, uint32_t> key,
+                                              std::shared_ptr<SegTable> tb) {
+  const size_t bytes = tb->host.size() * 4;
+  std::lock_guard<std::mutex> lk(g_seg_mu);
+  g_building.erase(key);
+  auto it = g_seg.find(key);
+  if (it != g_seg.end()) return it->second;
+  if (g_seg_bytes + 2 * bytes > kSegCacheBytes) return nullptr;
+  g_seg[key] = tb;
+  g_seg_bytes += 2 * bytes;
+  return tb;
+}
+
 bool plan_100_segments(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64_t base_addr,
-                       uint32_t seg_len, uint64_t max_slots, bool build, SegPlan *out) {
+                       uint32_t seg_len, uint64_t max_slots, uint64_t sync_slots, bool build,
+                       SegPlan *out) {
   if (inst == 0 || R < 2 || C < 2 || S == 0 || seg_len < 2 || seg_len > 4096) return false;
   const uint64_t RC = R * C;
   if (RC > max_slots || RC >= (uint64_t(1) << 31)) return false;
@@ -291,18 +318,15 @@ bool plan_100_segments(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64
     if (it != g_seg.end()) tb = it->second;
   }
   if (!tb) {
-    tb = build_table(R, C, seg_len);
-    const size_t bytes = tb->host.size() * 4;
-    std::lock_guard<std::mutex> lk(g_seg_mu);
-    auto key = std::make_tuple(R, C, seg_len);
-    auto it = g_seg.find(key);
-    if (it != g_seg.end()) {
-      tb = it->second;
-    } else {
-      if (g_seg_bytes + 2 * bytes > kSegCacheBytes) return false;
-      g_seg[key] = tb;
-      g_seg_bytes += 2 * bytes;
+    const auto key = std::make_tuple(R, C, seg_len);
+    if (RC > sync_slots) {  // walk it in the background; plan another form this time
+      std::lock_guard<std::mutex> lk(g_seg_mu);
+      if (g_building.insert(key).second)
+        g_builders.th.emplace_back([key] { insert_table(key, build_table(std::get<0>(key), std::get<1>(key), std::get<2>(key))); });
+      return false;
     }
+    tb = insert_table(key, build_table(R, C, seg_len));
+    if (!tb) return false;
   }
   if (tb->nseg == 0) return false;
   {
@@ -337,6 +361,13 @@ size_t seg_save_bytes(uint64_t inst, uint64_t S, const SegPlan &sp) {
 }
 
 void free_segment_tables() {
+  std::vector<std::thread> th;
+  {
+    std::lock_guard<std::mutex> lk(g_seg_mu);
+    th.swap(g_builders.th);
+  }
+  for (std::thread &t : th)
+    if (t.joinable()) t.join();
   std::lock_guard<std::mutex> lk(g_seg_mu);
   int cur = 0;
   hipGetDevice(&cur);
@@ -357,9 +388,13 @@ namespace {
 template <int UB, int UPL>
 hipError_t launch_seg_t(const SegParams &p, uint32_t blocks, hipStream_t st) {
   constexpr int W = UPL == 1 ? 16 : 8;
-  hipLaunchKernelGGL((k_seg_save<UB, UPL>), dim3(blocks), dim3(256), 0, st, p);
+  {
+    KernelSpan span(st, "k_seg_save", 0.0);
+    hipLaunchKernelGGL((k_seg_save<UB, UPL>), dim3(blocks), dim3(256), 0, st, p);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  KernelSpan span(st, "k_seg_walk", 2.0 * double(p.walkers / p.nsub / p.nseg) * double(p.RC) * double(p.S));
   switch (p.nt & 3u) {
     case 0: hipLaunchKernelGGL((k_seg_walk<UB, UPL, W, false, false>), dim3(blocks), dim3(256), 0, st, p); break;
     case 1: hipLaunchKernelGGL((k_seg_walk<UB, UPL, W, true, false>), dim3(blocks), dim3(256), 0, st, p); break;

@@ -467,6 +467,7 @@ __global__ __launch_bounds__(NT) void k_sq_swap(unsigned char *base, uint64_t n)
 
 hipError_t launch_sq_transpose(unsigned char *base, size_t es, uint64_t n, hipStream_t st) {
   if (n < 2) return hipSuccess;
+  KernelSpan span(st, "k_sq_swap", 2.0 * double(n) * double(n) * double(es));
   auto go = [&](auto tag, auto vec) -> hipError_t {
     constexpr int E = decltype(tag)::value;
     constexpr bool V = decltype(vec)::value;
@@ -592,6 +593,7 @@ __global__ __launch_bounds__(256) void k_copy_any(unsigned char *dst, const unsi
 hipError_t launch_copy(unsigned char *dst, const unsigned char *src, uint64_t bytes,
                        hipStream_t st) {
   if (bytes == 0) return hipSuccess;
+  KernelSpan span(st, "k_copy", 2.0 * double(bytes));
   const uint32_t cap = uint32_t(device_info().cus) * 16;
   const uint32_t cap16 = uint32_t(device_info().cus) * uint32_t(tuning("copy_blocks_per_cu"));
   const uint64_t per_block = 256 * kCopyUnroll;
@@ -718,6 +720,7 @@ hipError_t launch_unpitch(unsigned char *dst, const unsigned char *src, uint64_t
   if (total == 0) return hipSuccess;
   if (RB < 16 || P < RB || (P & 15u) || (reinterpret_cast<uintptr_t>(src) & 15u))
     return hipErrorInvalidValue;
+  KernelSpan span(st, "k_unpitch", 2.0 * double(total));
   const uint32_t head =
       uint32_t(std::min<uint64_t>((16u - (reinterpret_cast<uintptr_t>(dst) & 15u)) & 15u,
                                   total));
@@ -740,6 +743,7 @@ hipError_t launch_copy2d(unsigned char *dst, uint64_t dpitch, const unsigned cha
                          uint64_t spitch, uint64_t width, uint64_t rows, hipStream_t st) {
   if (width == 0 || rows == 0) return hipSuccess;
   const uint64_t n = width * rows;
+  KernelSpan span(st, "k_copy2d", 2.0 * double(n));
   const uint32_t blocks =
       uint32_t(std::min<uint64_t>((n + 255) / 256, uint64_t(device_info().cus) * 16));
   hipLaunchKernelGGL(k_copy2d, dim3(blocks), dim3(256), 0, st, dst, dpitch, src, spitch, width,
@@ -797,6 +801,7 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
   if (blocks == 0 || R == 0 || C == 0) return hipSuccess;
   if (dst_pitch == 0) dst_pitch = R * es;
   if (dst_pitch < R * es) return hipErrorInvalidValue;
+  KernelSpan span(st, "k_oop_transpose", 2.0 * double(blocks) * double(R) * double(C) * double(es));
   auto al = [](uint64_t v) { return (v & 15u) == 0; };
   const bool dst_al = al(reinterpret_cast<uintptr_t>(dst)) && al(dst_pitch) && al(dst_stride);
   const int64_t skew_knob = tuning("st_skew");
@@ -1069,6 +1074,7 @@ hipError_t launch_oop_gather(const unsigned char *src, unsigned char *dst, uint6
                              hipStream_t st) {
   if (inst == 0 || R == 0 || C == 0) return hipSuccess;
   if ((reinterpret_cast<uintptr_t>(src) & 15u) || !oop_gather_fits(S)) return hipErrorInvalidValue;
+  KernelSpan span(st, "k_oop_gather", 2.0 * double(inst) * double(R) * double(C) * double(S));
   GatherParams p{};
   p.src = src;
   p.dst = dst;

@@ -1,9 +1,12 @@
 // serve.hip - a request loop with continuous batching for in-place layout conversions.
 //
 // Clients submit conversions of device-resident buffers. Requests are admitted in arrival
-// order while the bytes of admitted, unfinished requests stay within the HBM budget (a
-// request's footprint is its buffer plus visited bits; the scratch of the through-scratch
-// and stripe forms is per stream, at most the scratch cap each). Small requests admitted
+// order while the device memory they hold stays within the HBM budget: the buffers of admitted,
+// unfinished requests (plus a bound on their visited bits) and every scratch allocation the
+// server holds (the one shared scratch of the large request that runs at a time, each stream's
+// scratch, visited bits and work lists), counting what an admission would have to grow. Each
+// request is planned once, at submission, with its scratch capped by the "serve_scratch_mb"
+// knob, so its exact scratch need is known before it is admitted. Small requests admitted
 // together form one dynamic batch of grouped persistent kernels (the batched form's work
 // lists) on an idle stream; a small request admitted alone, and every request of 64 MiB or
 // more, runs with its plain stage list and the stream's own scratch. Large requests run one
@@ -17,6 +20,7 @@
 #include "batch.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -38,7 +42,9 @@ double now_s() {
 
 struct Req {
   hip_marshal_desc d;
-  size_t bytes = 0;
+  size_t bytes = 0;    // buffer + a bound on its visited bits
+  size_t scratch = 0;  // scratch of its plain stage list
+  std::vector<StageRec> plain;  // planned at submission
   double t_arrive = 0, t_launch = 0, t_done = 0;
   int state = 0;  // 0 queued, 1 running, 2 done, -1 failed
 };
@@ -67,7 +73,12 @@ struct Server {
   size_t budget = 0;
   int max_batch = 256;
   size_t solo_bytes = size_t(64) << 20;  // requests this large run alone, on the plain path
+  uint64_t scratch_cap = 0;              // per-request scratch cap of the plain forms
   int device = 0;
+  // scratch of the large request that runs at a time (one at most), shared by all streams
+  unsigned char *solo_scr = nullptr;
+  size_t solo_cap = 0;
+  std::atomic<size_t> held{0};  // device bytes of every server allocation (budget accounting)
   std::mutex mu;
   std::condition_variable cv;
   std::deque<int64_t> queue;
@@ -92,28 +103,39 @@ size_t footprint(const hip_marshal_desc &d) {
   return data + data / 32 + 4096;  // buffer + an upper bound on its visited bits
 }
 
-bool grow(void **p, size_t *cap, size_t want, bool pinned) {
+// Grows a server-owned buffer to at least `want` bytes (a quarter more, at least 1 MiB, for
+// small ones; device buffers of 64 MiB and more exactly, so large scratch holds no slack);
+// `held` tracks the device bytes the server holds.
+bool grow(void **p, size_t *cap, size_t want, bool pinned, std::atomic<size_t> *held) {
   if (*cap >= want) return true;
-  size_t n = std::max<size_t>(want + want / 2, size_t(1) << 20);
+  size_t n = want >= (size_t(64) << 20) && !pinned ? want
+                                                   : std::max<size_t>(want + want / 4, size_t(1) << 20);
   if (*p) {
     if (pinned) hipHostFree(*p);
     else hipFree(*p);
+    if (!pinned && held) *held -= *cap;
     *p = nullptr;
     *cap = 0;
   }
   hipError_t e = pinned ? hipHostMalloc(p, n, hipHostMallocDefault) : hipMalloc(p, n);
   if (e != hipSuccess) return false;
   *cap = n;
+  if (!pinned && held) *held += n;
   return true;
 }
+
+// Growth of a server buffer of capacity `cap` to hold `want` bytes (0 when it fits).
+size_t growth(size_t cap, size_t want) { return want > cap ? want : 0; }
 
 // Plans and launches one dynamic batch on an idle slot. Called without the lock held.
 bool launch_batch(Server &s, Slot &sl, const std::vector<hip_marshal_desc> &descs) {
   BatchPlan plan;
   if (!plan_batch(descs.data(), int64_t(descs.size()), plan, "hip_marshal_server")) return false;
-  if (!grow(reinterpret_cast<void **>(&sl.flags), &sl.flags_cap, plan.flag_words * 4 + 4, false) ||
-      !grow(&sl.params, &sl.params_cap, plan.param_bytes + 256, false) ||
-      !grow(reinterpret_cast<void **>(&sl.host), &sl.host_cap, plan.param_bytes + 256, true))
+  if (!grow(reinterpret_cast<void **>(&sl.flags), &sl.flags_cap, plan.flag_words * 4 + 4, false,
+            &s.held) ||
+      !grow(&sl.params, &sl.params_cap, plan.param_bytes + 256, false, &s.held) ||
+      !grow(reinterpret_cast<void **>(&sl.host), &sl.host_cap, plan.param_bytes + 256, true,
+            nullptr))
     return false;
   std::vector<unsigned char> img;
   build_image(plan, sl.flags, img);
@@ -130,14 +152,19 @@ bool launch_batch(Server &s, Slot &sl, const std::vector<hip_marshal_desc> &desc
 // A large (or lone) request runs with the stage list its plain entry point would use (stripe
 // route included), on the slot's stream with the slot's own scratch: nothing to amortise,
 // and per-request kernels beat the grouped forms there.
-bool launch_solo(Server &s, Slot &sl, const hip_marshal_desc &d, bool alone) {
-  (void)s;
-  std::vector<StageRec> list;
-  std::string route;
-  if (!plan_desc(d, list, route, true)) return false;
-  size_t need = stages_scratch_bytes(list);
-  if (need && !grow(reinterpret_cast<void **>(&sl.scr), &sl.scr_cap, need, false)) return false;
-  if (run_stages_on(list, sl.st, sl.scr, alone)) return false;
+bool launch_solo(Server &s, Slot &sl, const std::vector<StageRec> &plain, size_t need,
+                 bool large, bool alone) {
+  unsigned char *scr = nullptr;
+  if (large) {  // one large request at a time: the shared scratch
+    if (need && !grow(reinterpret_cast<void **>(&s.solo_scr), &s.solo_cap, need, false, &s.held))
+      return false;
+    scr = s.solo_scr;
+  } else {
+    if (need && !grow(reinterpret_cast<void **>(&sl.scr), &sl.scr_cap, need, false, &s.held))
+      return false;
+    scr = sl.scr;
+  }
+  if (run_stages_on(plain, sl.st, scr, alone)) return false;
   return hipEventRecord(sl.ev, sl.st) == hipSuccess;
 }
 
@@ -162,13 +189,21 @@ void start_on_slot(Server &s, Slot &sl, const std::vector<int64_t> &ids,
   bool alone = true;  // no other stream busy: the request has the device to itself
   for (const Slot &o : s.slots)
     if (&o != &sl && o.busy) alone = false;
+  // copied while the lock is held: the request map may change once it is released
+  std::vector<StageRec> plain_list;
+  size_t plain_scratch = 0;
+  if (plain) {
+    plain_list = s.reqs[ids[0]].plain;
+    plain_scratch = s.reqs[ids[0]].scratch;
+  }
   s.batches++;
   s.max_in_batch = std::max<int64_t>(s.max_in_batch, int64_t(ids.size()));
   lk.unlock();
   int prev = -1;
   hipGetDevice(&prev);
   if (prev != s.device) hipSetDevice(s.device);
-  const bool ok = plain ? launch_solo(s, sl, descs[0], alone) : launch_batch(s, sl, descs);
+  const bool ok = plain ? launch_solo(s, sl, plain_list, plain_scratch, solo, alone)
+                        : launch_batch(s, sl, descs);
   if (prev != s.device && prev >= 0) hipSetDevice(prev);
   lk.lock();
   sl.launching = false;
@@ -220,6 +255,28 @@ bool retire(Server &s) {
   return progressed;
 }
 
+// Frees the buffers of idle streams (and the shared large-request scratch when no large
+// request runs), returning their bytes to the budget. Lock held; called only when nothing runs.
+void release_idle(Server &s) {
+  for (Slot &sl : s.slots) {
+    if (sl.busy) continue;
+    if (sl.flags) hipFree(sl.flags);
+    if (sl.params) hipFree(sl.params);
+    if (sl.scr) hipFree(sl.scr);
+    s.held -= sl.flags_cap + sl.params_cap + sl.scr_cap;
+    sl.flags = nullptr;
+    sl.params = nullptr;
+    sl.scr = nullptr;
+    sl.flags_cap = sl.params_cap = sl.scr_cap = 0;
+  }
+  if (s.solo_running == 0 && s.solo_scr) {
+    hipFree(s.solo_scr);
+    s.held -= s.solo_cap;
+    s.solo_scr = nullptr;
+    s.solo_cap = 0;
+  }
+}
+
 // Admits queued requests onto idle slots while the budget allows. Lock held (released while
 // launching). Order: the queue head if it can start; a large request (>= solo_bytes) waits
 // while another large one runs, and small requests behind it may then go ahead of it, so
@@ -235,13 +292,26 @@ void admit(Server &s, std::unique_lock<std::mutex> &lk) {
         break;
       }
     if (!free_slot) return;
+    bool idle = s.inflight == 0;
+    for (const Slot &sl : s.slots) idle = idle && !sl.busy;
     std::vector<int64_t> ids;
     std::vector<hip_marshal_desc> descs;
     size_t add = 0;
     bool solo = false;
     const Req &head = s.reqs[s.queue.front()];
+    size_t held = s.held.load();
+    if (idle) {
+      // Nothing runs: the head request always fits alone (checked at submission), so make
+      // room for it by freeing idle buffers if the server's own allocations are in the way.
+      if (head.bytes + head.scratch + held > s.budget) {
+        release_idle(s);
+        held = s.held.load();
+      }
+    }
     if (head.bytes >= s.solo_bytes && s.solo_running == 0) {
-      if (s.inflight + head.bytes > s.budget) return;  // wait for budget to free up
+      // its buffer, and the shared scratch grown to what it needs
+      if (!idle && s.inflight + head.bytes + held + growth(s.solo_cap, head.scratch) > s.budget)
+        return;  // wait for budget to free up (an idle server always starts the head)
       solo = true;
       ids.push_back(s.queue.front());
       descs.push_back(head.d);
@@ -254,7 +324,10 @@ void admit(Server &s, std::unique_lock<std::mutex> &lk) {
           ++it;
           continue;
         }
-        if (s.inflight + add + r.bytes > s.budget) break;
+        // (a request taken alone runs on its plain list with the stream's own scratch)
+        const size_t extra = ids.empty() ? growth(free_slot->scr_cap, r.scratch) : 0;
+        if (s.inflight + add + r.bytes + held + extra > s.budget && !(idle && ids.empty()))
+          break;
         add += r.bytes;
         ids.push_back(*it);
         descs.push_back(r.d);
@@ -309,6 +382,7 @@ hip_marshal_server *hip_marshal_server_create(size_t hbm_budget_bytes, int strea
   Server &s = h->s;
   s.budget = hbm_budget_bytes;
   s.max_batch = max_batch;
+  s.scratch_cap = uint64_t(tuning("serve_scratch_mb")) << 20;
   hipGetDevice(&s.device);
   s.slots.resize(size_t(streams));
   for (Slot &sl : s.slots) {
@@ -337,7 +411,17 @@ int64_t hip_marshal_server_submit_at(hip_marshal_server *h, const hip_marshal_de
             reinterpret_cast<uintptr_t>(d->ptr) % std::min<size_t>(es, 16) == 0;
   if (ok && d->op != HIP_MARSHAL_OP_TRANSPOSE) ok = d->tile >= 1 && d->rows % d->tile == 0;
   size_t fp = ok ? footprint(*d) : 0;
-  if (!ok || fp > s.budget) {
+  // The plain stage list, planned here once: in-place and through-scratch forms with scratch
+  // capped per request, no segment tables (their first upload would stall the other streams).
+  std::vector<StageRec> plain;
+  if (ok) {
+    RestrictedPlanning no_tables;
+    ScratchCapScope cap(s.scratch_cap);
+    std::string route;
+    ok = plan_desc(*d, plain, route, true);
+  }
+  const size_t scr = ok ? stages_scratch_bytes(plain) : 0;
+  if (!ok || fp + scr > s.budget) {
     fprintf(stderr, "[libmarshal] hip_marshal_server_submit: invalid request or larger than "
                     "the HBM budget\n");
     return -1;
@@ -347,6 +431,8 @@ int64_t hip_marshal_server_submit_at(hip_marshal_server *h, const hip_marshal_de
   Req &r = s.reqs[id];
   r.d = *d;
   r.bytes = fp;
+  r.scratch = scr;
+  r.plain = std::move(plain);
   r.t_arrive = arrival > 0 ? arrival : now_s();
   if (s.t_first < 0 || r.t_arrive < s.t_first) s.t_first = r.t_arrive;
   s.queue.push_back(id);
@@ -453,12 +539,16 @@ void hip_marshal_server_destroy(hip_marshal_server *h) {
     s.cv.notify_all();
   }
   if (s.th.joinable()) s.th.join();
+  if (s.solo_scr) hipFree(s.solo_scr);
   for (Slot &sl : s.slots) {
     if (sl.st) hipStreamSynchronize(sl.st);
     if (sl.flags) hipFree(sl.flags);
     if (sl.scr) hipFree(sl.scr);
     if (sl.params) hipFree(sl.params);
     if (sl.host) hipHostFree(sl.host);
+    sl.flags = nullptr;
+    sl.scr = nullptr;
+    sl.params = nullptr;
     if (sl.ev) hipEventDestroy(sl.ev);
     if (sl.st) hipStreamDestroy(sl.st);
   }

@@ -39,6 +39,8 @@ def lib():
             getattr(L, f).restype = None
             getattr(L, f).argtypes = [vp, vp, sz] + [i64] * n
         L.orc_transpose.restype = None; L.orc_transpose.argtypes = [vp, vp, sz, i64, i64]
+        L.orc_golden_transpose.restype = None
+        L.orc_golden_transpose.argtypes = [vp, vp, sz, i64, i64]
         L.orc_inplace_010_bs.restype = i32; L.orc_inplace_010_bs.argtypes = [vp, sz, i64, i64, i64]
         L.orc_inplace_010_pttwac.restype = i32
         L.orc_inplace_010_pttwac.argtypes = [vp, sz, i64, i64, i64, i32, i32]
@@ -169,6 +171,14 @@ def choose_tiles(h, w, min_limit, max_limit):
     vals = [ctypes.c_int64() for _ in range(4)]
     lib().orc_choose_tiles(h, w, min_limit, max_limit, *[ctypes.byref(v) for v in vals])
     return tuple(v.value for v in vals)
+
+
+def golden_transpose(x, M, N, es=None):
+    """In place, single-threaded: the reference test's two-call CPU golden (010 with unit tiles,
+    then 100; test/cl_unittest.cc:547-551) through a temporary of the same size."""
+    es = es or x.itemsize
+    tmp = np.empty_like(x)
+    lib().orc_golden_transpose(_p(x), _p(tmp), es, M, N)
 
 
 def par_010_bs(x, A, a, B, nthreads=0, es=None):

@@ -114,6 +114,15 @@ void orc_transpose(const void *src, void *dst, size_t es, i64 M, i64 N) {
   orc_100(src, dst, es, M, N, 1);
 }
 
+/* The reference's CPU golden for a full transpose exactly as its test runs it
+ * (test/cl_unittest.cc:547-551), single-threaded: cpu_aos_asta(src, dst, h, w, 1), a 010 with
+ * unit tiles (a copy into `tmp`), then cpu_soa_asta(dst, src, w, h, 1), the 100 that writes
+ * the transpose back into `buf`. `tmp` holds M*N elements. Timed as the CPU baseline. */
+void orc_golden_transpose(void *buf, void *tmp, size_t es, i64 M, i64 N) {
+  orc_010(buf, tmp, es, M, 1, N);
+  orc_100(tmp, buf, es, M, N, 1);
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* In-place restatements of the reference's device algorithms (sequential schedules)    */
 /* ------------------------------------------------------------------------------------ */

@@ -4,11 +4,14 @@
     python profsummary.py pmc    <fetch_counter_collection.csv>
                                                 <write_counter_collection.csv>
                                                 <kernel-substring> <copy-bytes> <out.json>
+    python profsummary.py route  <fetch.csv> <write.csv> <copy-bytes> <transposes>
+                                 <workload> <out.json> <command>...
 
 `pmc` scales the FETCH_SIZE / WRITE_SIZE readings (KiB per dispatch) of the named kernel by
 the factor that maps the same counter on the calibration copy kernel (__amd_rocclr_copyBuffer,
 a device copy of <copy-bytes> bytes: it reads and writes exactly that many) to its known
-traffic, and reports HBM bytes per launch.
+traffic, and reports HBM bytes per launch. `route` does the same for every library kernel of a
+run of full transposes and reports HBM bytes per transpose.
 """
 import csv
 import json
@@ -65,8 +68,37 @@ def pmc(fetch_csv, write_csv, kernel_sub, copy_bytes, out_path):
     return res
 
 
+def route(fetch_csv, write_csv, copy_bytes, calls, workload, out_path, commands):
+    """HBM traffic of every library kernel (names with "lm::") in a run of `calls` full
+    transposes, per transpose, scaled by the calibration copy in the same run."""
+    f = _per_kernel(fetch_csv, "FETCH_SIZE")
+    w = _per_kernel(write_csv, "WRITE_SIZE")
+    cf = statistics.mean(next(v for k, v in f.items() if "__amd_rocclr_copyBuffer" in k))
+    cw = statistics.mean(next(v for k, v in w.items() if "__amd_rocclr_copyBuffer" in k))
+    fs, ws = copy_bytes / (cf * 1024.0), copy_bytes / (cw * 1024.0)
+    per = {}
+    for k in f:
+        if "lm::" not in k:
+            continue
+        rd = sum(f[k]) * 1024 * fs
+        wr = sum(w.get(k, [0.0])) * 1024 * ws
+        per[k[:90]] = {"dispatches": len(f[k]), "hbm_read_bytes": round(rd / calls),
+                       "hbm_write_bytes": round(wr / calls)}
+    tot = sum(v["hbm_read_bytes"] + v["hbm_write_bytes"] for v in per.values())
+    res = {"workload": workload, "commands": commands, "transposes": calls,
+           "calibration": f"one {copy_bytes}-byte __amd_rocclr_copyBuffer in the same run: "
+                          f"FETCH_SIZE x{fs:.4f}, WRITE_SIZE x{ws:.4f}",
+           "hbm_bytes_per_transpose": tot, "per_kernel_per_transpose": per}
+    json.dump(res, open(out_path, "w"), indent=1)
+    return res
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "stats":
         stats(sys.argv[2], sys.argv[3], " ".join(sys.argv[4:]))
     elif sys.argv[1] == "pmc":
         print(pmc(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]), sys.argv[6]))
+    elif sys.argv[1] == "route":
+        # route <fetch.csv> <write.csv> <copy-bytes> <transposes> <workload> <out.json> <cmd>...
+        print(route(sys.argv[2], sys.argv[3], int(sys.argv[4]), int(sys.argv[5]), sys.argv[6],
+                    sys.argv[7], sys.argv[8:]))

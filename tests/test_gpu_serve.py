@@ -123,3 +123,44 @@ def test_server_large_requests_run_alone_and_stay_exact(hip, orc):
     say("large+small", st)
     assert st["completed"] == len(mixed) and st["failed"] == 0
     server.close()
+
+
+def test_server_counts_scratch_against_the_budget(hip, orc):
+    """A request's scratch counts against the HBM budget: a 101 MB transpose whose plain route
+    goes through scratch (a second copy of its buffer) is refused by a budget of 1.5x its buffer
+    and accepted, on an in-place route, once the per-request scratch cap is 0; under a budget
+    of 2.6x the largest buffer, large requests that need scratch queue behind each other and
+    every result stays exact."""
+    import torch
+    M, N, es = 4099, 6151, 4  # 100.9 MB, both dimensions prime
+    x = orc.tagged(M * N, es, seed=61)
+    want = orc.op_transpose(x, M, N, es=es)
+    nbytes = M * N * es
+    keep = hip.get_tuning("serve_scratch_mb")
+    try:
+        server = hip.Server(int(1.5 * nbytes), streams=2, max_batch=8)
+        d = torch.from_numpy(x.copy()).cuda()
+        with pytest.raises(ValueError):
+            server.submit(d, hip.OP_TRANSPOSE, es, M, N)
+        server.close()
+        hip.set_tuning("serve_scratch_mb", 0)  # in-place routes only: the buffer fits alone
+        server = hip.Server(int(1.5 * nbytes), streams=2, max_batch=8)
+        rid = server.submit(d, hip.OP_TRANSPOSE, es, M, N)
+        assert server.drain(120.0) is False and server.status(rid) == 2
+        assert np.array_equal(d.cpu().numpy(), want)
+        server.close()
+    finally:
+        hip.set_tuning("serve_scratch_mb", keep)
+    # through-scratch requests under a budget that holds about two of them with scratch
+    big = []
+    for k, (m, n) in enumerate(((4099, 6151), (4111, 6143), (4093, 6163))):
+        y = orc.tagged(m * n, es, seed=70 + k)
+        big.append((0.0, hip.OP_TRANSPOSE, es, m, n, 0, y, orc.op_transpose(y, m, n, es=es)))
+    small = small_requests(orc, 40, seed=12)
+    mixed = [big[0]] + small[:20] + [big[1]] + small[20:] + [big[2]]
+    server = hip.Server(int(2.6 * nbytes) + (64 << 20), streams=3, max_batch=8)
+    run_and_check(hip, mixed, server, realtime=False)
+    st = server.stats()
+    say("scratch budget", st)
+    assert st["completed"] == len(mixed) and st["failed"] == 0
+    server.close()

@@ -195,3 +195,14 @@ def test_oracle_detects_planted_index_error(orc):
     bad = orc.op_010(x, 3, 5, 4, es=4)  # a and B swapped: a transposed operand
     assert not np.array_equal(good, bad)
     assert np.array_equal(good, x.reshape(3, 4, 5, 4).transpose(0, 2, 1, 3).reshape(-1))
+
+
+@pytest.mark.parametrize("M,N,es", [(37, 53, 8), (1, 9, 4), (64, 3, 1), (30, 30, 16)])
+def test_golden_transpose_is_the_transpose(orc, M, N, es):
+    """The reference test's two-call CPU golden (a 010 with unit tiles, then a 100), which the
+    benchmark times as its CPU baseline, is the plain transpose (numpy)."""
+    x = orc.tagged(M * N, es, seed=M * N)
+    y = x.copy()
+    orc.golden_transpose(y, M, N, es=es)
+    want = x.reshape(M, N, es).transpose(1, 0, 2).reshape(-1)
+    assert np.array_equal(y, want)
