@@ -90,6 +90,9 @@ Knob g_knobs[] = {
     {"bs_upt", {16}},
     {"bs_blocks_per_cu", {192}},
     {"pt010_blocks_per_cu", {8}},
+    {"pt010_wave", {1}, 0},      // 1: the 010 cycle-following form owns one tile per wave
+    {"pt010_lead_order", {64}, 0},  // its leader-only form when every cycle is this short
+    {"pt010_lead_blocks_per_cu", {256}},
     {"pt100_vw_max", {8}},
     {"pt100_blocks_per_cu", {32}},
     {"lds0100_min_inst", {32}},
@@ -364,12 +367,7 @@ bool aligned_ok(const char *who, const void *p, size_t es) {
   return true;
 }
 
-// Workgroups for a grid-stride launch: CUs x a per-CU knob, computed in 64 bits and clamped
-// to [1, 2^24] (knobs accept up to 2^30, so the product could wrap in 32 bits).
-uint32_t grid_cap(int64_t per_cu) {
-  const uint64_t v = uint64_t(std::max(device_info().cus, 1)) * uint64_t(std::max<int64_t>(per_cu, 1));
-  return uint32_t(std::min<uint64_t>(v, uint64_t(1) << 24));
-}
+uint32_t grid_cap(int64_t per_cu) { return grid_blocks(per_cu); }
 
 }  // namespace
 
@@ -380,6 +378,14 @@ uint32_t grid_cap(int64_t per_cu) {
 size_t current_lds_budget() { return lds_budget(); }
 
 void set_last_route(const std::string &route) { t_route = route; }
+
+// Workgroups for a grid-stride launch: CUs x a per-CU knob, computed in 64 bits and clamped
+// to [1, 2^24] (knobs accept up to 2^30, so the product could wrap in 32 bits).
+uint32_t grid_blocks(int64_t per_cu) {
+  const uint64_t v =
+      uint64_t(std::max(device_info().cus, 1)) * uint64_t(std::max<int64_t>(per_cu, 1));
+  return uint32_t(std::min<uint64_t>(v, uint64_t(1) << 24));
+}
 
 // Which form a 010 step can use under the current LDS budget.
 M010 choose_010(size_t es, uint64_t A, uint64_t a, uint64_t B, int R, int P,

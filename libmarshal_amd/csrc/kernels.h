@@ -51,6 +51,8 @@ struct Pt010Params {
   FastDiv32 div_B, div_sh;
 };
 uint64_t pt010_flag_words(uint64_t a, uint64_t B, int R, int P);
+// Picks its form: leader-only (every cycle short, "pt010_lead_order"), else visited bits with
+// one tile per wave ("pt010_wave") or per workgroup.
 hipError_t launch_010_pttwac(unsigned char *base, size_t es, uint64_t A, uint64_t a,
                              uint64_t B, int R, int P, uint32_t max_blocks, hipStream_t st);
 
@@ -327,6 +329,9 @@ class KernelSpan {
   hipStream_t st_;
   int64_t idx_ = -1;
 };
+
+// Workgroups of a grid-stride launch: CUs x per_cu, clamped to [1, 2^24].
+uint32_t grid_blocks(int64_t per_cu);
 
 // Sets what hip_marshal_last_route reports on this thread.
 void set_last_route(const std::string &route);

@@ -73,6 +73,112 @@ __global__ __launch_bounds__(NT) void k_010_pttwac(Pt010Params p) {
   for (uint64_t t = blockIdx.x; t < p.A; t += gridDim.x) pt010_tile<ES, NT>(p, t, lflags);
 }
 
+// Same stage with one tile per wave: 4 waves per workgroup, each with its own bit region of
+// `sh_stride` words, no workgroup barrier (pt010_wave_tile).
+template <int ES>
+__global__ __launch_bounds__(256) void k_010_pt_wave(Pt010Params p, uint32_t sh_stride) {
+  extern __shared__ __align__(16) uint32_t lflags[];
+  const uint32_t wv = threadIdx.x >> 6;
+  uint32_t *wflags = lflags + wv * sh_stride;
+  const uint64_t nw = uint64_t(gridDim.x) * 4;
+  for (uint64_t t = uint64_t(blockIdx.x) * 4 + wv; t < p.A; t += nw)
+    pt010_wave_tile<ES>(p, t, wflags);
+}
+
+// Same stage for tiles whose cycles are all short (the order k of a modulo a*B - 1 is small):
+// the reference's leader-only variant (src/cl/cl_aos_asta.cl:221-236), where each cycle is
+// moved by the one work-item that starts at its smallest slot, rebuilt for gfx950. The tile map
+// is the same for every tile, so each workgroup finds the leaders once: every lane walks the
+// index map from its slots (at most k steps, no data), a wave64 ballot and a popcount prefix
+// compact the leaders into an LDS list. Each wave then owns whole tiles (no workgroup barrier
+// per tile): lane i takes leader i of a round of 64, loads the W slots of its cycle that follow
+// (all W loads in flight together), waits, and stores each value one slot on; the value that
+// returns to the leader is stored last. No visited bits: cycles are disjoint, so no two lanes
+// touch the same element.
+template <int ES, int W>
+__global__ __launch_bounds__(256) void k_010_pt_lead(Pt010Params p, uint32_t order,
+                                                      uint32_t max_leaders, FastDiv32 div_nl) {
+  using E = typename Elem<ES>::T;
+  extern __shared__ __align__(16) uint32_t lds[];
+  uint32_t *count = lds;      // [0]: leaders found
+  uint32_t *list = lds + 4;   // the leaders
+  const uint32_t tid = threadIdx.x, lane = tid & 63u;
+  const uint32_t a = p.a, B = p.B, last = p.n - 1;
+  auto nextp = [&](uint32_t s) {
+    const uint32_t q = p.div_B.div(s);
+    return (s - q * B) * a + q;
+  };
+  if (tid == 0) count[0] = 0;
+  __syncthreads();
+  for (uint32_t s0 = 0; s0 < last; s0 += 256) {  // uniform over the workgroup
+    const uint32_t s = s0 + tid;
+    bool lead = false;
+    if (s < last) {
+      uint32_t t = nextp(s);
+      lead = t != s;
+      for (uint32_t j = 1; lead && t != s; j++) {
+        if (t < s || j > order) {
+          lead = false;
+          break;
+        }
+        t = nextp(t);
+      }
+    }
+    const uint64_t m = __ballot(lead);
+    if (m) {
+      const uint32_t first = uint32_t(__builtin_ctzll(m));
+      uint32_t base = 0;
+      if (lane == first) base = atomicAdd(count, uint32_t(__builtin_popcountll(m)));
+      base = __shfl(base, int(first));
+      const uint32_t k = base + uint32_t(__builtin_popcountll(m & ((1ull << lane) - 1)));
+      if (lead && k < max_leaders) list[k] = s;
+    }
+  }
+  __syncthreads();
+  const uint32_t nl = max_leaders;  // every leader is found: the count is exact (host-side)
+  if (nl == 0 || count[0] != nl) return;  // never true; keeps the list reads in bounds
+  const uint32_t wv = tid >> 6;
+  const uint64_t nw = uint64_t(gridDim.x) * 4;
+  const uint64_t w0 = uint64_t(blockIdx.x) * 4 + wv;
+  if (w0 >= p.A) return;
+  // The wave's (tile, leader) pairs in one flat sequence, 64 at a time: no lane idles at the
+  // end of a tile. Pair i is leader i mod nl of the wave's tile i / nl.
+  const uint64_t tiles = (p.A - w0 + nw - 1) / nw;
+  const uint32_t pairs = uint32_t(tiles * nl);  // < 2^32: checked by the launcher
+  for (uint32_t i = lane; i < pairs; i += 64) {
+    const uint32_t k = div_nl.div(i);  // the wave's k-th tile
+    const uint64_t t = w0 + uint64_t(k) * nw;
+    {
+      E *in = reinterpret_cast<E *>(p.base) + t * uint64_t(p.n);
+      const uint32_t s = list[i - k * nl];
+      E carry = in[s];
+      uint32_t c = s;
+      bool done = false;
+      while (!done) {
+        uint32_t q[W];
+        E d[W];
+        uint32_t n = 0;
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          if (!done) {
+            c = nextp(c);
+            q[j] = c;
+            n = uint32_t(j) + 1;
+            if (c == s) done = true;  // back at the leader: its value was read first
+            else d[j] = in[c];
+          }
+        }
+        wait_vm_loads();  // every slot of the window is read before any is stored
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+          if (uint32_t(j) < n) in[q[j]] = j == 0 ? carry : d[j - 1];
+        }
+        if (!done) carry = d[W - 1];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------------------
@@ -103,6 +209,35 @@ hipError_t launch_pt_t(const Pt010Params &p, size_t lds_bytes, uint32_t blocks,
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k, dim3(blocks), dim3(NT), lds_bytes, st, p);
+  return hipGetLastError();
+}
+
+template <int ES>
+hipError_t launch_ptw_t(const Pt010Params &p, uint32_t stride, uint32_t blocks, hipStream_t st) {
+  auto k = k_010_pt_wave<ES>;
+  const size_t lds = size_t(stride) * 4 * 4;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, st, p, stride);
+  return hipGetLastError();
+}
+
+template <int ES>
+hipError_t launch_ptl_t(const Pt010Params &p, uint32_t order, uint32_t max_leaders,
+                        uint32_t blocks, hipStream_t st) {
+  constexpr int W = ES >= 16 ? 4 : 8;
+  auto k = k_010_pt_lead<ES, W>;
+  const size_t lds = (size_t(max_leaders) + 4) * 4;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, st, p, order, max_leaders,
+                     FastDiv32(max_leaders));
   return hipGetLastError();
 }
 
@@ -255,6 +390,41 @@ hipError_t launch_010_pttwac(unsigned char *base, size_t es, uint64_t A, uint64_
   p.div_B = FastDiv32(p.B);
   p.div_sh = FastDiv32(p.sh);
   size_t lds = size_t(p.sh) * 4;
+  // Short cycles everywhere (order of a modulo a*B - 1 at most "pt010_lead_order"): the
+  // leader-only form, one tile per wave, no bits.
+  PlPlan pc;
+  const int64_t lead_order = tuning("pt010_lead_order");
+  if (lead_order > 0 && a >= 2 && B >= 2 &&
+      plan_100_leaders(a, B, uint32_t(std::min<int64_t>(lead_order, 4096)), &pc) &&
+      pc.cycles >= 1 && (pc.cycles + 4) * 4 <= current_lds_budget() &&
+      pc.cycles <= (uint64_t(1) << 15)) {
+    const uint32_t lblocks = uint32_t(std::max<uint64_t>(
+        1, std::min<uint64_t>((A + 3) / 4, grid_blocks(tuning("pt010_lead_blocks_per_cu")))));
+    const uint64_t nw = uint64_t(lblocks) * 4;
+    if ((A + nw - 1) / nw * pc.cycles >= (uint64_t(1) << 32)) return hipErrorInvalidValue;
+    const uint32_t ml = uint32_t(pc.cycles);
+    switch (es) {
+      case 1: return launch_ptl_t<1>(p, pc.order, ml, lblocks, st);
+      case 2: return launch_ptl_t<2>(p, pc.order, ml, lblocks, st);
+      case 4: return launch_ptl_t<4>(p, pc.order, ml, lblocks, st);
+      case 8: return launch_ptl_t<8>(p, pc.order, ml, lblocks, st);
+      case 16: return launch_ptl_t<16>(p, pc.order, ml, lblocks, st);
+    }
+    return hipErrorInvalidValue;
+  }
+  // One tile per wave when four bit regions fit the LDS budget (knob "pt010_wave" = 1).
+  const uint32_t stride = (p.sh + 3u) & ~3u;
+  if (tuning("pt010_wave") && size_t(stride) * 16 <= current_lds_budget()) {
+    const uint32_t wblocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((A + 3) / 4, max_blocks)));
+    switch (es) {
+      case 1: return launch_ptw_t<1>(p, stride, wblocks, st);
+      case 2: return launch_ptw_t<2>(p, stride, wblocks, st);
+      case 4: return launch_ptw_t<4>(p, stride, wblocks, st);
+      case 8: return launch_ptw_t<8>(p, stride, wblocks, st);
+      case 16: return launch_ptw_t<16>(p, stride, wblocks, st);
+    }
+    return hipErrorInvalidValue;
+  }
   uint32_t blocks = uint32_t(std::min<uint64_t>(A, max_blocks));
   switch (es) {
     case 1: return launch_pt_t<1, NT>(p, lds, blocks, st);

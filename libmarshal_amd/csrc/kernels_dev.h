@@ -275,6 +275,71 @@ __device__ __forceinline__ void pt010_tile(const Pt010Params &p, uint64_t t, uin
   __syncthreads();
 }
 
+// One a x B tile by cycle following, owned by one wave (no workgroup barrier), with the tile's
+// bits in the wave's own LDS region (sh words). The wave sweeps the tile's slots 64 at a time;
+// a wave64 ballot picks the lanes whose start slot is still unclaimed, and each of them walks
+// its cycle with the claim protocol of pt010_tile, keeping the load of the slot after the one
+// it is claiming in flight (a value is stored only after its own slot's claim is won, and only
+// the winner of a slot stores to it, so a prefetched value is either the slot's original
+// content or never used).
+template <int ES>
+__device__ __forceinline__ void pt010_wave_tile(const Pt010Params &p, uint64_t t,
+                                                uint32_t *wflags) {
+  using E = typename Elem<ES>::T;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t a = p.a, B = p.B, n = p.n, sh = p.sh;
+  E *in = reinterpret_cast<E *>(p.base) + t * uint64_t(n);
+  for (uint32_t w = lane; w < sh; w += 64)
+    __hip_atomic_store(&wflags[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  auto nextp = [&](uint32_t s) {
+    const uint32_t q = p.div_B.div(s);
+    return (s - q * B) * a + q;
+  };
+  auto bit = [&](uint32_t s, uint32_t &word) {
+    const uint32_t bi = p.div_sh.div(s);
+    word = s - bi * sh;
+    return 1u << bi;
+  };
+  const uint32_t last = n - 1;
+  for (uint32_t s0 = 0; s0 < last; s0 += 64) {  // uniform over the wave
+    const uint32_t s = s0 + lane;
+    uint32_t word = 0, mask = 0, nxt = 0;
+    bool walk = s < last;
+    if (walk) {
+      nxt = nextp(s);
+      walk = nxt != s;
+    }
+    if (walk) {
+      mask = bit(s, word);
+      walk = !(__hip_atomic_load(&wflags[word], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) &
+               mask);
+    }
+    if (!__ballot(walk)) continue;  // every start of this block is claimed: nothing to load
+    E d1{}, d2{};
+    if (walk) {
+      d1 = in[s];
+      d2 = in[nxt];
+    }
+    wait_vm_loads();  // the start value is read before its bit is tested
+    if (walk)
+      walk = !(__hip_atomic_load(&wflags[word], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) &
+               mask);
+    while (walk) {
+      const uint32_t nn = nextp(nxt);
+      const E d3 = in[nn];  // in flight while nxt is claimed and stored
+      uint32_t wj = 0;
+      const uint32_t mj = bit(nxt, wj);
+      const uint32_t old =
+          __hip_atomic_fetch_or(&wflags[wj], mj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      if (old & mj) break;
+      in[nxt] = d1;
+      d1 = d2;
+      d2 = d3;
+      nxt = nn;
+    }
+  }
+}
+
 // ---- 100 / 0100 cycle following ----
 // Walkers wb .. wb + 64/VW - 1 of a 100/0100 stage with global visited bits (one wave).
 template <int UB, int UPL, bool I64>

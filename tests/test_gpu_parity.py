@@ -105,13 +105,27 @@ def test_010_pttwac_bug536_shapes(hip, orc):
                 check(hip.transpose_010_pttwac, x, want, h // t, t, w, R=R, P=1, elem_bytes=4)
 
 
+@pytest.mark.parametrize("mode", ["lead", "wave", "wg"])
 @pytest.mark.parametrize("es", [1, 4, 8, 16])
-def test_010_pttwac_element_sizes(hip, orc, es):
-    for (A, a, B) in ((50, 33, 47), (7, 64, 100), (3, 300, 211), (1000, 2, 3)):
-        say("pttwac", es, A, a, B)
-        x = orc.tagged(A * a * B, es, seed=A + a + B)
-        check(hip.transpose_010_pttwac, x, orc.op_010(x, A, a, B, es=es), A, a, B, R=1, P=1,
-              elem_bytes=es)
+def test_010_pttwac_element_sizes(hip, orc, es, mode):
+    """Every form of the cycle-following 010: leader-only when all cycles are short ("lead",
+    the default where it applies), visited bits with one tile per wave ("wave") and visited bits
+    with one tile per 256-lane workgroup ("wg"), forced with the knobs "pt010_lead_order" and
+    "pt010_wave"."""
+    keep = {k: hip.get_tuning(k) for k in ("pt010_wave", "pt010_lead_order")}
+    hip.set_tuning("pt010_lead_order", 64 if mode == "lead" else 0)
+    hip.set_tuning("pt010_wave", 0 if mode == "wg" else 1)
+    try:
+        for (A, a, B) in ((50, 33, 47), (7, 64, 100), (3, 300, 211), (1000, 2, 3), (4099, 64, 16),
+                          (257, 16, 64), (9, 8, 8), (5, 2, 1000)):
+            say("pttwac", es, mode, A, a, B)
+            x = orc.tagged(A * a * B, es, seed=A + a + B)
+            for R, P in ((1, 1), (4, 0)):
+                check(hip.transpose_010_pttwac, x, orc.op_010(x, A, a, B, es=es), A, a, B, R=R,
+                      P=P, elem_bytes=es)
+    finally:
+        for k, v in keep.items():
+            hip.set_tuning(k, v)
 
 
 def test_100_bug537_shapes(hip, orc):
