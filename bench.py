@@ -182,7 +182,7 @@ def extra_batch(lm, reps=3):
     bufs = [torch.empty(r * c * es, dtype=torch.uint8, device="cuda") for (_, es, r, c, _) in items]
     stream = torch.cuda.current_stream()
 
-    def measure(idx):
+    def measure(idx, with_sync=False):
         sel = [items[i] for i in idx]
         sb_ = [bufs[i] for i in idx]
         total = sum(r * c * es for (_, es, r, c, _) in sel)
@@ -208,9 +208,28 @@ def extra_batch(lm, reps=3):
                "batched_GBs": round(2 * total / sb / 1e9, 1), "batched_ms": round(sb * 1e3, 2),
                "plain_GBs": round(2 * total / sp / 1e9, 1), "plain_ms": round(sp * 1e3, 2),
                "speedup": round(sp / sb, 2)}
+        if with_sync:
+            # The same plain calls with the library's default timing convention (each call
+            # measured and synchronized, as a ported caller of the reference gets by default).
+            et = lm.Elapsed()
+            lm.set_profiling(True)
+            t0 = time.perf_counter()
+            for b, (op, es, r, c, t) in zip(sb_, sel):
+                if op == wl.OP_TRANSPOSE:
+                    e = lm.transpose_full(b, r, c, elem_bytes=es, elapsed=et)
+                elif op == wl.OP_AOS_ASTA:
+                    e = lm.aos_asta(b, r, c, t, elem_bytes=es, elapsed=et)
+                else:
+                    e = lm.soa_asta(b, r, c, t, elem_bytes=es, elapsed=et)
+                if e:
+                    raise RuntimeError("plain call failed")
+            ss = time.perf_counter() - t0
+            lm.set_profiling(False)
+            out["plain_synchronized_GBs"] = round(2 * total / ss / 1e9, 1)
+            out["speedup_vs_synchronized"] = round(ss / sb, 2)
         batch.close()
         return out
-    out = measure(range(len(items)))
+    out = measure(range(len(items)), with_sync=True)
     classes = {}
     for name, lo, hi in (("lt_1MB", 0, 1 << 20), ("1_16MB", 1 << 20, 16 << 20),
                          ("16_128MB", 16 << 20, 128 << 20), ("ge_128MB", 128 << 20, 1 << 62)):

@@ -104,7 +104,8 @@ bool hip_padding(hipStream_t stream, void *src, int x_size, int y_size, int pad_
 bool hip_unpadding(hipStream_t stream, void *src, int x_size, int y_size, int pad_size,
                    uint64_t *elapsed_ns);
 
-/* Releases the scratch pools of every device. Replaces cl_marshal_finalize,
+/* Releases the scratch pools, cycle-leader lists and segment tables of every device and
+ * forgets measured choices (autotuned shapes and classes). Replaces cl_marshal_finalize,
  * src/cl_marshal.cc:116-119 (exported there but missing from its header). */
 void hip_marshal_finalize(void);
 
@@ -323,6 +324,20 @@ bool hip_marshal_set_tuning(const char *key, int64_t value);
  * test/cl_unittest.cc:396-495). */
 bool hip_marshal_autotune(hipStream_t stream, void *buf, size_t elem_bytes, int64_t M,
                           int64_t N, int *stages);
+/* Measured autotuner per shape class. For an AoS/SoA <-> ASTA descriptor, every form its step
+ * can take (the 010 step: LDS-staged with three grid sizes, through scratch, split, row/column,
+ * cycle following; the 100 step: cycle leaders, the segmented walk with three segment
+ * lengths, through scratch, row/column, LDS bits, global bits) is run on `desc->ptr` and timed,
+ * each followed by the inverse step, so the buffer ends bit-identical; the fastest becomes the
+ * choice for its class (step kind, element or super-element size, power-of-two buckets of the
+ * dimensions and of the bytes), used by every later plan in the class: plain calls, batches,
+ * the server. *form receives the chosen form code. A transpose descriptor (or a 100 over 1-16
+ * byte super-elements) is tuned as by hip_marshal_autotune for its exact shape, and *form
+ * receives the stage code. The buffer is first copied to a device snapshot (when it fits the
+ * scratch cap); if a candidate fails, the snapshot is copied back and true is returned.
+ * Synchronizes the stream; not under graph capture. With the knob "autotune" set,
+ * hip_marshal_batch_create tunes each class of its descriptors (once per class) this way. */
+bool hip_marshal_autotune_desc(hipStream_t stream, const hip_marshal_desc *desc, int *form);
 int64_t hip_marshal_get_tuning(const char *key);
 /* The planner's choice for an M x N transpose of elem_bytes elements: tile factors a | M,
  * b | N and the stage sequence (2, 22, 3, 32; 5 = row/column shuffles, 6/7 = stripes of

@@ -101,6 +101,7 @@ def lib():
         "hip_marshal_version": (ctypes.c_char_p, []),
         "hip_marshal_set_tuning": (b, [ctypes.c_char_p, i64]),
         "hip_marshal_autotune": (b, [vp, vp, sz, i64, i64, ctypes.POINTER(ctypes.c_int)]),
+        "hip_marshal_autotune_desc": (b, [vp, ctypes.POINTER(Desc), ctypes.POINTER(ctypes.c_int)]),
         "hip_marshal_plan_transpose": (b, [i64, i64, sz, ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_int)]),
@@ -150,7 +151,8 @@ EXPORTED = [
     "hip_marshal_set_profiling", "hip_marshal_last_route", "hip_marshal_reserve",
     "hip_marshal_flag_bytes_0100", "hip_marshal_cycle_count", "hip_marshal_version",
     "hip_marshal_set_tuning",
-    "hip_marshal_get_tuning", "hip_marshal_plan_transpose", "hip_marshal_autotune", "hip_marshal_batch_create", "hip_marshal_batch_run",
+    "hip_marshal_get_tuning", "hip_marshal_plan_transpose", "hip_marshal_autotune",
+    "hip_marshal_autotune_desc", "hip_marshal_batch_create", "hip_marshal_batch_run",
     "hip_marshal_batch_info", "hip_marshal_batch_route", "hip_marshal_batch_destroy",
     "hip_marshal_server_create", "hip_marshal_server_submit", "hip_marshal_server_submit_at",
     "hip_marshal_server_now", "hip_marshal_server_status", "hip_marshal_server_wait",
@@ -331,6 +333,22 @@ def autotune(buf, M, N, elem_bytes=None, stream=None):
                                   ctypes.byref(st)):
         raise RuntimeError("hip_marshal_autotune failed")
     return st.value
+
+
+# Form codes hip_marshal_autotune_desc reports for AoS/SoA <-> ASTA classes.
+FORM_010 = {1: "010_BS-", 2: "010_OOP-", 3: "010_SPLIT-", 4: "010_RC-", 5: "010_PTTWAC-"}
+FORM_0100 = {11: "0100_CYC-", 12: "0100_SEG-", 13: "0100_OOP-", 14: "0100_RC-", 15: "0100_LDS-",
+             16: "0100_PTTWAC-"}
+
+
+def autotune_desc(buf, op, elem_bytes, rows, cols, tile=0, stream=None):
+    """Measures the forms of the conversion's shape class on `buf` (left unchanged) and makes
+    the fastest the choice for the class; returns its form code (a stage code for transposes)."""
+    d = Desc(_ptr(buf, _need(elem_bytes, rows, cols)).value, op, elem_bytes, rows, cols, tile)
+    f = ctypes.c_int(0)
+    if lib().hip_marshal_autotune_desc(_stream(stream), ctypes.byref(d), ctypes.byref(f)):
+        raise RuntimeError("hip_marshal_autotune_desc failed")
+    return f.value
 
 
 def cycle_count(R, C, max_order=64):

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <array>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -426,6 +427,46 @@ static uint64_t oop_scratch_bytes(const StageRec &r) {
   return P ? r.chunk * r.d2 * P : r.chunk * r.d1 * r.d2 * r.d3;
 }
 
+// ---------------------------------------------------------------------------------------
+// Measured choices per shape class (hip_marshal_autotune_desc). A class is the stage kind,
+// the element or super-element size and the power-of-two buckets of the dimensions and of the
+// total bytes; its entry names the form the planner takes for every stage of that class (when
+// the form can run there) and, for LDS-staged and segmented stages, a launch parameter.
+// While a candidate is measured, a thread-local choice overrides the table.
+// ---------------------------------------------------------------------------------------
+namespace {
+std::mutex g_class_mu;
+std::map<ClassKey, ClassChoice> g_class;
+thread_local const ClassChoice *t_force = nullptr;
+
+int64_t lg2(uint64_t v) {
+  int64_t l = 0;
+  while (v > 1) {
+    v >>= 1;
+    l++;
+  }
+  return l;
+}
+int64_t unit_of(uint64_t S) { return S % 16 == 0 ? 16 : S % 8 == 0 ? 8 : S % 4 == 0 ? 4 : S % 2 == 0 ? 2 : 1; }
+}  // namespace
+
+ClassKey class_key_010(size_t es, uint64_t A, uint64_t a, uint64_t B) {
+  return {1, int64_t(es), lg2(a), lg2(B), lg2(A * a * B * es), 0};
+}
+ClassKey class_key_0100(uint64_t inst, uint64_t R, uint64_t C, uint64_t S) {
+  return {2, lg2(S), unit_of(S), lg2(R), lg2(C), lg2(inst * R * C * S)};
+}
+bool class_tuned(const ClassKey &k) {
+  std::lock_guard<std::mutex> lk(g_class_mu);
+  return g_class.count(k) != 0;
+}
+ClassChoice class_choice(const ClassKey &k) {
+  if (t_force) return *t_force;
+  std::lock_guard<std::mutex> lk(g_class_mu);
+  auto it = g_class.find(k);
+  return it == g_class.end() ? ClassChoice{} : it->second;
+}
+
 // Through scratch: groups of blocks are copied to scratch and moved back out of place,
 // [inst][R][C] super-elements of S bytes -> [inst][C][R]; every byte is read twice and
 // written twice by streaming kernels. One block must fit the scratch cap ("st_scratch_mb").
@@ -462,66 +503,94 @@ void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64
   r.d2 = C;
   r.d3 = b;
   plan_100(es, inst, R, C, b, &r.pt, 0, reinterpret_cast<uintptr_t>(buf));
+  const uint64_t S = b * es;
+  const ClassChoice cc = class_choice(class_key_0100(inst, R, C, S));
   // Many small instances: visited bits of each instance in LDS, one workgroup per instance.
   const bool lds_form = inst >= uint64_t(tuning("lds0100_min_inst")) &&
                         R * C * r.pt.nsub < (1ull << 31) &&
                         lds0100_bytes(R, C, r.pt) <= lds_budget();
   // Short cycles (small order of R modulo R*C - 1) over super-elements of some size: one
   // in-place pass with no claims, no scratch beyond the leader list.
-  if (allow_oop && b * es >= uint64_t(tuning("pl_min_bytes"))) {
+  StageRec cy = r;
+  bool cyc_ok = false;
+  if (allow_oop && (S >= uint64_t(tuning("pl_min_bytes")) || cc.form == kForm0100Cyc)) {
     PlPlan pc;
     if (plan_100_leaders(R, C, uint32_t(tuning("pl_max_order")), &pc) &&
-        pc.cycles * inst * r.pt.nsub >= uint64_t(tuning("pl_min_walkers"))) {
-      r.kind = StageKind::PT100C;
-      r.pc = pc;
-      if (tag) *tag = "0100_CYC-";
-      out.push_back(r);
-      return;
+        (pc.cycles * inst * r.pt.nsub >= uint64_t(tuning("pl_min_walkers")) ||
+         cc.form == kForm0100Cyc) && pc.cycles >= 1) {
+      cy.kind = StageKind::PT100C;
+      cy.pc = pc;
+      cyc_ok = true;
     }
   }
   // Long cycles over super-elements of some size: the segmented walk, one pass with a side
   // buffer of each segment's start piece (worth it while that buffer stays under a quarter of
   // the data; grids of many short cycles have better forms).
+  // (buf == nullptr: pricing a candidate route only; the table is not built, its segment
+  // count is estimated.)
   StageRec sg = r;
   sg.kind = StageKind::SEG;
   bool seg_ok = false;
-  // (buf == nullptr: pricing a candidate route only; the table is not built, its segment
-  // count is estimated.)
-  if (!planning_restricted() && b * es >= uint64_t(tuning("seg_min_bytes")) &&
-      plan_100_segments(inst, R, C, b * es, reinterpret_cast<uintptr_t>(buf),
-                        uint32_t(tuning("seg_len")), uint64_t(tuning("seg_max_slots")),
-                        uint64_t(tuning("seg_sync_slots")), buf != nullptr, &sg.sg))
-    seg_ok = 4.0 * double(seg_save_bytes(inst, b * es, sg.sg)) <=
-             double(inst) * double(R) * double(C) * double(b * es);
+  const uint32_t seg_len =
+      uint32_t(cc.form == kForm0100Seg && cc.param > 0 ? cc.param : tuning("seg_len"));
+  if (!planning_restricted() &&
+      (S >= uint64_t(tuning("seg_min_bytes")) || cc.form == kForm0100Seg) &&
+      plan_100_segments(inst, R, C, S, reinterpret_cast<uintptr_t>(buf), seg_len,
+                        uint64_t(tuning("seg_max_slots")), uint64_t(tuning("seg_sync_slots")),
+                        buf != nullptr, &sg.sg))
+    seg_ok = 4.0 * double(seg_save_bytes(inst, S, sg.sg)) <=
+             double(inst) * double(R) * double(C) * double(S);
+  std::vector<StageRec> o;
+  const bool oop_ok = allow_oop && push_oop(o, buf, es, inst, R, C, S);
+  // A single 100 over short super-elements is a full R x C transpose of S-byte elements: the
+  // LDS-staged row / column decomposition beats cycle following there.
+  std::vector<StageRec> rc;
+  const bool rc_ok = inst == 1 && (S < uint64_t(tuning("rc_max_super_bytes")) ||
+                                   cc.form == kForm0100Rc) &&
+                     push_rc(rc, buf, es, R, C, S);
   StageRec l = r;
   l.kind = lds_form ? StageKind::PT0100L : StageKind::PT100G;
-  if (allow_oop) {
-    std::vector<StageRec> o;
-    if (push_oop(o, buf, es, inst, R, C, b * es)) {
-      double inplace = lds_form ? stage_bw_tbs(l) : 0.0;
-      if (seg_ok) inplace = std::max(inplace, tuning("seg_force") ? 1e9 : stage_bw_tbs(sg));
-      if (inplace < stage_bw_tbs(o[0])) {
-        out.push_back(o[0]);
-        if (tag) *tag = "0100_OOP-";
+  auto emit = [&](const StageRec &x, const char *t) {
+    out.push_back(x);
+    if (tag) *tag = t;
+  };
+  // A measured choice for this class, where its form can run.
+  switch (cc.form) {
+    case kForm0100Cyc: if (cyc_ok) return emit(cy, "0100_CYC-"); break;
+    case kForm0100Seg: if (seg_ok) return emit(sg, "0100_SEG-"); break;
+    case kForm0100Oop: if (oop_ok) return emit(o[0], "0100_OOP-"); break;
+    case kForm0100Rc:
+      if (rc_ok) {
+        out.insert(out.end(), rc.begin(), rc.end());
+        if (tag) *tag = "0100_RC-";
         return;
       }
+      break;
+    case kForm0100Lds: if (lds_form) return emit(l, "0100_LDS-"); break;
+    case kForm0100Bits: {
+      StageRec g = r;
+      g.kind = StageKind::PT100G;
+      return emit(g, "0100_PTTWAC-");
     }
+    default: break;
   }
-  if (seg_ok && (stage_bw_tbs(sg) > stage_bw_tbs(l) || tuning("seg_force"))) {
-    if (tag) *tag = "0100_SEG-";
-    out.push_back(sg);
-    return;
+  // The model's choice.
+  if (cyc_ok && S >= uint64_t(tuning("pl_min_bytes")) &&
+      cy.pc.cycles * inst * r.pt.nsub >= uint64_t(tuning("pl_min_walkers")))
+    return emit(cy, "0100_CYC-");
+  if (oop_ok) {
+    double inplace = lds_form ? stage_bw_tbs(l) : 0.0;
+    if (seg_ok) inplace = std::max(inplace, tuning("seg_force") ? 1e9 : stage_bw_tbs(sg));
+    if (inplace < stage_bw_tbs(o[0])) return emit(o[0], "0100_OOP-");
   }
-  // A single 100 over short super-elements is a full R x C transpose of (b*es)-byte
-  // elements: the LDS-staged row / column decomposition beats cycle following there.
-  if (inst == 1 && b * es < uint64_t(tuning("rc_max_super_bytes")) &&
-      push_rc(out, buf, es, R, C, b * es)) {
+  if (seg_ok && (stage_bw_tbs(sg) > stage_bw_tbs(l) || tuning("seg_force")))
+    return emit(sg, "0100_SEG-");
+  if (rc_ok && S < uint64_t(tuning("rc_max_super_bytes"))) {
+    out.insert(out.end(), rc.begin(), rc.end());
     if (tag) *tag = "0100_RC-";
     return;
   }
-  r.kind = lds_form ? StageKind::PT0100L : StageKind::PT100G;
-  if (tag) *tag = lds_form ? "0100_LDS-" : "0100_PTTWAC-";
-  out.push_back(r);
+  emit(l, lds_form ? "0100_LDS-" : "0100_PTTWAC-");
 }
 
 bool plan_step_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_t inst,
@@ -580,6 +649,45 @@ bool push_010(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_
               uint64_t a, uint64_t B, M010 m, int R, int P, const char **tag, bool allow_oop) {
   if (tag) *tag = "";
   if (A == 0 || a <= 1 || B <= 1) return true;  // 1 x B and a x 1 tiles are unchanged
+  int64_t bs_grid = 0;  // a measured grid for the LDS-staged form (0: the knob)
+  if (m == M010::Auto) {
+    // A measured choice for this class, where its form can run.
+    const ClassChoice cc = class_choice(class_key_010(es, A, a, B));
+    Bs010Plan pl{};
+    switch (cc.form) {
+      case kForm010Bs:
+        if (plan_010_bs(es, A, a, B, lds_budget(), reinterpret_cast<uintptr_t>(buf),
+                        int(tuning("bs_upt")), &pl)) {
+          m = M010::BS;
+          bs_grid = cc.param;
+        }
+        break;
+      case kForm010Oop:
+        if (allow_oop && push_oop(out, buf, es, A, a, B, es)) {
+          if (tag) *tag = "010_OOP-";
+          return true;
+        }
+        break;
+      case kForm010Split:
+        if (push_010_split(out, buf, es, A, a, B)) {
+          if (tag) *tag = "010_SPLIT-";
+          return true;
+        }
+        break;
+      case kForm010Rc:
+        if (push_rc(out, buf, es, a, B, es, A)) {
+          if (tag) *tag = "010_RC-";
+          return true;
+        }
+        break;
+      case kForm010Cyc:
+        if (R >= 1 && P >= 0 && a * B < (1ull << 31) &&
+            pt010_flag_words(a, B, R, P) * 4 <= lds_budget())
+          m = M010::PTTWAC;
+        break;
+      default: break;
+    }
+  }
   if (m == M010::Auto) {
     m = choose_010(es, A, a, B, R, P, reinterpret_cast<uintptr_t>(buf));
     // Through scratch (two streaming passes) when allowed, for tiles too large for LDS, for
@@ -626,6 +734,7 @@ bool push_010(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64_
       return false;
     }
     r.kind = StageKind::BS010;
+    r.grid = bs_grid;
     if (tag) *tag = "010_BS-";
     out.push_back(r);
     return true;
@@ -804,7 +913,7 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
     switch (r.kind) {
       case StageKind::BS010:
         LM_CHECK_HIP(launch_010_bs(r.buf, r.es, r.d0, r.d1, r.d2, r.bs,
-                                   grid_cap(tuning("bs_blocks_per_cu")), st));
+                                   grid_cap(r.grid ? r.grid : tuning("bs_blocks_per_cu")), st));
         break;
       case StageKind::PT010:
         LM_CHECK_HIP(launch_010_pttwac(r.buf, r.es, r.d0, r.d1, r.d2, r.R, r.P,
@@ -1285,6 +1394,158 @@ static bool autotune_full(hipStream_t st, unsigned char *buf, size_t es, uint64_
   return false;
 }
 
+// ---------------------------------------------------------------------------------------
+// Measured autotuner per class of 010 / 0100 steps (AoS/SoA <-> ASTA conversions): every
+// form the class could take (LDS-staged with three grid sizes, through scratch, the split,
+// row/column shuffles, cycle following; for 0100 also cycle leaders, the segmented walk with
+// three segment lengths, LDS bits, global bits) is planned for the caller's buffer, run and
+// timed, and followed by the inverse step (planned by the model), which restores the buffer
+// bit for bit. The fastest becomes the class's entry. Before the first candidate runs, the
+// buffer is copied to a device snapshot (when one fits the scratch cap); if any launch fails,
+// the snapshot is copied back and the call reports the failure. No counterpart in the
+// reference (its kernels and work-group sizes are fixed per vendor,
+// src/cl_marshal.cc:165-231, 388-465).
+// ---------------------------------------------------------------------------------------
+// The route tag a step planned in a given form reports.
+static const char *form_tag(int form) {
+  switch (form) {
+    case kForm010Bs: return "010_BS-";
+    case kForm010Oop: return "010_OOP-";
+    case kForm010Split: return "010_SPLIT-";
+    case kForm010Rc: return "010_RC-";
+    case kForm010Cyc: return "010_PTTWAC-";
+    case kForm0100Cyc: return "0100_CYC-";
+    case kForm0100Seg: return "0100_SEG-";
+    case kForm0100Oop: return "0100_OOP-";
+    case kForm0100Rc: return "0100_RC-";
+    case kForm0100Lds: return "0100_LDS-";
+    case kForm0100Bits: return "0100_PTTWAC-";
+  }
+  return "";
+}
+
+static bool autotune_step(hipStream_t st, const hip_marshal_desc &d, ClassChoice *out,
+                          int *route_code) {
+  const size_t es = size_t(d.elem_bytes);
+  unsigned char *buf = static_cast<unsigned char *>(d.ptr);
+  const uint64_t h = uint64_t(d.rows), w = uint64_t(d.cols), t = uint64_t(d.tile);
+  const bool is010 = d.op == HIP_MARSHAL_OP_AOS_ASTA || d.op == HIP_MARSHAL_OP_ASTA_AOS;
+  const bool fwd = d.op == HIP_MARSHAL_OP_AOS_ASTA || d.op == HIP_MARSHAL_OP_SOA_ASTA;
+  // 010(A, a, B) and its inverse 010(A, B, a); 100(R, C, b) and its inverse 100(C, R, b)
+  const uint64_t A = h / t, a = is010 ? (fwd ? t : w) : 0, B = is010 ? (fwd ? w : t) : 0;
+  const uint64_t R = is010 ? 0 : (fwd ? w : h / t), C = is010 ? 0 : (fwd ? h / t : w), b = t;
+  const uint64_t S = b * es;
+  if (!is010 && (S == 1 || S == 2 || S == 4 || S == 8 || S == 16) &&
+      reinterpret_cast<uintptr_t>(buf) % S == 0) {
+    // a single 100 over 1-16-byte super-elements is a full transpose: tune that shape
+    Choice c;
+    if (autotune_full(st, buf, S, R, C, true, &c)) return true;
+    out->form = kFormModel;
+    if (route_code) *route_code = c.stages;
+    return false;
+  }
+  ClassKey key;
+  std::vector<ClassChoice> cands;
+  if (is010) {
+    key = class_key_010(es, A, a, B);
+    cands = {{kForm010Bs, 64}, {kForm010Bs, 192}, {kForm010Bs, 512}, {kForm010Oop, 0},
+             {kForm010Split, 0}, {kForm010Rc, 0}, {kForm010Cyc, 0}};
+  } else {
+    key = class_key_0100(1, R, C, S);
+    cands = {{kForm0100Cyc, 0}, {kForm0100Seg, 16}, {kForm0100Seg, 32}, {kForm0100Seg, 64},
+             {kForm0100Oop, 0},  {kForm0100Rc, 0},   {kForm0100Lds, 0},  {kForm0100Bits, 0}};
+  }
+  auto plan = [&](std::vector<StageRec> &v, std::string &tag, bool inverse) {
+    if (is010) {
+      const char *tg = "";
+      const bool ok = inverse ? push_010(v, buf, es, A, B, a, M010::Auto, 1, 1, &tg, true)
+                              : push_010(v, buf, es, A, a, B, M010::Auto, 1, 1, &tg, true);
+      tag = tg;
+      return ok;
+    }
+    return inverse ? plan_step_0100(v, buf, es, 1, C, R, b, true, tag)
+                   : plan_step_0100(v, buf, es, 1, R, C, b, true, tag);
+  };
+  std::vector<StageRec> inv;
+  std::string itag;
+  if (!plan(inv, itag, true)) return true;
+  const uint64_t bytes = h * w * es;
+  void *snap = nullptr;
+  if (bytes <= (uint64_t(tuning("st_scratch_mb")) << 20) &&
+      hipMalloc(&snap, bytes) == hipSuccess &&
+      hipMemcpyAsync(snap, buf, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+    hipFree(snap);
+    snap = nullptr;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool err = hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess;
+  float best_ms = 1e30f;
+  int best = -1;
+  std::vector<std::string> seen;
+  for (size_t i = 0; i < cands.size() && !err; i++) {
+    std::vector<StageRec> v;
+    std::string tag;
+    t_force = &cands[i];
+    const bool ok = plan(v, tag, false);
+    t_force = nullptr;
+    if (!ok || v.empty() || tag != form_tag(cands[i].form)) continue;  // cannot run here
+    const std::string id = tag + ":" + std::to_string(cands[i].param);
+    if (std::find(seen.begin(), seen.end(), id) != seen.end()) continue;
+    seen.push_back(id);
+    {
+      Pool &p = pool_here();
+      std::lock_guard<std::mutex> lk(p.mu);
+      if (pool_grow(p, std::max(stages_scratch_bytes(v), stages_scratch_bytes(inv)))) {
+        err = true;
+        break;
+      }
+    }
+    float ms = 0;
+    if (bytes < (uint64_t(1) << 30))  // one untimed round trip: first launches of a form
+      err = run_stages(v, st) || run_stages(inv, st);
+    err = err || hipEventRecord(e0, st) != hipSuccess || run_stages(v, st) ||
+          hipEventRecord(e1, st) != hipSuccess || run_stages(inv, st) ||
+          hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess || hipStreamSynchronize(st) != hipSuccess;
+    if (plan_debug())
+      fprintf(stderr, "[libmarshal] autotune class %s: form %d param %lld %s %.3f ms\n",
+              is010 ? "010" : "0100", cands[i].form, (long long)cands[i].param, tag.c_str(), ms);
+    if (!err && ms < best_ms) {
+      best_ms = ms;
+      best = int(i);
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (err) {
+    hipStreamSynchronize(st);
+    if (snap) {
+      hipMemcpy(buf, snap, bytes, hipMemcpyDeviceToDevice);
+      fprintf(stderr, "[libmarshal] autotune: a candidate failed; the buffer was restored\n");
+    } else {
+      fprintf(stderr, "[libmarshal] autotune: a candidate failed; the buffer was not "
+                      "restored (larger than the scratch cap)\n");
+    }
+  }
+  if (snap) {
+    hipStreamSynchronize(st);
+    hipFree(snap);
+  }
+  if (err) return true;
+  if (best < 0) {  // nothing to move (unit tiles) or no form can run: the model decides
+    *out = ClassChoice{};
+    if (route_code) *route_code = 0;
+    return false;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_class_mu);
+    g_class[key] = cands[size_t(best)];
+  }
+  *out = cands[size_t(best)];
+  if (route_code) *route_code = 0;
+  return false;
+}
+
 // Stage list of an M x N transpose as chosen by the planner (stages = 0).
 bool plan_transpose_stages(std::vector<StageRec> &out, unsigned char *buf, size_t es,
                            uint64_t M, uint64_t N, std::string &route, bool allow_st) {
@@ -1325,6 +1586,31 @@ bool hip_marshal_autotune(hipStream_t st, void *buf, size_t es, int64_t M, int64
       autotune_full(st, static_cast<unsigned char *>(buf), es, uint64_t(M), uint64_t(N), true, &c))
     return true;
   if (stages) *stages = c.stages;
+  return false;
+}
+
+bool hip_marshal_autotune_desc(hipStream_t st, const hip_marshal_desc *d, int *form) {
+  if (!d) return true;
+  const size_t es = size_t(d->elem_bytes);
+  uint64_t n;
+  if (!dims_ok("hip_marshal_autotune_desc", {d->rows, d->cols}, es, &n) ||
+      !aligned_ok("hip_marshal_autotune_desc", d->ptr, es))
+    return true;
+  if (d->op < HIP_MARSHAL_OP_TRANSPOSE || d->op > HIP_MARSHAL_OP_ASTA_SOA ||
+      (d->op != HIP_MARSHAL_OP_TRANSPOSE && (d->tile < 1 || d->rows % d->tile))) {
+    fprintf(stderr, "[libmarshal] hip_marshal_autotune_desc: invalid descriptor\n");
+    return true;
+  }
+  if (d->op == HIP_MARSHAL_OP_TRANSPOSE) {
+    int stages = 0;
+    if (hip_marshal_autotune(st, d->ptr, es, d->rows, d->cols, &stages)) return true;
+    if (form) *form = stages;
+    return false;
+  }
+  ClassChoice c;
+  int code = 0;
+  if (autotune_step(st, *d, &c, &code)) return true;
+  if (form) *form = c.form == kFormModel ? code : c.form;
   return false;
 }
 
@@ -1369,6 +1655,15 @@ size_t hip_marshal_flag_bytes_0100(int64_t A, int64_t a, int64_t B, int64_t b,
 
 void hip_marshal_finalize(void) {
   free_segment_tables();
+  {  // measured choices go with the rest of the library's state
+    std::lock_guard<std::mutex> lk(g_class_mu);
+    g_class.clear();
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    g_tuned.clear();
+    g_plan_cache.clear();
+  }
   {
     std::lock_guard<std::mutex> lk(g_lc_mu);
     int cur = 0;

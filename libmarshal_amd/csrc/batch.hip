@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -388,6 +389,7 @@ bool plan_desc(const hip_marshal_desc &d, std::vector<StageRec> &out, std::strin
 // word offsets into the batch's flag space (see build_image).
 bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, const char *who,
                 bool allow_solo) {
+  SegSyncBuild sync_tables;  // creation is not latency-critical: no background table builds
   // 1. routes
   std::vector<std::vector<StageRec>> per(count);
   b.routes.resize(count);
@@ -613,6 +615,40 @@ struct hip_marshal_batch {
   bool use_graph = true;
 };
 
+namespace lm {
+// Measures each shape class of the descriptors once (its first descriptor's buffer, which ends
+// bit-identical), on a private stream, before the batch is planned: transposes per exact
+// shape, AoS/SoA <-> ASTA steps per class (hip_marshal_autotune_desc).
+bool autotune_classes(const hip_marshal_desc *descs, int64_t count) {
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return true;
+  std::set<ClassKey> done;
+  bool err = false;
+  for (int64_t i = 0; i < count && !err; i++) {
+    const hip_marshal_desc &d = descs[i];
+    const uint64_t h = uint64_t(std::max<int64_t>(d.rows, 1)),
+                   w = uint64_t(std::max<int64_t>(d.cols, 1));
+    const uint64_t t = uint64_t(std::max<int64_t>(d.tile, 1));
+    const size_t es = size_t(d.elem_bytes);
+    ClassKey k{};
+    switch (d.op) {
+      case HIP_MARSHAL_OP_TRANSPOSE: k = {0, int64_t(es), int64_t(h), int64_t(w), 0, 0}; break;
+      case HIP_MARSHAL_OP_AOS_ASTA: k = class_key_010(es, h / t, t, w); break;
+      case HIP_MARSHAL_OP_ASTA_AOS: k = class_key_010(es, h / t, w, t); break;
+      case HIP_MARSHAL_OP_SOA_ASTA: k = class_key_0100(1, w, h / t, t * es); break;
+      case HIP_MARSHAL_OP_ASTA_SOA: k = class_key_0100(1, h / t, w, t * es); break;
+      default: continue;
+    }
+    if (!done.insert(k).second || (d.op != HIP_MARSHAL_OP_TRANSPOSE && class_tuned(k))) continue;
+    int form = 0;
+    err = hip_marshal_autotune_desc(st, &d, &form);
+  }
+  hipStreamSynchronize(st);
+  hipStreamDestroy(st);
+  return err;
+}
+}  // namespace lm
+
 extern "C" {
 
 hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64_t count,
@@ -620,6 +656,10 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
   const char *who = "hip_marshal_batch_create";
   if (!descs || count < 1) {
     fprintf(stderr, "[libmarshal] %s: empty batch\n", who);
+    return nullptr;
+  }
+  if (tuning("autotune") && autotune_classes(descs, count)) {
+    fprintf(stderr, "[libmarshal] %s: autotuning failed\n", who);
     return nullptr;
   }
   auto *hb = new hip_marshal_batch();

@@ -3,6 +3,7 @@
 
 #include "common.h"
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -235,6 +236,12 @@ struct SegPlan {
 bool plan_100_segments(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64_t base_addr,
                        uint32_t seg_len, uint64_t max_slots, uint64_t sync_slots, bool build,
                        SegPlan *out);
+// While one of these is alive on a thread, segment tables of any size are built on that thread
+// when first needed (planning that is not latency-critical, such as a batch's creation).
+struct SegSyncBuild {
+  SegSyncBuild();
+  ~SegSyncBuild();
+};
 // Bytes of the side buffer (every segment's start piece, per instance).
 size_t seg_save_bytes(uint64_t inst, uint64_t S, const SegPlan &sp);
 hipError_t launch_100_segments(unsigned char *base, unsigned char *save, uint64_t inst,
@@ -261,6 +268,18 @@ struct StageRec {
   uint64_t pitch;           // OOP: scratch row pitch of the pitched form (0: plain form)
   PlPlan pc;                // PT100C: cycle count and order (pt: the walker shape)
   SegPlan sg;               // SEG: segment table and walker shape
+  int64_t grid;             // BS010: workgroups per CU chosen for its class (0: the knob)
+};
+// Forms a measured class choice can name (ClassChoice::form; 0 = the cost model decides).
+enum : int {
+  kFormModel = 0,
+  kForm010Bs = 1, kForm010Oop = 2, kForm010Split = 3, kForm010Rc = 4, kForm010Cyc = 5,
+  kForm0100Cyc = 11, kForm0100Seg = 12, kForm0100Oop = 13, kForm0100Rc = 14,
+  kForm0100Lds = 15, kForm0100Bits = 16,
+};
+struct ClassChoice {
+  int form = kFormModel;
+  int64_t param = 0;  // BS: workgroups per CU; SEG: slots per segment (0: the knob)
 };
 // While one of these is alive on a thread, planning on that thread uses only the forms whose
 // scratch is visited bits or nothing (no side buffers, no device tables): the batch's grouped
@@ -332,6 +351,13 @@ class KernelSpan {
 
 // Workgroups of a grid-stride launch: CUs x per_cu, clamped to [1, 2^24].
 uint32_t grid_blocks(int64_t per_cu);
+
+// Measured class choices (hip_marshal_autotune_desc) as the planner consults them.
+using ClassKey = std::array<int64_t, 6>;
+ClassKey class_key_010(size_t es, uint64_t A, uint64_t a, uint64_t B);
+ClassKey class_key_0100(uint64_t inst, uint64_t R, uint64_t C, uint64_t S);
+ClassChoice class_choice(const ClassKey &k);
+bool class_tuned(const ClassKey &k);
 
 // Sets what hip_marshal_last_route reports on this thread.
 void set_last_route(const std::string &route);

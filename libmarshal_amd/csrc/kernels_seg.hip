@@ -232,6 +232,7 @@ struct Builders {
   }
 };
 Builders g_builders;
+thread_local int t_sync_build = 0;  // SegSyncBuild scopes alive on this thread
 
 std::shared_ptr<SegTable> build_table(uint64_t R, uint64_t C, uint32_t L) {
   auto tb = std::make_shared<SegTable>();
@@ -319,7 +320,7 @@ bool plan_100_segments(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64
   }
   if (!tb) {
     const auto key = std::make_tuple(R, C, seg_len);
-    if (RC > sync_slots) {  // walk it in the background; plan another form this time
+    if (RC > sync_slots && !t_sync_build) {  // walk it in the background; plan another form
       std::lock_guard<std::mutex> lk(g_seg_mu);
       if (g_building.insert(key).second)
         g_builders.th.emplace_back([key] { insert_table(key, build_table(std::get<0>(key), std::get<1>(key), std::get<2>(key))); });
@@ -355,6 +356,9 @@ bool plan_100_segments(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64
   *out = sp;
   return true;
 }
+
+SegSyncBuild::SegSyncBuild() { t_sync_build++; }
+SegSyncBuild::~SegSyncBuild() { t_sync_build--; }
 
 size_t seg_save_bytes(uint64_t inst, uint64_t S, const SegPlan &sp) {
   return size_t((inst * sp.nseg * S + 255) & ~uint64_t(255));
