@@ -315,7 +315,9 @@ void hip_marshal_set_profiling(int on);
 bool hip_marshal_set_tuning(const char *key, int64_t value);
 /* Measured autotuner for an M x N transpose of elem_bytes elements: the planner's cheapest
  * candidate routes and the best of each route family are run on `buf` (on `stream`), each
- * timed and followed by the inverse transpose, so `buf` ends bit-identical to how it started.
+ * timed and followed by the inverse transpose, so `buf` ends bit-identical to how it started
+ * (if a candidate fails part-way, a device snapshot taken first, when the buffer fits the
+ * scratch cap, is copied back, and true is returned).
  * The fastest becomes the plan for this shape (hip_transpose_full_es, batches and
  * hip_marshal_plan_transpose use it from then on); *stages receives its stage code. With the
  * knob "autotune" set to 1, hip_transpose_full_es does this by itself on the first call for
@@ -336,7 +338,8 @@ bool hip_marshal_autotune(hipStream_t stream, void *buf, size_t elem_bytes, int6
  * receives the stage code. The buffer is first copied to a device snapshot (when it fits the
  * scratch cap); if a candidate fails, the snapshot is copied back and true is returned.
  * Synchronizes the stream; not under graph capture. With the knob "autotune" set,
- * hip_marshal_batch_create tunes each class of its descriptors (once per class) this way. */
+ * hip_marshal_batch_create tunes each class of its AoS/SoA <-> ASTA descriptors (once per
+ * class) this way. */
 bool hip_marshal_autotune_desc(hipStream_t stream, const hip_marshal_desc *desc, int *form);
 int64_t hip_marshal_get_tuning(const char *key);
 /* The planner's choice for an M x N transpose of elem_bytes elements: tile factors a | M,

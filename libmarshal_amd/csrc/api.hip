@@ -204,6 +204,9 @@ struct LeaderEntry {
   uint32_t *list = nullptr;
   hipEvent_t ready = nullptr;
   size_t bytes = 0;
+  // set by a lookup outside capture that finds the search finished: under capture no event may
+  // be queried (that would invalidate the capture), so only such an entry is used there
+  bool done = false;
 };
 std::mutex g_lc_mu;
 std::map<std::tuple<int, uint64_t, uint64_t>, LeaderEntry> g_lc;
@@ -220,8 +223,10 @@ const uint32_t *cached_leaders(uint64_t R, uint64_t C, const PlPlan &pc, hipStre
   const auto key = std::make_tuple(dev, R, C);
   auto it = g_lc.find(key);
   if (it != g_lc.end()) {
-    if (capturing) return hipEventQuery(it->second.ready) == hipSuccess ? it->second.list : nullptr;
-    if (hipStreamWaitEvent(st, it->second.ready, 0) != hipSuccess) return nullptr;
+    if (capturing) return it->second.done ? it->second.list : nullptr;
+    if (!it->second.done && hipEventQuery(it->second.ready) == hipSuccess) it->second.done = true;
+    if (!it->second.done && hipStreamWaitEvent(st, it->second.ready, 0) != hipSuccess)
+      return nullptr;
     return it->second.list;
   }
   const size_t bytes = size_t(pc.cycles + 1) * 4;
@@ -1358,19 +1363,27 @@ static bool autotune_full(hipStream_t st, unsigned char *buf, size_t es, uint64_
     std::lock_guard<std::mutex> lk(p.mu);
     if (pool_grow(p, need)) return true;
   }
+  // A device snapshot of the buffer (when it fits the scratch cap), copied back if a candidate
+  // fails part-way; without one a failure is reported as leaving the buffer unrestored.
+  const uint64_t bytes = M * N * es;
+  void *snap = nullptr;
+  if (bytes <= (uint64_t(tuning("st_scratch_mb")) << 20) &&
+      hipMalloc(&snap, bytes) == hipSuccess &&
+      hipMemcpyAsync(snap, buf, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+    hipFree(snap);
+    snap = nullptr;
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  LM_CHECK_HIP(hipEventCreate(&e0));
-  LM_CHECK_HIP(hipEventCreate(&e1));
   float best_ms = 1e30f;
   size_t best = SIZE_MAX;
-  bool err = false;
+  bool err = hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess;
   const bool warm = double(M) * double(N) * double(es) < double(1u << 30);
   for (size_t i = 0; i < cand.size() && !err; i++) {
     if (lists[i].empty()) continue;
     float ms = 0;
     if (warm)  // first launches of a kernel form carry one-time costs: one untimed round trip
       err = run_stages(lists[i], st) || run_stages(inv, st);
-    err = hipEventRecord(e0, st) != hipSuccess || run_stages(lists[i], st) ||
+    err = err || hipEventRecord(e0, st) != hipSuccess || run_stages(lists[i], st) ||
           hipEventRecord(e1, st) != hipSuccess || run_stages(inv, st) ||
           hipEventSynchronize(e1) != hipSuccess ||
           hipEventElapsedTime(&ms, e0, e1) != hipSuccess || hipStreamSynchronize(st) != hipSuccess;
@@ -1383,8 +1396,19 @@ static bool autotune_full(hipStream_t st, unsigned char *buf, size_t es, uint64_
       best = i;
     }
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (err) {
+    hipStreamSynchronize(st);
+    if (snap) hipMemcpy(buf, snap, bytes, hipMemcpyDeviceToDevice);
+    fprintf(stderr, "[libmarshal] autotune %llux%llu: a candidate failed; the buffer %s\n",
+            (unsigned long long)M, (unsigned long long)N,
+            snap ? "was restored" : "was not restored (larger than the scratch cap)");
+  }
+  if (snap) {
+    hipStreamSynchronize(st);
+    hipFree(snap);
+  }
   if (err || best == SIZE_MAX) return true;
   Choice c = cand[best];
   c.cost = double(best_ms);
@@ -1435,6 +1459,11 @@ static bool autotune_step(hipStream_t st, const hip_marshal_desc &d, ClassChoice
   const uint64_t A = h / t, a = is010 ? (fwd ? t : w) : 0, B = is010 ? (fwd ? w : t) : 0;
   const uint64_t R = is010 ? 0 : (fwd ? w : h / t), C = is010 ? 0 : (fwd ? h / t : w), b = t;
   const uint64_t S = b * es;
+  if (is010 ? (a <= 1 || B <= 1 || A == 0) : (R <= 1 || C <= 1)) {  // nothing moves
+    *out = ClassChoice{};
+    if (route_code) *route_code = 0;
+    return false;
+  }
   if (!is010 && (S == 1 || S == 2 || S == 4 || S == 8 || S == 16) &&
       reinterpret_cast<uintptr_t>(buf) % S == 0) {
     // a single 100 over 1-16-byte super-elements is a full transpose: tune that shape
@@ -1468,7 +1497,10 @@ static bool autotune_step(hipStream_t st, const hip_marshal_desc &d, ClassChoice
   };
   std::vector<StageRec> inv;
   std::string itag;
-  if (!plan(inv, itag, true)) return true;
+  if (!plan(inv, itag, true)) {
+    fprintf(stderr, "[libmarshal] autotune: the inverse step cannot be planned\n");
+    return true;
+  }
   const uint64_t bytes = h * w * es;
   void *snap = nullptr;
   if (bytes <= (uint64_t(tuning("st_scratch_mb")) << 20) &&

@@ -616,9 +616,11 @@ struct hip_marshal_batch {
 };
 
 namespace lm {
-// Measures each shape class of the descriptors once (its first descriptor's buffer, which ends
-// bit-identical), on a private stream, before the batch is planned: transposes per exact
-// shape, AoS/SoA <-> ASTA steps per class (hip_marshal_autotune_desc).
+// Measures each shape class of the AoS/SoA <-> ASTA descriptors once (on its first
+// descriptor's buffer, which ends bit-identical), on a private stream, before the batch is
+// planned (hip_marshal_autotune_desc). Full transposes keep the cost model's routes: their
+// measured choices are per exact shape, which a batch of many shapes would spend most of its
+// creation on.
 bool autotune_classes(const hip_marshal_desc *descs, int64_t count) {
   hipStream_t st = nullptr;
   if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return true;
@@ -632,14 +634,14 @@ bool autotune_classes(const hip_marshal_desc *descs, int64_t count) {
     const size_t es = size_t(d.elem_bytes);
     ClassKey k{};
     switch (d.op) {
-      case HIP_MARSHAL_OP_TRANSPOSE: k = {0, int64_t(es), int64_t(h), int64_t(w), 0, 0}; break;
+      case HIP_MARSHAL_OP_TRANSPOSE: continue;
       case HIP_MARSHAL_OP_AOS_ASTA: k = class_key_010(es, h / t, t, w); break;
       case HIP_MARSHAL_OP_ASTA_AOS: k = class_key_010(es, h / t, w, t); break;
       case HIP_MARSHAL_OP_SOA_ASTA: k = class_key_0100(1, w, h / t, t * es); break;
       case HIP_MARSHAL_OP_ASTA_SOA: k = class_key_0100(1, h / t, w, t * es); break;
       default: continue;
     }
-    if (!done.insert(k).second || (d.op != HIP_MARSHAL_OP_TRANSPOSE && class_tuned(k))) continue;
+    if (!done.insert(k).second || class_tuned(k)) continue;
     int form = 0;
     err = hip_marshal_autotune_desc(st, &d, &form);
   }
