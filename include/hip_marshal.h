@@ -285,6 +285,8 @@ void hip_marshal_set_profiling(int on);
  * "copy_nt" (0/1: non-temporal source loads in the copy into scratch), "copy_nt_store_mb"
  * (copies of at least this many MiB store non-temporally; default 256), "oop_nt" (bit 0:
  * non-temporal loads, bit 1: non-temporal stores in the out-of-place transpose; default 2),
+ * "oop_order" (its tile order when tiles are at least as wide as tall: 0 row-major, k >= 1
+ * tile columns in groups of k walked row by row; default 2),
  * "pad_nt" (0/1: non-temporal loads and stores in the vector padding kernel),
  * "pl_min_bytes", "pl_max_order", "pl_min_walkers" (when a 100 / 0100 step takes the
  * cycle-leader form: super-elements of at least this many bytes, an order of R modulo R*C - 1
@@ -309,8 +311,8 @@ void hip_marshal_set_profiling(int on);
  * 388-465). The setter returns true on an unknown key or a value outside [min, 2^30] (min is
  * 0 for "st_scratch_mb", "st_rotate", "st_wstore", "st_skew", "copy_nt", "oop_nt", "pad_nt",
  * "oop_elem_tile_kb", "oop_pitched", "oop_min_rows", "oop_min_tiles", "copy_nt_store_mb",
- * "gather_row_bytes", "st_tile_rows", "pad_shape", "autotune", "seg_nt", "seg_xcd" and
- * "seg_force", 1 otherwise); the getter
+ * "gather_row_bytes", "st_tile_rows", "pad_shape", "autotune", "seg_nt", "seg_xcd",
+ * "seg_force" and "oop_order", 1 otherwise); the getter
  * returns 0 for an unknown key. */
 bool hip_marshal_set_tuning(const char *key, int64_t value);
 /* Measured autotuner for an M x N transpose of elem_bytes elements: the planner's cheapest

@@ -121,6 +121,7 @@ Knob g_knobs[] = {
     {"copy_blocks_per_cu", {64}},
     {"copy_nt_store_mb", {256}, 0},  // 0: non-temporal stores for every copy
     {"oop_nt", {2}, 0},
+    {"oop_order", {2}, 0},  // tile order of the out-of-place transpose: 0 row-major, k: groups of k tile columns
     {"pad_nt", {1}, 0},
     {"pl_min_bytes", {128}},
     {"pl_max_order", {64}},
