@@ -338,7 +338,7 @@ def autotune(buf, M, N, elem_bytes=None, stream=None):
 # Form codes hip_marshal_autotune_desc reports for AoS/SoA <-> ASTA classes.
 FORM_010 = {1: "010_BS-", 2: "010_OOP-", 3: "010_SPLIT-", 4: "010_RC-", 5: "010_PTTWAC-"}
 FORM_0100 = {11: "0100_CYC-", 12: "0100_SEG-", 13: "0100_OOP-", 14: "0100_RC-", 15: "0100_LDS-",
-             16: "0100_PTTWAC-"}
+             16: "0100_PTTWAC-", 17: "0100_CL-"}
 
 
 def autotune_desc(buf, op, elem_bytes, rows, cols, tile=0, stream=None):

@@ -14,7 +14,7 @@ OUT = os.path.join(HERE, "libmarshal_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["kernels_010.hip", "kernels_100.hip", "kernels_rc.hip", "kernels_pad.hip", "kernels_st.hip", "kernels_seg.hip", "batch.hip",
+SOURCES = ["kernels_010.hip", "kernels_100.hip", "kernels_rc.hip", "kernels_pad.hip", "kernels_st.hip", "kernels_seg.hip", "kernels_cl.hip", "batch.hip",
            "serve.hip", "api.hip", "noalloc.hip", "ktiming.hip"]
 
 FLAGS = [

@@ -121,6 +121,10 @@ Knob g_knobs[] = {
     {"copy_blocks_per_cu", {64}},
     {"copy_nt_store_mb", {256}, 0},  // 0: non-temporal stores for every copy
     {"oop_nt", {2}, 0},
+    {"cl_max_wgs", {16}, 0},  // workgroups per instance of the cluster-staged 100 (<= 16; 0: off)
+    {"cl_min_bytes", {256 << 10}, 0},  // its smallest instance
+    {"cl_min_inst", {64}, 0},  // and fewest instances
+    {"cl_spin", {1 << 22}},  // bound of its barrier wait, in polls (~2 s)
     {"oop_order", {2}, 0},  // tile order of the out-of-place transpose: 0 row-major, k: groups of k tile columns
     {"pad_nt", {1}, 0},
     {"pl_min_bytes", {128}},
@@ -546,6 +550,20 @@ void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64
                         buf != nullptr, &sg.sg))
     seg_ok = 4.0 * double(seg_save_bytes(inst, S, sg.sg)) <=
              double(inst) * double(R) * double(C) * double(S);
+  // Instances that fit a cluster of workgroups: staged on chip, one pass (not under
+  // restricted planning: the batch's grouped kernels have no cluster form).
+  StageRec cs = r;
+  cs.kind = StageKind::CL;
+  // Measured on MI355X it beats the other forms only with many instances of a few hundred KB
+  // (100 instances of 100 x 72 super-elements of 72 bytes: 30 us against 41 us segmented);
+  // single instances of 1-5 MB and many small ones ran 1.1-5x slower than the cycle-leader or
+  // segmented forms (one workgroup per CU holding a large share is latency-bound).
+  const bool cl_ok = !planning_restricted() && tuning("cl_max_wgs") > 0 &&
+                     inst >= uint64_t(tuning("cl_min_inst")) &&
+                     double(R) * double(C) * double(S) >= double(tuning("cl_min_bytes")) &&
+                     plan_100_cluster(inst, R, C, S, reinterpret_cast<uintptr_t>(buf),
+                                      uint64_t(std::min<int64_t>(16, tuning("cl_max_wgs"))),
+                                      &cs.cl);
   std::vector<StageRec> o;
   const bool oop_ok = allow_oop && push_oop(o, buf, es, inst, R, C, S);
   // A single 100 over short super-elements is a full R x C transpose of S-byte elements: the
@@ -578,9 +596,13 @@ void push_0100(std::vector<StageRec> &out, unsigned char *buf, size_t es, uint64
       g.kind = StageKind::PT100G;
       return emit(g, "0100_PTTWAC-");
     }
+    case kForm0100Cl: if (cl_ok) return emit(cs, "0100_CL-"); break;
     default: break;
   }
-  // The model's choice.
+  // The model's choice: the cluster form where it applies (one pass at streaming rates;
+  // "seg_force" keeps the segmented walk first).
+  if (seg_ok && tuning("seg_force")) return emit(sg, "0100_SEG-");
+  if (cl_ok) return emit(cs, "0100_CL-");
   if (cyc_ok && S >= uint64_t(tuning("pl_min_bytes")) &&
       cy.pc.cycles * inst * r.pt.nsub >= uint64_t(tuning("pl_min_walkers")))
     return emit(cy, "0100_CYC-");
@@ -904,6 +926,7 @@ size_t stages_scratch_bytes(const std::vector<StageRec> &stages) {
     if (r.kind == StageKind::ST) m = std::max<size_t>(m, r.stp.scratch_bytes);
     if (r.kind == StageKind::OOP) m = std::max<size_t>(m, size_t(oop_scratch_bytes(r)));
     if (r.kind == StageKind::SEG) m = std::max<size_t>(m, seg_save_bytes(r.d0, r.d3 * r.es, r.sg));
+    if (r.kind == StageKind::CL) m = std::max<size_t>(m, size_t(cluster_flag_words(r.d0)) * 4);
   }
   return m;
 }
@@ -1048,6 +1071,17 @@ static bool run_stages_impl(const std::vector<StageRec> &stages, hipStream_t st,
                                          st));
         break;
       }
+      case StageKind::CL: {
+        std::unique_ptr<Lease> lease;
+        uint32_t *flags = reinterpret_cast<uint32_t *>(scr);
+        if (!flags) {
+          lease.reset(new Lease(pool_here(), st));
+          flags = lease->get(cluster_flag_words(r.d0) * 4);
+        }
+        if (!flags) return true;
+        LM_CHECK_HIP(launch_100_cluster(r.buf, flags, r.d0, r.d1, r.d2, r.d3 * r.es, r.cl, st));
+        break;
+      }
       case StageKind::PT100G: {
         std::unique_ptr<Lease> lease;
         uint32_t *flags = reinterpret_cast<uint32_t *>(scr);
@@ -1166,6 +1200,15 @@ double stage_bw_tbs(const StageRec &r) {
       const double extra = 2.0 * double(r.sg.nseg) / double(std::max<uint64_t>(r.d1 * r.d2, 1));
       return std::min(5.5, 6.5 * useful) * unit / (1.0 + extra);
     }
+    case StageKind::CL: {
+      // A contiguous read of every instance and one scattered write of S-byte runs: a run at
+      // an offset aligned to a = gcd(S, 128) touches (S + 128 - a) / 128 cache lines.
+      const uint64_t S = r.d3 * es;
+      uint64_t a = 128;
+      while (a > 1 && S % a) a >>= 1;
+      const double useful = double(S) / double(S + 128 - a);
+      return 2.0 / (1.0 / 5.5 + 1.0 / std::min(5.5, 6.5 * useful));
+    }
     case StageKind::RC_ROW: return r.rc.ub >= 4 ? 3.0 : r.rc.ub == 2 ? 2.0 : 1.2;
     case StageKind::RC_COL1:
     case StageKind::RC_COL2: return r.rc.ub >= 4 ? 0.55 : r.rc.ub == 2 ? 0.4 : 0.3;
@@ -1202,6 +1245,7 @@ double stage_bytes(const StageRec &r) {
     case StageKind::PT100C:
     case StageKind::PT0100L:
     case StageKind::SEG:
+    case StageKind::CL:
       return double(r.d0) * double(r.d1) * double(r.d2) * double(r.d3) * double(r.es);
     case StageKind::RC_ROW:
     case StageKind::RC_COL1:
@@ -1445,6 +1489,7 @@ static const char *form_tag(int form) {
     case kForm0100Rc: return "0100_RC-";
     case kForm0100Lds: return "0100_LDS-";
     case kForm0100Bits: return "0100_PTTWAC-";
+    case kForm0100Cl: return "0100_CL-";
   }
   return "";
 }
@@ -1483,7 +1528,8 @@ static bool autotune_step(hipStream_t st, const hip_marshal_desc &d, ClassChoice
   } else {
     key = class_key_0100(1, R, C, S);
     cands = {{kForm0100Cyc, 0}, {kForm0100Seg, 16}, {kForm0100Seg, 32}, {kForm0100Seg, 64},
-             {kForm0100Oop, 0},  {kForm0100Rc, 0},   {kForm0100Lds, 0},  {kForm0100Bits, 0}};
+             {kForm0100Oop, 0},  {kForm0100Rc, 0},   {kForm0100Lds, 0},  {kForm0100Bits, 0},
+             {kForm0100Cl, 0}};
   }
   auto plan = [&](std::vector<StageRec> &v, std::string &tag, bool inverse) {
     if (is010) {
@@ -1688,6 +1734,7 @@ size_t hip_marshal_flag_bytes_0100(int64_t A, int64_t a, int64_t B, int64_t b,
 
 void hip_marshal_finalize(void) {
   free_segment_tables();
+  free_cluster_flags();
   {  // measured choices go with the rest of the library's state
     std::lock_guard<std::mutex> lk(g_class_mu);
     g_class.clear();

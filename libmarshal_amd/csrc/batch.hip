@@ -423,7 +423,7 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
       if (plan_desc(d, list, r2, true) &&
           (has({StageKind::ST}) ||
            (mbytes >= solo_min && has({StageKind::SQ, StageKind::OOP, StageKind::PT100C,
-                                       StageKind::SEG})))) {
+                                       StageKind::SEG, StageKind::CL})))) {
         b.solo.push_back(std::move(list));
         b.solo_bytes.push_back(double(d.rows) * double(d.cols) * double(d.elem_bytes));
         b.routes[i] = r2;
@@ -540,6 +540,7 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
         case StageKind::OOP:
         case StageKind::PT100C:
         case StageKind::SEG:
+        case StageKind::CL:
           fprintf(stderr, "[libmarshal] %s: stripe route in a batch\n", who);
           return false;
       }

@@ -249,10 +249,29 @@ hipError_t launch_100_segments(unsigned char *base, unsigned char *save, uint64_
                                uint32_t max_blocks, uint32_t nt, uint32_t xcd, hipStream_t st);
 void free_segment_tables();
 
+// ---- 100 / 0100 by workgroup clusters: one instance staged on chip ---------------------
+// G workgroups (one CU each) hold an instance of R x C super-elements of S >= 64 bytes in LDS
+// and registers, meet at a barrier and write it back transposed: one pass, in place, no bits.
+// Barrier words: a buffer kept per stream (every launch leaves it zero); under stream capture
+// the caller's `flags` (cluster_flag_words(inst) 32-bit words), cleared by the launch.
+struct ClPlan {
+  uint32_t G = 0;           // workgroups per instance (at most max_wgs)
+  uint32_t chunk = 0;       // instance bytes per workgroup
+  uint32_t ub = 16;         // piece bytes (16, 8 or 4)
+  uint32_t lds_pieces = 0;  // pieces per workgroup staged in LDS
+  size_t lds_bytes = 0;
+};
+bool plan_100_cluster(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64_t base_addr,
+                      uint64_t max_wgs, ClPlan *out);
+uint64_t cluster_flag_words(uint64_t inst);
+hipError_t launch_100_cluster(unsigned char *base, uint32_t *flags, uint64_t inst, uint64_t R,
+                              uint64_t C, uint64_t S, const ClPlan &pl, hipStream_t st);
+void free_cluster_flags();
+
 // ---- stage records (a route as a list of elementary stages) -----------------------------
 enum class M010 { Auto, BS, PTTWAC, Global };
 enum class StageKind : int {
-  BS010, PT010, PT100G, PT0100L, PT100C, RC_ROW, RC_COL1, RC_COL2, ST, SQ, OOP, SEG
+  BS010, PT010, PT100G, PT0100L, PT100C, RC_ROW, RC_COL1, RC_COL2, ST, SQ, OOP, SEG, CL
 };
 struct StageRec {
   StageKind kind;
@@ -268,6 +287,7 @@ struct StageRec {
   uint64_t pitch;           // OOP: scratch row pitch of the pitched form (0: plain form)
   PlPlan pc;                // PT100C: cycle count and order (pt: the walker shape)
   SegPlan sg;               // SEG: segment table and walker shape
+  ClPlan cl;                // CL: cluster shape
   int64_t grid;             // BS010: workgroups per CU chosen for its class (0: the knob)
 };
 // Forms a measured class choice can name (ClassChoice::form; 0 = the cost model decides).
@@ -275,7 +295,7 @@ enum : int {
   kFormModel = 0,
   kForm010Bs = 1, kForm010Oop = 2, kForm010Split = 3, kForm010Rc = 4, kForm010Cyc = 5,
   kForm0100Cyc = 11, kForm0100Seg = 12, kForm0100Oop = 13, kForm0100Rc = 14,
-  kForm0100Lds = 15, kForm0100Bits = 16,
+  kForm0100Lds = 15, kForm0100Bits = 16, kForm0100Cl = 17,
 };
 struct ClassChoice {
   int form = kFormModel;

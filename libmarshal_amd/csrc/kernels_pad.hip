@@ -169,7 +169,9 @@ __global__ __launch_bounds__(kPadThreads) void k_pad(PadParams p) {
 // two aligned 16-byte chunks that cover it and shifted into place (v_alignbyte). Vectors that
 // touch a row end, the pad, or the end of the buffer are done element by element.
 
-template <int ES, int NT, int VPL>
+// NTM: non-temporal loads and stores (a template argument, so that the two load forms never
+// share result registers: a select between them would make every load wait for the last).
+template <int ES, int NT, int VPL, bool NTM>
 __global__ __launch_bounds__(NT) void k_pad_vec(PadParams p) {
   constexpr uint64_t kPadVecTile = uint64_t(NT) * VPL;
   constexpr uint32_t kPadVecTileLog2 = __builtin_ctzll(kPadVecTile);
@@ -200,12 +202,13 @@ __global__ __launch_bounds__(NT) void k_pad_vec(PadParams p) {
   U128 wlo[VPL], whi[VPL];
   uint64_t wsb[VPL];
   bool win[VPL];
-  // Issue every load first (window chunks, or single elements near row ends and the pad),
-  // then shift the windows into place: the loads of all vectors overlap.
+  // Issue every window load first, then the single-element loads of the vectors near row
+  // ends and the pad, then shift the windows into place: the loads of all vectors overlap.
+  // Registers that receive window chunks get no other value before the join (an assignment
+  // there would have to wait for the load in flight).
 #pragma unroll
   for (int j = 0; j < VPL; j++) {
     const uint64_t vi = lo + threadIdx.x + uint64_t(j) * NT;
-    v[j].v = U128{0u, 0u, 0u, 0u};
     win[j] = false;
     wsb[j] = 0;
     if (vi >= hi) continue;
@@ -217,19 +220,22 @@ __global__ __launch_bounds__(NT) void k_pad_vec(PadParams p) {
     if (c + EPV <= X && d + EPV <= live && (sb & ~uint64_t(15)) + 32 <= bytes) {
       win[j] = true;
       wsb[j] = sb;
-      if (p.nt)
-        window_raw<true>(p.base, sb, wlo[j], whi[j]);
-      else
-        window_raw(p.base, sb, wlo[j], whi[j]);
-    } else {
+      window_raw<NTM>(p.base, sb, wlo[j], whi[j]);
+    }
+  }
 #pragma unroll
-      for (uint32_t e = 0; e < EPV; e++) {
-        const uint64_t dd = d + e;
-        if (dd >= live) break;
-        const uint64_t r2 = p.unpad ? p.div_x.div(dd) : p.div_pad.div(dd);
-        const uint64_t c2 = dd - r2 * rowlen;
-        if (c2 < X) v[j].e[e] = m[r2 * src_pitch + c2];
-      }
+  for (int j = 0; j < VPL; j++) {
+    const uint64_t vi = lo + threadIdx.x + uint64_t(j) * NT;
+    if (win[j] || vi >= hi) continue;
+    v[j].v = U128{0u, 0u, 0u, 0u};
+    const uint64_t d = vi * EPV;
+#pragma unroll
+    for (uint32_t e = 0; e < EPV; e++) {
+      const uint64_t dd = d + e;
+      if (dd >= live) break;
+      const uint64_t r2 = p.unpad ? p.div_x.div(dd) : p.div_pad.div(dd);
+      const uint64_t c2 = dd - r2 * rowlen;
+      if (c2 < X) v[j].e[e] = m[r2 * src_pitch + c2];
     }
   }
 #pragma unroll
@@ -247,7 +253,7 @@ __global__ __launch_bounds__(NT) void k_pad_vec(PadParams p) {
     const uint64_t vi = lo + threadIdx.x + uint64_t(j) * NT;
     if (vi >= hi) continue;
     if ((vi + 1) * 16 <= bytes) {
-      if (p.nt)
+      if (NTM)
         __builtin_nontemporal_store(v[j].v, reinterpret_cast<U128 *>(p.base + vi * 16));
       else
         *reinterpret_cast<U128 *>(p.base + vi * 16) = v[j].v;
@@ -263,8 +269,11 @@ namespace {
 template <int ES, int NT, int VPL>
 hipError_t launch_pad_vec_t(const PadParams &p, uint64_t nvec, hipStream_t st) {
   const uint64_t tile = uint64_t(NT) * VPL;
-  hipLaunchKernelGGL((k_pad_vec<ES, NT, VPL>), dim3(uint32_t((nvec + tile - 1) / tile)),
-                     dim3(NT), 0, st, p);
+  const dim3 grid(uint32_t((nvec + tile - 1) / tile));
+  if (p.nt)
+    hipLaunchKernelGGL((k_pad_vec<ES, NT, VPL, true>), grid, dim3(NT), 0, st, p);
+  else
+    hipLaunchKernelGGL((k_pad_vec<ES, NT, VPL, false>), grid, dim3(NT), 0, st, p);
   return hipGetLastError();
 }
 template <int ES>

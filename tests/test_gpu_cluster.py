@@ -1,0 +1,129 @@
+"""The cluster-staged form of 100 / 0100 steps (route tag "0100_CL-"): G workgroups hold one
+instance on chip (LDS and registers), meet at a barrier and write it back transposed. Checked
+bit-exactly against the oracle for every piece width (16-, 8- and 4-byte pieces), clusters of
+1 to 16 workgroups, more workgroups than the device holds at once (clusters formed over
+several rounds), a 4-byte-aligned base, repeated calls (the barrier words are cleared by each
+launch) and the inverse step restoring the input."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture
+def anywhere(hip):
+    """The cluster form wherever it can run (by default only many instances of 256 KB or more
+    take it)."""
+    keep = {k: hip.get_tuning(k) for k in ("cl_min_bytes", "cl_min_inst")}
+    hip.set_tuning("cl_min_bytes", 0)
+    hip.set_tuning("cl_min_inst", 1)
+    yield
+    for k, v in keep.items():
+        hip.set_tuning(k, v)
+
+
+CASES = [  # (instances, R, C, b, elem_bytes, base offset in elements)
+    (100, 100, 72, 18, 4, 0),    # 72-byte super-elements, 518 KB instances (config 1 stage 3)
+    (3, 97, 131, 5, 16, 0),      # 80-byte, 16-byte pieces, 1 MB instances
+    (2, 64, 33, 200, 8, 0),      # 1600-byte, 3.4 MB instances
+    (1, 30, 40, 1024, 4, 0),     # 4 KiB super-elements, 4.9 MB: 14 workgroups
+    (1, 128, 100, 45, 8, 0),     # 360-byte, 4.6 MB: 16 workgroups of 8-byte pieces
+    (5, 41, 29, 25, 4, 0),       # 100-byte: 4-byte pieces, one workgroup per instance
+    (600, 17, 23, 16, 4, 0),     # 600 small instances: more workgroups than CUs
+    (4, 60, 25, 24, 4, 1),       # 96-byte super-elements at a 4-byte-aligned base
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c)))
+def test_cluster_0100_matches_oracle(hip, orc, anywhere, case):
+    import torch
+    inst, R, C, b, es, off = case
+    n = inst * R * C * b
+    x = orc.tagged(n, es, seed=R * C + b + inst)
+    want = orc.op_0100(x, inst, R, C, b, es=es)
+    buf = torch.zeros((n + off) * es, dtype=torch.uint8, device="cuda")
+    buf[off * es:] = torch.from_numpy(x.view(np.uint8)).cuda()
+    d = buf[off * es:]
+    for rep in range(2):  # forward, back, forward again: the barrier words start from zero
+        assert hip.transpose_0100(d, inst, R, C, b, elem_bytes=es) is False
+        assert hip.last_route() == "0100_CL-", (case, hip.last_route())
+        got = buf.cpu().numpy()
+        assert np.array_equal(got[off * es:], want.view(np.uint8)), (case, rep)
+        assert not got[:off * es].any()
+        assert hip.transpose_0100(d, inst, C, R, b, elem_bytes=es) is False
+        assert hip.last_route() == "0100_CL-", (case, hip.last_route())
+        assert np.array_equal(buf.cpu().numpy()[off * es:], x.view(np.uint8)), (case, rep)
+    say("cluster", case, "ok")
+
+
+def test_cluster_limits_fall_back(hip, orc, anywhere):
+    """Instances beyond the cluster cap, super-elements under 64 bytes or not a multiple of 4
+    take the other forms, and the knob "cl_max_wgs" = 0 switches the form off."""
+    import torch
+    keep = hip.get_tuning("cl_max_wgs")
+    try:
+        for case, knob in (((1, 60, 40, 1024, 4), 16),   # 9.8 MB: more than 16 workgroups
+                           ((3, 40, 30, 15, 4), 16),     # 60-byte super-elements
+                           ((2, 37, 45, 65, 1), 16),     # 65-byte: no 4-byte pieces
+                           ((100, 100, 72, 18, 4), 0)):  # switched off
+            hip.set_tuning("cl_max_wgs", knob)
+            inst, R, C, b, es = case
+            n = inst * R * C * b
+            x = orc.tagged(n, es, seed=n)
+            d = torch.from_numpy(x.view(np.uint8).copy()).cuda()
+            assert hip.transpose_0100(d, inst, R, C, b, elem_bytes=es) is False
+            assert hip.last_route() != "0100_CL-", case
+            assert np.array_equal(d.cpu().numpy(), orc.op_0100(x, inst, R, C, b, es=es).view(np.uint8))
+    finally:
+        hip.set_tuning("cl_max_wgs", keep)
+
+
+def test_default_takes_cluster_form_for_many_mid_sized_instances(hip, orc):
+    """By default the form is taken for config 1's stage-3 shape (100 instances of 518 KB) and
+    not for a single 4 MB grid, where the segmented and cycle-leader forms measured faster."""
+    import torch
+    for case, want in (((100, 100, 72, 18, 4), True), ((1, 16, 1024, 64, 4), False),
+                       ((16, 64, 64, 64, 4), False)):
+        inst, R, C, b, es = case
+        n = inst * R * C * b
+        x = orc.tagged(n, es, seed=n + 1)
+        d = torch.from_numpy(x.view(np.uint8).copy()).cuda()
+        assert hip.transpose_0100(d, inst, R, C, b, elem_bytes=es) is False
+        assert (hip.last_route() == "0100_CL-") == want, (case, hip.last_route())
+        assert np.array_equal(d.cpu().numpy(), orc.op_0100(x, inst, R, C, b, es=es).view(np.uint8))
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_cluster_step_in_a_batch(hip, orc, anywhere, use_graph):
+    """A 4 MB SoA -> ASTA conversion runs on a batch's side stream in the cluster form; under
+    graph capture its barrier words come from the graph's scratch and are cleared by the
+    launch on every replay. Each replay applies the step again: checked after every one."""
+    import torch
+    from libmarshal_amd import workloads as wl
+    es, h, w, t = 4, 64 * 256, 64, 64  # 100 over a 64 x 256 grid of 256-byte super-elements
+    x = orc.tagged(h * w, es, seed=31)
+    d = torch.from_numpy(x.view(np.uint8).copy()).cuda()
+    small = orc.tagged(32 * 48, es, seed=32)  # a grouped item beside it
+    ds = torch.from_numpy(small.view(np.uint8).copy()).cuda()
+    batch = hip.Batch([(d, wl.OP_SOA_ASTA, es, h, w, t), (ds, wl.OP_TRANSPOSE, es, 32, 48, 0)],
+                      use_graph=use_graph)
+    try:
+        assert batch.route(0).endswith("0100_CL-"), batch.route(0)
+        want, want_s = x, small
+        shape = (32, 48)
+        for rep in range(3):
+            assert batch.run() is False
+            torch.cuda.synchronize()
+            want = orc.op_100(want, w, h // t, t, es=es)
+            assert np.array_equal(d.cpu().numpy(), want.view(np.uint8)), rep
+            want_s = orc.op_transpose(want_s, shape[0], shape[1], es=es)  # same descriptor
+            assert np.array_equal(ds.cpu().numpy(), want_s.view(np.uint8)), rep
+    finally:
+        batch.close()

@@ -87,8 +87,9 @@ def test_soa_asta_any_super_element_size(hip, orc, case):
     S = t * es
     if S in (1, 2, 4, 8, 16):
         assert route.startswith("0100_FULL:"), route
-    else:  # through scratch, or for long super-elements the one-pass segmented walk
-        assert route in ("0100_OOP-", "0100_SEG-"), route
+    else:  # through scratch, on chip in a cluster of workgroups (grids of a few MB), or for
+        # long super-elements the one-pass segmented walk
+        assert route in ("0100_OOP-", "0100_SEG-", "0100_CL-"), route
     assert hip.asta_soa(d, h, w, t, elem_bytes=es) is False
     assert np.array_equal(host(d), x), (case, hip.last_route())
 

@@ -20,8 +20,9 @@ def say(*a):
 def seg_only(hip):
     """The segmented walk wherever it can run (no through-scratch or cycle-leader forms)."""
     keep = {k: hip.get_tuning(k) for k in ("st_scratch_mb", "pl_max_order", "seg_len",
-                                             "seg_force")}
+                                             "seg_force", "cl_max_wgs")}
     hip.set_tuning("st_scratch_mb", 0)
+    hip.set_tuning("cl_max_wgs", 0)
     hip.set_tuning("pl_max_order", 1)
     hip.set_tuning("seg_force", 1)  # also where the model prefers another form (narrow units)
     yield
@@ -62,8 +63,9 @@ def test_segmented_0100_matches_oracle(hip, orc, seg_only, case):
 
 
 def test_config1_three_stage_route_segments(hip, orc):
-    """1800 x 7200 int32 with a = 18, b = 72 and back by the 3-stage route: its 100 / 0100
-    stages (288- and 72-byte super-elements) take the segmented walk."""
+    """1800 x 7200 int32 with a = 18, b = 72 and back by the 3-stage route: its 100 stage
+    (288-byte super-elements, one 51.84 MB grid) takes the segmented walk, its 0100 stage
+    (100 instances of 72-byte super-elements) the cluster-staged form."""
     import torch
     M, N = 1800, 7200
     x = (orc.splitmix64(77, M * N) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
