@@ -306,13 +306,20 @@ void hip_marshal_set_profiling(int on);
  * many bytes, default 64), "seg_max_slots" (grids of at most this many slots, default 2^24),
  * "seg_blocks_per_cu", "seg_nt" (bit 0 / 1: non-temporal loads / stores, default 0),
  * "seg_xcd" (0/1: keep each XCD on a contiguous run of segments, default 1) and "seg_force"
- * (0/1: take it wherever it can run, default 0).
+ * (0/1: take it wherever it can run, default 0); for the cluster-staged form of 100 / 0100
+ * steps (route tag "0100_CL-": a few workgroups hold each instance in LDS and registers, meet
+ * at a barrier and write it back transposed, one pass, for super-elements of at least 64
+ * bytes) "cl_max_wgs" (workgroups per instance, at most 16; 0 switches the form off),
+ * "cl_min_bytes" (smallest instance, default 256 KiB), "cl_min_inst" (fewest instances,
+ * default 64) and "cl_spin" (bound of the barrier wait in polls; a cluster that reaches it
+ * leaves its instance unchanged).
  * Replaces the reference's hard-coded work-group tables (src/cl_marshal.cc:165-231,
  * 388-465). The setter returns true on an unknown key or a value outside [min, 2^30] (min is
  * 0 for "st_scratch_mb", "st_rotate", "st_wstore", "st_skew", "copy_nt", "oop_nt", "pad_nt",
  * "oop_elem_tile_kb", "oop_pitched", "oop_min_rows", "oop_min_tiles", "copy_nt_store_mb",
  * "gather_row_bytes", "st_tile_rows", "pad_shape", "autotune", "seg_nt", "seg_xcd",
- * "seg_force" and "oop_order", 1 otherwise); the getter
+ * "seg_force", "oop_order", "cl_max_wgs", "cl_min_bytes" and "cl_min_inst", 1 otherwise); the
+ * getter
  * returns 0 for an unknown key. */
 bool hip_marshal_set_tuning(const char *key, int64_t value);
 /* Measured autotuner for an M x N transpose of elem_bytes elements: the planner's cheapest
