@@ -260,9 +260,15 @@ struct ClPlan {
   uint32_t ub = 16;         // piece bytes (16, 8 or 4)
   uint32_t lds_pieces = 0;  // pieces per workgroup staged in LDS
   size_t lds_bytes = 0;
+  // fused form (fes > 0): a 010 over tiles of fa x fb elements of fes bytes, then the 0100
+  // over the [R][fb] grid of fa-element super-elements, in one pass (LDS only)
+  uint32_t fa = 0, fb = 0, fes = 0, flv = 0;
 };
 bool plan_100_cluster(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64_t base_addr,
                       uint64_t max_wgs, ClPlan *out);
+// The fused form for `inst` instances of R tiles of a x b elements of es (4 or 8) bytes.
+bool plan_cluster_fused(uint64_t inst, uint64_t R, uint64_t a, uint64_t b, size_t es,
+                        uint64_t base_addr, uint64_t max_wgs, ClPlan *out);
 uint64_t cluster_flag_words(uint64_t inst);
 hipError_t launch_100_cluster(unsigned char *base, uint32_t *flags, uint64_t inst, uint64_t R,
                               uint64_t C, uint64_t S, const ClPlan &pl, hipStream_t st);

@@ -16,6 +16,13 @@
 // the wait is bounded: a cluster whose members have not all arrived after the bound gives up
 // as a whole (no member of it stores anything, so its instance is left unchanged).
 //
+// Fused form (FES > 0): the instance is [R][a][b] elements of FES bytes, R tiles of a x b, and
+// the step is the 010 of every tile followed by the 0100 of the [R][b] grid of a-element
+// super-elements, i.e. [R][a][b] -> [b][R][a] (the reference's 3-stage route runs these as its
+// stages 2 and 3). Members hold whole tiles in LDS only; a tile's elements are written to LDS
+// already transposed ([b][a]), so what LDS holds is the 010's output, and the store is the
+// 0100's. One pass instead of two.
+//
 // The launch leaves its barrier words zero again (the last member to leave a cluster's
 // barrier clears that cluster's words, the last workgroup to finish clears the counters), so
 // a buffer kept for one stream needs clearing only once, when it is allocated.
@@ -24,6 +31,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace lm {
@@ -46,7 +54,8 @@ struct ClParams {
   uint32_t chunk;       // bytes per workgroup, a multiple of the piece size
   uint32_t lds_pieces;  // pieces staged in LDS per workgroup (the rest in registers)
   uint32_t spin;        // bound of the barrier wait, in polls
-  FastDiv32 div_S, div_C, div_G;
+  uint32_t fa, fb;      // fused form: tiles of fa x fb elements (the 010 before the 0100)
+  FastDiv32 div_S, div_C, div_G, div_ab, div_b;
 };
 
 // Instance byte offset o of a piece -> its destination offset: super-element u = o / S at row
@@ -74,7 +83,45 @@ __device__ __forceinline__ void cl_store(const ClParams &p, unsigned char *ib,
   }
 }
 
-template <int UB, int NT>
+// Fused form's load: the chunk (whole tiles of a x b elements) arrives as FLV-byte vectors
+// (each inside one tile row, since b * FES is a multiple of FLV) and every element is written
+// to LDS at its place in the transposed tile: tile t, element (i, j) -> t * a * b + j * a + i.
+template <int FES, int FLV, int NT>
+__device__ __forceinline__ void cl_load_fused(const ClParams &p, const unsigned char *src,
+                                              unsigned char *lds, uint32_t bytes) {
+  using E = typename Elem<FES>::T;
+  using LV = typename Elem<FLV>::T;
+  constexpr uint32_t EPV = FLV / FES;
+  const uint32_t nv = bytes / FLV;
+  const LV *sv = reinterpret_cast<const LV *>(src);
+  E *L = reinterpret_cast<E *>(lds);
+  const uint32_t a = p.fa, ab = p.fa * p.fb;
+  constexpr int KB = 8;
+  for (uint32_t i0 = threadIdx.x; i0 < nv; i0 += KB * NT) {
+    union {
+      LV v;
+      E e[EPV];
+    } t[KB];
+#pragma unroll
+    for (int k = 0; k < KB; k++) {
+      const uint32_t i = i0 + uint32_t(k) * NT;
+      if (i < nv) t[k].v = sv[i];
+    }
+#pragma unroll
+    for (int k = 0; k < KB; k++) {
+      const uint32_t i = i0 + uint32_t(k) * NT;
+      if (i >= nv) continue;
+      const uint32_t e = i * EPV;  // first element of the vector within the chunk
+      const uint32_t tl = p.div_ab.div(e), rem = e - tl * ab;
+      const uint32_t ri = p.div_b.div(rem), cj = rem - ri * p.fb;
+      const uint32_t o = tl * ab + cj * a + ri;
+#pragma unroll
+      for (uint32_t x = 0; x < EPV; x++) L[o + x * a] = t[k].e[x];
+    }
+  }
+}
+
+template <int UB, int NT, int FES, int FLV>
 __global__ __launch_bounds__(NT) void k_100_cluster(ClParams p) {
   using V = typename Elem<UB>::T;
   extern __shared__ __align__(16) unsigned char lds[];
@@ -94,16 +141,21 @@ __global__ __launch_bounds__(NT) void k_100_cluster(ClParams p) {
   V *L = reinterpret_cast<V *>(lds);
 
   // Load: the registers' share first (no waits), then the LDS share in batches of KB pieces
-  // per lane, every load of a batch issued before any of them is waited for.
-  constexpr int NR = cl_regs(UB);
+  // per lane, every load of a batch issued before any of them is waited for. (Fused form:
+  // everything in LDS, transposed per tile on the way in.)
+  constexpr int NR = FES ? 1 : cl_regs(UB);
   V reg[NR];
+  if (FES) {
+    cl_load_fused<FES ? FES : 4, FLV ? FLV : 4, NT>(p, ib + lo, lds, hi - lo);
+  } else {
 #pragma unroll
-  for (int k = 0; k < NR; k++) {
-    const uint32_t i = nl + uint32_t(k) * NT + threadIdx.x;
-    if (i < np) reg[k] = src[i];
+    for (int k = 0; k < NR; k++) {
+      const uint32_t i = nl + uint32_t(k) * NT + threadIdx.x;
+      if (i < np) reg[k] = src[i];
+    }
   }
   constexpr int KB = 8;
-  for (uint32_t i0 = threadIdx.x; i0 < nl; i0 += KB * NT) {
+  for (uint32_t i0 = threadIdx.x; !FES && i0 < nl; i0 += KB * NT) {
     V t[KB];
 #pragma unroll
     for (int k = 0; k < KB; k++) {
@@ -149,7 +201,7 @@ __global__ __launch_bounds__(NT) void k_100_cluster(ClParams p) {
     }
   }
   __syncthreads();
-  if (s_go) cl_store<UB, NT>(p, ib, L, reg, lo, nl, np);
+  if (s_go) cl_store<UB, NT>(p, ib, L, reg, lo, nl, FES ? nl : np);
   // the last workgroup of the launch clears the counters (every workgroup took its id first)
   if (threadIdx.x == 0 &&
       __hip_atomic_fetch_add(p.flags + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 ==
@@ -194,6 +246,36 @@ bool plan_100_cluster(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64_
   out->ub = ub;
   out->lds_pieces = uint32_t(std::min<uint64_t>(lds, chunk) / ub);
   out->lds_bytes = size_t(out->lds_pieces) * ub;
+  return true;
+}
+
+bool plan_cluster_fused(uint64_t inst, uint64_t R, uint64_t a, uint64_t b, size_t es,
+                        uint64_t base_addr, uint64_t max_wgs, ClPlan *out) {
+  if (inst == 0 || R <= 1 || a <= 1 || b <= 1 || !(es == 4 || es == 8)) return false;
+  const uint64_t S = a * es, row = b * es, tile = a * b * es;
+  if (S < 64 || base_addr % es) return false;
+  const uint32_t ub = (S % 16 == 0 && base_addr % 16 == 0) ? 16
+                      : (S % 8 == 0 && base_addr % 8 == 0) ? 8 : 4;
+  const uint32_t lv = (row % 16 == 0 && base_addr % 16 == 0) ? 16
+                      : (row % 8 == 0 && base_addr % 8 == 0) ? 8 : 0;
+  if (!lv || ub < es) return false;
+  const double ib = double(R) * double(tile);
+  if (ib >= double(1u << 31)) return false;
+  const uint64_t lds = cl_lds_bytes();
+  if (lds == 0 || tile > lds) return false;
+  const uint64_t per = lds / tile;               // tiles a member can hold
+  const uint64_t G = (R + per - 1) / per;
+  const uint64_t tpm = (R + G - 1) / G;          // tiles per member, every chunk non-empty
+  if (G > max_wgs || inst * G >= (1ull << 31)) return false;
+  out->G = uint32_t(G);
+  out->chunk = uint32_t(tpm * tile);
+  out->ub = ub;
+  out->lds_pieces = uint32_t(tpm * tile / ub);
+  out->lds_bytes = size_t(tpm * tile);
+  out->fa = uint32_t(a);
+  out->fb = uint32_t(b);
+  out->fes = uint32_t(es);
+  out->flv = lv;
   return true;
 }
 
@@ -288,6 +370,14 @@ hipError_t launch_100_cluster(unsigned char *base, uint32_t *flags, uint64_t ins
   p.div_S = FastDiv32(uint32_t(S));
   p.div_C = FastDiv32(uint32_t(C));
   p.div_G = FastDiv32(pl.G);
+  p.fa = pl.fa;
+  p.fb = pl.fb;
+  if (pl.fes) {  // fused: the instance is R tiles of fa x fb elements, units of fa elements
+    if (uint64_t(pl.fa) * pl.fes != S || C != pl.fb || pl.chunk % (pl.fa * pl.fb * pl.fes))
+      return hipErrorInvalidValue;
+    p.div_ab = FastDiv32(pl.fa * pl.fb);
+    p.div_b = FastDiv32(pl.fb);
+  }
   auto go = [&](auto kern) {
     if (pl.lds_bytes > 65536) {
       hipError_t a = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
@@ -298,10 +388,26 @@ hipError_t launch_100_cluster(unsigned char *base, uint32_t *flags, uint64_t ins
     hipLaunchKernelGGL(kern, dim3(uint32_t(p.wgs)), dim3(kClThreads), pl.lds_bytes, st, p);
     return hipGetLastError();
   };
+  auto fused = [&](auto ubt) -> hipError_t {
+    constexpr int U = decltype(ubt)::value;
+    if (pl.fes == 4 && pl.flv == 16) return go(k_100_cluster<U, kClThreads, 4, 16>);
+    if (pl.fes == 4 && pl.flv == 8) return go(k_100_cluster<U, kClThreads, 4, 8>);
+    if (pl.fes == 8 && pl.flv == 16) return go(k_100_cluster<U, kClThreads, 8, 16>);
+    if (pl.fes == 8 && pl.flv == 8) return go(k_100_cluster<U, kClThreads, 8, 8>);
+    return hipErrorInvalidValue;
+  };
+  if (pl.fes) {
+    switch (pl.ub) {
+      case 16: return fused(std::integral_constant<int, 16>{});
+      case 8: return fused(std::integral_constant<int, 8>{});
+      case 4: return fused(std::integral_constant<int, 4>{});
+    }
+    return hipErrorInvalidValue;
+  }
   switch (pl.ub) {
-    case 16: return go(k_100_cluster<16, kClThreads>);
-    case 8: return go(k_100_cluster<8, kClThreads>);
-    case 4: return go(k_100_cluster<4, kClThreads>);
+    case 16: return go(k_100_cluster<16, kClThreads, 0, 0>);
+    case 8: return go(k_100_cluster<8, kClThreads, 0, 0>);
+    case 4: return go(k_100_cluster<4, kClThreads, 0, 0>);
   }
   return hipErrorInvalidValue;
 }

@@ -127,3 +127,42 @@ def test_cluster_step_in_a_batch(hip, orc, anywhere, use_graph):
             assert np.array_equal(ds.cpu().numpy(), want_s.view(np.uint8)), rep
     finally:
         batch.close()
+
+
+FUSED = [  # (M, N, a, b, elem_bytes): the 3-stage route; its stages 2 and 3 fuse
+    (1800, 7200, 18, 72, 4),     # config 1: 72-byte super-elements, 16-byte tile-row vectors
+    (7200, 1800, 72, 18, 4),     # and back: 288-byte super-elements, 8-byte vectors
+    (1800, 7200, 180, 180, 4),   # one 130 KB tile per workgroup
+    (1200, 2400, 24, 48, 8),     # 8-byte elements
+    (960, 1920, 16, 24, 8),      # 128-byte super-elements, 192-byte tile rows
+]
+
+
+@pytest.mark.parametrize("case", FUSED, ids=lambda c: "x".join(map(str, c)))
+def test_three_stage_route_fuses_stages_2_and_3(hip, orc, case):
+    """The 3-stage route's 010 (stage 2) and 0100 (stage 3) run as one cluster-staged pass:
+    no 010 kernel is launched, and the result equals the oracle's transpose; with the fusion
+    switched off the same route runs both stages and gives the same bytes."""
+    import torch
+    M, N, a, b, es = case
+    x = orc.tagged(M * N, es, seed=M + N + a)
+    want = orc.op_transpose(x, M, N, es=es)
+    keep = hip.get_tuning("cl_fuse")
+    try:
+        for fuse in (1, 0):
+            hip.set_tuning("cl_fuse", fuse)
+            d = torch.from_numpy(x.view(np.uint8).copy()).cuda()
+            torch.cuda.synchronize()
+            hip.set_kernel_timing(True)
+            assert hip.transpose(d, M // a, a, N // b, b, 1, 3, elem_bytes=es) is False
+            names = {k["name"] for k in hip.kernel_times()}
+            hip.set_kernel_timing(False)
+            assert hip.last_route().endswith("3.1-"), hip.last_route()
+            assert np.array_equal(d.cpu().numpy(), want.view(np.uint8)), (case, fuse, names)
+            if fuse:
+                assert "k_100_cluster" in names and "k_010_bs" not in names, names
+            else:
+                assert "k_010_bs" in names, names
+            say("fused", case, fuse, sorted(names))
+    finally:
+        hip.set_tuning("cl_fuse", keep)
