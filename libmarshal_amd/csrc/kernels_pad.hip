@@ -229,13 +229,17 @@ __global__ __launch_bounds__(NT) void k_pad_vec(PadParams p) {
     if (win[j] || vi >= hi) continue;
     v[j].v = U128{0u, 0u, 0u, 0u};
     const uint64_t d = vi * EPV;
+    // row and column of the first element, then stepped along (one division per vector)
+    uint64_t r2 = p.unpad ? p.div_x.div(d) : p.div_pad.div(d);
+    uint64_t c2 = d - r2 * rowlen;
 #pragma unroll
     for (uint32_t e = 0; e < EPV; e++) {
-      const uint64_t dd = d + e;
-      if (dd >= live) break;
-      const uint64_t r2 = p.unpad ? p.div_x.div(dd) : p.div_pad.div(dd);
-      const uint64_t c2 = dd - r2 * rowlen;
+      if (d + e >= live) break;
       if (c2 < X) v[j].e[e] = m[r2 * src_pitch + c2];
+      if (++c2 == rowlen) {
+        c2 = 0;
+        r2++;
+      }
     }
   }
 #pragma unroll
