@@ -440,7 +440,11 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
       }
     }
   } else if (VIN) {  // one 16-byte window per lane (TC >= EPV)
-    constexpr uint32_t KB = 2048 / NT > 0 ? 2048 / NT : 1;  // windows in flight per lane
+    // Two windows per batch: with four the kernel needed 73-77 VGPRs (three 512-lane
+    // workgroups per CU), with two 39-48 (four, the LDS limit for 32 KiB tiles). Measured:
+    // f32 through-scratch transposes +3-7 % (2500 x 5100: 4.83 -> 5.19 TB/s), 3a's f64 stripes
+    // unchanged, u8 within 1 %.
+    constexpr uint32_t KB = 2;
     const uint32_t vl = p.tc_log2 - EPV_LOG2;
     const uint32_t total = TRL << vl;
     for (uint32_t i0 = threadIdx.x; i0 < total; i0 += KB * NT) {
