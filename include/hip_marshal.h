@@ -33,9 +33,12 @@
  * after the other. Under stream capture the cross-stream wait is not recorded, so a
  * captured graph must not run concurrently with other pool users on the same device; the
  * pool does not grow under capture (reserve it first), and a pool buffer handed out under
- * capture is kept, not freed, when the pool later grows. hip_marshal_finalize frees the pool,
- * the cycle-leader lists and the segment tables: graphs that captured library calls must not
- * be replayed after it.
+ * capture is kept, not freed, when the pool later grows. Steps in the cluster-staged form
+ * (route tag "0100_CL-") keep their barrier words in a small buffer per stream (16 KiB or
+ * more, allocated on the first such step outside capture; every launch leaves it zeroed);
+ * under capture they use pool scratch instead. hip_marshal_finalize frees the pool, the
+ * cycle-leader lists, the segment tables and those buffers: graphs that captured library
+ * calls must not be replayed after it.
  */
 #ifndef LIBMARSHAL_AMD_HIP_MARSHAL_H_
 #define LIBMARSHAL_AMD_HIP_MARSHAL_H_
