@@ -166,3 +166,27 @@ def test_three_stage_route_fuses_stages_2_and_3(hip, orc, case):
             say("fused", case, fuse, sorted(names))
     finally:
         hip.set_tuning("cl_fuse", keep)
+
+
+@pytest.mark.parametrize("case", [
+    (1800, 7200, 18, 72, 4, 1),   # base 4-byte aligned only: no 8-byte vectors, not fused
+    (600, 1200, 6, 12, 4, 0),     # 24-byte super-elements (< 64): not fused
+    (1024, 2048, 8, 16, 8, 0),    # 64-byte super-elements, 128-byte tile rows
+    (2000, 3000, 40, 60, 4, 2),   # base 8-byte aligned: 8-byte vectors
+])
+def test_three_stage_route_fusion_edges(hip, orc, case):
+    """Bases of any element alignment and super-elements around the 64-byte limit: the route
+    fuses where it can and every result equals the oracle's transpose, with the bytes around
+    the matrix untouched."""
+    import torch
+    M, N, a, b, es, off = case
+    x = orc.tagged(M * N, es, seed=M * 3 + b)
+    want = orc.op_transpose(x, M, N, es=es)
+    buf = torch.zeros((M * N + off + 1) * es, dtype=torch.uint8, device="cuda")
+    buf[off * es:(off + M * N) * es] = torch.from_numpy(x.view(np.uint8)).cuda()
+    d = buf[off * es:(off + M * N) * es]
+    assert hip.transpose(d, M // a, a, N // b, b, 1, 3, elem_bytes=es) is False
+    got = buf.cpu().numpy()
+    assert np.array_equal(got[off * es:(off + M * N) * es], want.view(np.uint8)), case
+    assert not got[:off * es].any() and not got[(off + M * N) * es:].any()
+    say("fusion edge", case, hip.last_route())
