@@ -41,6 +41,8 @@ def parse():
     p.add_argument("--profile-st", action="store_true",
                    help="only a 1 GiB calibration copy and one 98304 x 131071 f64 transpose and "
                         "its inverse (for HBM byte counters of the route's kernels)")
+    p.add_argument("--profile-st-u8", action="store_true",
+                   help="the same for the 250007 x 199999 u8 transpose and its inverse")
     p.add_argument("--profile-large", action="store_true",
                    help="only run the two large prime-dimension transposes (for kernel traces)")
     p.add_argument("--profile-soa", type=int, default=0,
@@ -362,24 +364,32 @@ def main():
 
     torch.cuda.set_device(0)
     lm.set_profiling(False)  # asynchronous calls: no per-call synchronization
-    if args.profile_st:
+    if args.profile_st or args.profile_st_u8:
         c1 = torch.rand(1 << 28, device="cuda")  # 1 GiB
         c2 = torch.empty_like(c1)
         c2.copy_(c1)  # the calibration kernel: reads and writes exactly 1 GiB
         del c1, c2
         torch.cuda.empty_cache()
-        M, N = 98304, 131071
-        x = torch.empty(M * N, dtype=torch.float64, device="cuda")
-        for s in range(0, M * N, 1 << 30):
-            torch.rand(min(M * N, s + (1 << 30)) - s, dtype=torch.float64, out=x[s:s + (1 << 30)])
+        if args.profile_st:
+            M, N, dt = 98304, 131071, torch.float64
+        else:
+            M, N, dt = 250007, 199999, torch.uint8
+        x = torch.empty(M * N, dtype=dt, device="cuda")
+        step = 1 << 30
+        for s0 in range(0, M * N, step):
+            s1 = min(M * N, s0 + step)
+            if dt == torch.uint8:
+                x[s0:s1].random_(0, 256)
+            else:
+                torch.rand(s1 - s0, dtype=dt, out=x[s0:s1])
         if lm.transpose_full(x, M, N):
             raise RuntimeError("transpose failed")
         r1 = lm.last_route()
         if lm.transpose_full(x, N, M):
             raise RuntimeError("transpose failed")
         torch.cuda.synchronize()
-        print(f"profile run: 1 GiB copy, {M} x {N} f64 transpose ({r1}) and back "
-              f"({lm.last_route()})", flush=True)
+        print(f"profile run: 1 GiB copy, {M} x {N} {str(dt).split('.')[-1]} transpose ({r1}) "
+              f"and back ({lm.last_route()})", flush=True)
         return
     if args.profile_batch:
         print(json.dumps({"profile_batch": extra_batch(lm)}), flush=True)
