@@ -297,19 +297,19 @@ std::vector<std::pair<int, uint32_t *>> g_clf_retired;
 void free_cluster_flags() {
   std::lock_guard<std::mutex> lk(g_clf_mu);
   int cur = 0;
-  hipGetDevice(&cur);
+  (void)hipGetDevice(&cur);
   for (auto &kv : g_clf) {
-    hipSetDevice(kv.first.first);
-    hipDeviceSynchronize();
-    hipFree(kv.second.ptr);
+    (void)hipSetDevice(kv.first.first);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(kv.second.ptr);
   }
   for (auto &r : g_clf_retired) {
-    hipSetDevice(r.first);
-    hipFree(r.second);
+    (void)hipSetDevice(r.first);
+    (void)hipFree(r.second);
   }
   g_clf.clear();
   g_clf_retired.clear();
-  hipSetDevice(cur);
+  (void)hipSetDevice(cur);
 }
 
 // The stream's kept buffer of at least `words` zero words (cleared here when new), or null
@@ -330,7 +330,7 @@ static uint32_t *stream_cluster_flags(uint64_t words, hipStream_t st) {
     return nullptr;
   }
   if (launch_fill_u32(p, want, 0, st) != hipSuccess) {
-    hipFree(p);
+    (void)hipFree(p);
     return nullptr;
   }
   if (f.ptr) g_clf_retired.push_back({dev, f.ptr});

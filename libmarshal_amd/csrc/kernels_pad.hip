@@ -232,7 +232,9 @@ __global__ __launch_bounds__(NT) void k_pad_vec(PadParams p) {
     // row and column of the first element, then stepped along (one division per vector)
     uint64_t r2 = p.unpad ? p.div_x.div(d) : p.div_pad.div(d);
     uint64_t c2 = d - r2 * rowlen;
-#pragma unroll
+    // A loop, not unrolled: unrolled, the u8 form needs 124 VGPRs instead of 66 (this path
+    // runs only at row ends and the pad).
+#pragma unroll 1
     for (uint32_t e = 0; e < EPV; e++) {
       if (d + e >= live) break;
       if (c2 < X) v[j].e[e] = m[r2 * src_pitch + c2];

@@ -356,7 +356,6 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
   constexpr uint32_t EPV = 16 / ES;  // elements per 16-byte vector
   constexpr uint32_t EPV_LOG2 = ES == 1 ? 4 : ES == 2 ? 3 : ES == 4 ? 2 : ES == 8 ? 1 : 0;
   constexpr int SW = VIN ? int(EPV_LOG2) : 0;
-  constexpr bool VOUT = OUT == 1;
   constexpr uint32_t KOFS = OUT == 2 ? EPV : 0;  // LDS element index of source row r0
   extern __shared__ __align__(16) unsigned char lds[];
   if (OUT == 2 && blockIdx.x >= p.per_xcd * 8) {
