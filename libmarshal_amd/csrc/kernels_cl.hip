@@ -4,8 +4,9 @@
 // written once, in place, with no cycle following, no visited bits and no side buffer.
 //
 // Not a reference algorithm: the reference moves such grids by cycle following with visited
-// bits (src/cl/cl_aos_asta.cl:397-460 for 0100, :471-560 for 100). It is used for super-
-// elements of at least 64 bytes, where each super-element is one contiguous destination run.
+// bits (_transpose_100, src/cl/cl_aos_asta.cl:348-569, behind transpose_100 and
+// transpose_0100 at :721-761). It is used for super-elements of at least 64 bytes, where each
+// super-element is one contiguous destination run.
 //
 // Workgroups take cluster slots by a dynamic id (an atomic counter, taken when the workgroup
 // starts), so ids follow the order in which workgroups became resident: at any time only the
