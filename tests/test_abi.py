@@ -117,7 +117,8 @@ def test_tuning_keys_defaults_and_limits():
                 "bs_upt": 16, "bs_blocks_per_cu": 192, "oop_elem_tile_kb": 8,
                 "copy_blocks_per_cu": 64, "copy_nt_store_mb": 256, "oop_pitched": 1, "oop_min_rows": 256, "oop_min_tiles": 512, "st_super_bytes": 8192,
                 "st_max_height": 4096,
-                "autotune": 0}
+                "autotune": 0, "oop_order": 2, "cl_max_wgs": 16, "cl_min_bytes": 256 << 10,
+                "cl_min_inst": 64, "cl_fuse": 1, "cl_spin": 1 << 22}
     for k, v in defaults.items():
         assert lm.get_tuning(k) == v, k
     assert lm.get_tuning("no_such_key") == 0
@@ -133,6 +134,29 @@ def test_tuning_keys_defaults_and_limits():
     lm.set_tuning("oop_elem_tile_kb", 8)
     with pytest.raises(ValueError):
         lm.set_tuning("copy_blocks_per_cu", 0)
+    for k in ("cl_max_wgs", "cl_fuse", "oop_order"):  # 0 switches these off
+        keep = lm.get_tuning(k)
+        lm.set_tuning(k, 0)
+        assert lm.get_tuning(k) == 0
+        lm.set_tuning(k, keep)
+    with pytest.raises(ValueError):
+        lm.set_tuning("cl_spin", 0)  # the barrier wait needs at least one poll
+
+
+def test_planner_survives_random_shapes_host_only():
+    """Planning (cost model over every candidate route, including the cluster-staged and fused
+    forms) for random shapes and element sizes returns a valid stage code and never crashes."""
+    import random
+    import libmarshal_amd as lm
+    rng = random.Random(11)
+    valid = {2, 22, 3, 32, 4, -1, 5, 6, 7, 8, 9}
+    for _ in range(300):
+        M = int(2 ** rng.uniform(1, 17))
+        N = int(2 ** rng.uniform(1, 17))
+        es = rng.choice((1, 2, 4, 8, 16))
+        a, b, stages = lm.plan_transpose(M, N, es)
+        assert stages in valid, (M, N, es, stages)
+        assert a >= 1 and b >= 1 and M % a == 0 and N % b == 0, (M, N, es, a, b, stages)
 
 
 def test_cycle_count_matches_the_oracle():
