@@ -133,6 +133,8 @@ Knob g_knobs[] = {
     {"pl_min_walkers", {16384}},
     {"batch_solo_kb", {64}},
     {"batch_solo_lanes", {4}},
+    {"batch_arena_mb", {1024}, 0},    // a batch's through-scratch round (0: no rounds)
+    {"batch_arena_max_mb", {16}, 0},  // largest matrix taken into them
     {"serve_scratch_mb", {8192}, 0},  // per-request scratch cap of server requests
     {"seg_len", {16}},           // slots per segment of the segmented cycle walk
     {"seg_min_bytes", {64}},     // super-elements of at least this many bytes take it
@@ -419,16 +421,21 @@ M010 choose_010(size_t es, uint64_t A, uint64_t a, uint64_t B, int R, int P,
 // stages of 1 GB and more, and 1-5 % slower for 4- and 8-byte elements below that, so knob
 // "oop_pitched" = 1 takes it for 1- and 2-byte elements and from 512 MiB (2: always, 0:
 // never). Returns the scratch row pitch, or 0 for the plain form.
-static uint64_t oop_pitch(const StageRec &r) {
-  const uint64_t S = r.d3, row_bytes = r.d1 * S;
-  const uintptr_t b = reinterpret_cast<uintptr_t>(r.buf);
+uint64_t oop_pitch_for(const unsigned char *buf, uint64_t inst, uint64_t R, uint64_t C,
+                       uint64_t S) {
+  const uint64_t row_bytes = R * S;
+  const uintptr_t b = reinterpret_cast<uintptr_t>(buf);
   const int64_t mode = tuning("oop_pitched");
   if (mode == 0) return 0;
-  if (mode == 1 && S > 2 && r.d0 * r.d1 * r.d2 * S < (uint64_t(512) << 20)) return 0;
+  if (mode == 1 && S > 2 && inst * R * C * S < (uint64_t(512) << 20)) return 0;
   if (!(S == 1 || S == 2 || S == 4 || S == 8 || S == 16) || b % S) return 0;
   if (((b | row_bytes) & 15u) == 0) return 0;
   if (row_bytes < std::max<uint64_t>(16, uint64_t(tuning("gather_row_bytes")))) return 0;
   return (row_bytes + 15) & ~uint64_t(15);
+}
+
+static uint64_t oop_pitch(const StageRec &r) {
+  return oop_pitch_for(r.buf, r.d0, r.d1, r.d2, r.d3);
 }
 
 // Uses the pitch fixed when the stage was planned (StageRec::pitch), so scratch sized from a

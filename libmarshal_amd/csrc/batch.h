@@ -24,6 +24,7 @@ struct Group {
   std::vector<unsigned char> items;  // packed parameter blocks
   size_t item_size = 0;
   std::vector<uint64_t> prefix{0};   // prefix sum of work units per item
+  std::vector<uint64_t> lut;         // item of every kFindStep-th unit (stored after prefix)
   size_t lds = 0;
   size_t items_off = 0, prefix_off = 0;  // offsets in the parameter buffer
   const void *func = nullptr;
@@ -46,6 +47,12 @@ struct BatchPlan {
   std::vector<double> solo_bytes;
   size_t solo_scratch[8] = {};  // per lane: the most any of its lists needs
   int solo_lanes = 0;           // lanes in use
+  // Through-scratch rounds: matrices copied into a batch-owned arena by one grouped copy, then
+  // moved back out of place by grouped transpose / gather kernels (their levels are the last
+  // ones; item pointers into the arena hold arena offsets until build_image).
+  uint64_t arena_bytes = 0;
+  int arena_rounds = 0;
+  int64_t arena_matrices = 0;
 };
 constexpr int kSoloLanes = 8;  // at most; the "batch_solo_lanes" knob picks how many
 
@@ -64,8 +71,10 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
 // Stage list of one descriptor; allow_st admits the stripe route (plain execution only).
 bool plan_desc(const hip_marshal_desc &d, std::vector<StageRec> &out, std::string &route,
                bool allow_st);
-// Host image of the parameter buffer, with the flag offsets turned into pointers into `flags`.
-void build_image(const BatchPlan &b, const uint32_t *flags, std::vector<unsigned char> &host);
+// Host image of the parameter buffer, with the flag offsets turned into pointers into `flags`
+// and the arena offsets into pointers into `arena`.
+void build_image(const BatchPlan &b, const uint32_t *flags, std::vector<unsigned char> &host,
+                 unsigned char *arena = nullptr);
 // Clears the flags and launches every group, level by level, on `st`. Solo lists (if any)
 // run on the side streams of `solo` (ordered after the work already on `st`, which waits for
 // all of them at the end).

@@ -25,21 +25,11 @@
 
 namespace lm {
 
-__device__ __forceinline__ uint32_t find_item(const uint64_t *prefix, uint32_t n, uint64_t q) {
-  uint32_t lo = 0, hi = n;  // prefix[lo] <= q < prefix[hi]
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (prefix[mid] <= q) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
 // ---- group kernels -------------------------------------------------------------------------
 
 template <int ES, int G, int UPT, int LAYOUT>
-__global__ __launch_bounds__(256) void kg_010_bs(const Bs010Params *items,
-                                                 const uint64_t *prefix, uint32_t n,
+__global__ __launch_bounds__(256) void kg_010_bs(const Bs010Params *__restrict__ items,
+                                                 const uint64_t *__restrict__ prefix, uint32_t n,
                                                  uint64_t total) {
   using E = typename Elem<ES>::T;
   using U = typename UnitOf<ES, G>::T;
@@ -75,8 +65,8 @@ __global__ __launch_bounds__(256) void kg_010_bs(const Bs010Params *items,
 }
 
 template <int ES>
-__global__ __launch_bounds__(256) void kg_010_pttwac(const Pt010Params *items,
-                                                     const uint64_t *prefix, uint32_t n,
+__global__ __launch_bounds__(256) void kg_010_pttwac(const Pt010Params *__restrict__ items,
+                                                     const uint64_t *__restrict__ prefix, uint32_t n,
                                                      uint64_t total) {
   extern __shared__ __align__(16) uint32_t lflags[];
   for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
@@ -87,7 +77,7 @@ __global__ __launch_bounds__(256) void kg_010_pttwac(const Pt010Params *items,
 }
 
 template <int UB, int UPL, bool I64>
-__global__ __launch_bounds__(256) void kg_100(const Pt100Params *items, const uint64_t *prefix,
+__global__ __launch_bounds__(256) void kg_100(const Pt100Params *__restrict__ items, const uint64_t *__restrict__ prefix,
                                               uint32_t n, uint64_t tasks) {
   const uint64_t wave = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
   const uint64_t nwaves = uint64_t(gridDim.x) * 4;
@@ -100,8 +90,8 @@ __global__ __launch_bounds__(256) void kg_100(const Pt100Params *items, const ui
 }
 
 template <int UB, int UPL>
-__global__ __launch_bounds__(256) void kg_0100_lds(const Pt100Params *items,
-                                                   const uint64_t *prefix, uint32_t n,
+__global__ __launch_bounds__(256) void kg_0100_lds(const Pt100Params *__restrict__ items,
+                                                   const uint64_t *__restrict__ prefix, uint32_t n,
                                                    uint64_t total) {
   extern __shared__ __align__(16) uint32_t lflags[];
   for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
@@ -112,7 +102,7 @@ __global__ __launch_bounds__(256) void kg_0100_lds(const Pt100Params *items,
 }
 
 template <int UB>
-__global__ __launch_bounds__(1024) void kg_rc_row(const RcParams *items, const uint64_t *prefix,
+__global__ __launch_bounds__(1024) void kg_rc_row(const RcParams *__restrict__ items, const uint64_t *__restrict__ prefix,
                                                  uint32_t n, uint64_t total) {
   using V = typename Elem<UB>::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -128,7 +118,7 @@ __global__ __launch_bounds__(1024) void kg_rc_row(const RcParams *items, const u
 // multiple of 8, so (unit mod 8) is this workgroup's XCD and each XCD sweeps a contiguous
 // range of an item's column blocks.
 template <int UB, bool SHUFFLE>
-__global__ __launch_bounds__(1024) void kg_rc_col(const RcParams *items, const uint64_t *prefix,
+__global__ __launch_bounds__(1024) void kg_rc_col(const RcParams *__restrict__ items, const uint64_t *__restrict__ prefix,
                                                  uint32_t n, uint64_t total) {
   using V = typename Elem<UB>::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -192,6 +182,34 @@ Pt100Params make_pt100(const StageRec &r, uint32_t *flags) {
   p.div_C32 = FastDiv32(uint32_t(std::min<uint64_t>(p.C, 0xffffffffu)));
   return p;
 }
+
+// The out-of-place form of a descriptor: [inst][R][C] units of S bytes -> [inst][C][R]
+// (false when the conversion moves nothing).
+bool oop_shape(const hip_marshal_desc &d, uint64_t &inst, uint64_t &R, uint64_t &C,
+               uint64_t &S) {
+  const uint64_t h = uint64_t(d.rows), w = uint64_t(d.cols);
+  const uint64_t t = uint64_t(std::max<int64_t>(d.tile, 1)), es = uint64_t(d.elem_bytes);
+  switch (d.op) {
+    case HIP_MARSHAL_OP_TRANSPOSE: inst = 1; R = h; C = w; S = es; break;
+    case HIP_MARSHAL_OP_AOS_ASTA: inst = h / t; R = t; C = w; S = es; break;     // 010(h/t, t, w)
+    case HIP_MARSHAL_OP_ASTA_AOS: inst = h / t; R = w; C = t; S = es; break;     // 010(h/t, w, t)
+    case HIP_MARSHAL_OP_SOA_ASTA: inst = 1; R = w; C = h / t; S = t * es; break;  // 100(w, h/t, t)
+    case HIP_MARSHAL_OP_ASTA_SOA: inst = 1; R = h / t; C = w; S = t * es; break;  // 100(h/t, w, t)
+    default: return false;
+  }
+  return inst >= 1 && R > 1 && C > 1;
+}
+
+struct ArenaItem {
+  int64_t desc;
+  unsigned char *buf;
+  uint64_t inst, R, C, S, bytes;
+  uint64_t P;       // pitched order: arena row pitch (0: copy, then transpose back)
+  uint64_t abytes;  // arena bytes it takes
+  uint64_t off;     // its offset in the arena
+};
+
+uint64_t round8(uint64_t v) { return (v + 7) / 8 * 8; }
 
 template <int ES, int G, int LAYOUT>
 const void *bs_func_l(int upt) {
@@ -331,12 +349,48 @@ bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_
   return false;
 }
 
-void build_image(const BatchPlan &b, const uint32_t *flags, std::vector<unsigned char> &host) {
+void build_image(const BatchPlan &b, const uint32_t *flags, std::vector<unsigned char> &host,
+                 unsigned char *arena) {
   host.assign(b.param_bytes, 0);
   for (auto &lvl : b.levels)
     for (const Group &g : lvl) {
       std::memcpy(host.data() + g.items_off, g.items.data(), g.items.size());
       std::memcpy(host.data() + g.prefix_off, g.prefix.data(), g.prefix.size() * 8);
+      std::memcpy(host.data() + g.prefix_off + g.prefix.size() * 8, g.lut.data(),
+                  g.lut.size() * 8);
+      unsigned char *img = host.data() + g.items_off;
+      // arena offsets become pointers
+      if (g.key.kind == 7)
+        for (size_t off = 0; off < g.items.size(); off += sizeof(CopyItem)) {
+          CopyItem c;
+          std::memcpy(&c, img + off, sizeof(c));
+          c.dst = arena + reinterpret_cast<uintptr_t>(c.dst);
+          std::memcpy(img + off, &c, sizeof(c));
+        }
+      if (g.key.kind == 8 || g.key.kind == 10)  // from the arena / into it (pitched)
+        for (size_t off = 0; off < g.items.size(); off += sizeof(OopParams)) {
+          OopParams p;
+          std::memcpy(&p, img + off, sizeof(p));
+          if (g.key.kind == 8)
+            p.src = arena + reinterpret_cast<uintptr_t>(p.src);
+          else
+            p.dst = arena + reinterpret_cast<uintptr_t>(p.dst);
+          std::memcpy(img + off, &p, sizeof(p));
+        }
+      if (g.key.kind == 11)
+        for (size_t off = 0; off < g.items.size(); off += sizeof(UnpitchItem)) {
+          UnpitchItem u;
+          std::memcpy(&u, img + off, sizeof(u));
+          u.src = arena + reinterpret_cast<uintptr_t>(u.src);
+          std::memcpy(img + off, &u, sizeof(u));
+        }
+      if (g.key.kind == 9)
+        for (size_t off = 0; off < g.items.size(); off += sizeof(GatherParams)) {
+          GatherParams p;
+          std::memcpy(&p, img + off, sizeof(p));
+          p.src = arena + reinterpret_cast<uintptr_t>(p.src);
+          std::memcpy(img + off, &p, sizeof(p));
+        }
       if (g.key.kind != 2) continue;  // global visited bits: offsets become pointers
       for (size_t off = 0; off < g.items.size(); off += sizeof(Pt100Params)) {
         Pt100Params p;
@@ -403,6 +457,12 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
     batch_bytes += double(descs[i].rows) * double(descs[i].cols) * double(descs[i].elem_bytes);
   double solo_min = double(uint64_t(tuning("batch_solo_kb")) << 10);
   if (batch_bytes < double(1ull << 30)) solo_min = std::max(solo_min, double(1u << 20));
+  // Through-scratch rounds ("batch_arena_mb" per round, 0: off) for matrices of at most
+  // "batch_arena_max_mb" whose best route is not a single streaming pass.
+  const uint64_t arena_cap = uint64_t(tuning("batch_arena_mb")) << 20;
+  const double arena_max =
+      std::min(double(uint64_t(tuning("batch_arena_max_mb")) << 20), double(arena_cap));
+  std::vector<ArenaItem> arena;
   for (int64_t i = 0; i < count; i++) {
     const hip_marshal_desc &d = descs[i];
     std::string route;
@@ -420,16 +480,43 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
         return false;
       };
       // (stage kinds, not route tags: a composer route's tag does not name its stage forms)
-      if (plan_desc(d, list, r2, true) &&
-          (has({StageKind::ST}) ||
-           (mbytes >= solo_min && has({StageKind::SQ, StageKind::OOP, StageKind::PT100C,
-                                       StageKind::SEG, StageKind::CL})))) {
-        b.solo.push_back(std::move(list));
-        b.solo_bytes.push_back(double(d.rows) * double(d.cols) * double(d.elem_bytes));
-        b.routes[i] = r2;
-        b.stages += 1;
-        b.bytes += 2.0 * double(d.rows) * double(d.cols) * double(d.elem_bytes);
-        continue;
+      if (plan_desc(d, list, r2, true)) {
+        const bool stripe = has({StageKind::ST});
+        const bool solo_form = mbytes >= solo_min && has({StageKind::SQ, StageKind::OOP,
+                                                          StageKind::PT100C, StageKind::SEG,
+                                                          StageKind::CL});
+        // A single streaming pass as it would run here: the LDS-staged 010 (grouped below),
+        // the square, cycle-leader and cluster forms on the side streams.
+        const StageKind k0 = list.size() == 1 ? list[0].kind : StageKind::OOP;
+        const bool one_pass =
+            list.size() == 1 &&
+            (k0 == StageKind::BS010 ||
+             (solo_form && (k0 == StageKind::SQ || k0 == StageKind::PT100C ||
+                            k0 == StageKind::CL)));
+        ArenaItem ai{};
+        if (!stripe && !one_pass && arena_cap && mbytes <= arena_max &&
+            (reinterpret_cast<uintptr_t>(d.ptr) & 15u) == 0 &&
+            oop_shape(d, ai.inst, ai.R, ai.C, ai.S) &&
+            ((ai.S <= 16 && (ai.S & (ai.S - 1)) == 0) || oop_gather_fits(ai.S))) {
+          ai.desc = i;
+          ai.buf = static_cast<unsigned char *>(d.ptr);
+          ai.bytes = ai.inst * ai.R * ai.C * ai.S;
+          ai.P = oop_pitch_for(ai.buf, ai.inst, ai.R, ai.C, ai.S);
+          ai.abytes = ai.P ? ai.inst * ai.C * ai.P : ai.bytes;
+          arena.push_back(ai);
+          b.routes[i] = "ARENA-";
+          b.stages += 1;
+          b.bytes += 2.0 * mbytes;
+          continue;
+        }
+        if (stripe || solo_form) {
+          b.solo.push_back(std::move(list));
+          b.solo_bytes.push_back(mbytes);
+          b.routes[i] = r2;
+          b.stages += 1;
+          b.bytes += 2.0 * mbytes;
+          continue;
+        }
       }
     }
     bool planned;
@@ -590,14 +677,156 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
     }
     b.levels.push_back(std::move(lvl));
   }
+  // 3b. through-scratch rounds: first fit, largest first, into rounds of `arena_cap` bytes.
+  // Round k: one grouped copy of its matrices into the arena, then the grouped out-of-place
+  // kernels back into their buffers; the next round's copy waits for them (same stream).
+  if (!arena.empty()) {
+    std::stable_sort(arena.begin(), arena.end(),
+                     [](const ArenaItem &x, const ArenaItem &y) { return x.abytes > y.abytes; });
+    std::vector<uint64_t> fill;
+    std::vector<std::vector<size_t>> rounds;
+    for (size_t k = 0; k < arena.size(); k++) {
+      const uint64_t need = (arena[k].abytes + 255) / 256 * 256;
+      size_t r = 0;
+      while (r < fill.size() && fill[r] + need > arena_cap) r++;
+      if (r == fill.size()) {
+        fill.push_back(0);
+        rounds.emplace_back();
+      }
+      arena[k].off = fill[r];
+      fill[r] += need;
+      rounds[r].push_back(k);
+      b.arena_bytes = std::max(b.arena_bytes, fill[r]);
+    }
+    b.arena_rounds = int(rounds.size());
+    b.arena_matrices = int64_t(arena.size());
+    const size_t tile = size_t(tuning("st_tile_kb")) << 10;
+    const size_t gtile = size_t(tuning("gather_tile_kb")) << 10;
+    const uint64_t grow = uint64_t(tuning("gather_row_bytes"));
+    for (auto &rk : rounds) {
+      // level A: copies into the arena and pitched transposes into it; level B: transposes,
+      // gathers and unpitch copies back into the buffers
+      Group cp;
+      cp.key = GroupKey{7, 0, 0, 0};
+      std::map<GroupKey, Group> fwd, back;
+      for (size_t k : rk) {
+        const ArenaItem &a = arena[k];
+        // arena-relative addresses; build_image adds the arena's base
+        unsigned char *scr = reinterpret_cast<unsigned char *>(uintptr_t(a.off));
+        const uint64_t block = a.R * a.C * a.S, row_bytes = a.R * a.S;
+        if (a.P) {
+          OopLaunch o;
+          if (prepare_oop_transpose(a.buf, scr, a.S, a.inst, a.R, a.C, block, a.C * a.P, tile,
+                                    0, a.P, -1, &o) != hipSuccess ||
+              !kg_oop_transpose_func(o.es, o.vin, o.out)) {
+            fprintf(stderr, "[libmarshal] %s: no grouped transpose for descriptor %lld\n", who,
+                    (long long)a.desc);
+            return false;
+          }
+          GroupKey gk{10, o.es, o.vin ? 1 : 0, o.out};
+          Group &g = fwd[gk];
+          g.key = gk;
+          g.lds = std::max(g.lds, o.lds);
+          add_item(g, o.p, round8(o.grid));
+          const UnpitchItem u = make_unpitch_item(a.buf, scr, row_bytes, a.P, a.inst * a.C);
+          const int nts = (tuning("oop_nt") & 2) ? 1 : 0;
+          GroupKey uk{11, nts, 0, 0};
+          Group &ug = back[uk];
+          ug.key = uk;
+          add_item(ug, u, std::max<uint64_t>(1, (u.nb + kUnpitchItemUnit - 1) / kUnpitchItemUnit));
+          continue;
+        }
+        add_item(cp, CopyItem{scr, a.buf, a.bytes}, (a.bytes + kCopyItemUnit - 1) / kCopyItemUnit);
+        const bool aligned_rows = ((reinterpret_cast<uintptr_t>(a.buf) | row_bytes) & 15u) == 0;
+        const bool unit = a.S == 1 || a.S == 2 || a.S == 4 || a.S == 8 || a.S == 16;
+        if (unit && (aligned_rows || row_bytes >= grow)) {
+          OopLaunch o;
+          if (prepare_oop_transpose(scr, a.buf, a.S, a.inst, a.R, a.C, block, block, tile, 0, 0,
+                                    -1, &o) != hipSuccess ||
+              !kg_oop_transpose_func(o.es, o.vin, o.out)) {
+            fprintf(stderr, "[libmarshal] %s: no grouped transpose for descriptor %lld\n", who,
+                    (long long)a.desc);
+            return false;
+          }
+          GroupKey gk{8, o.es, o.vin ? 1 : 0, o.out};
+          Group &g = back[gk];
+          g.key = gk;
+          g.lds = std::max(g.lds, o.lds);
+          add_item(g, o.p, round8(o.grid));
+        } else {
+          GatherLaunch o;
+          if (prepare_oop_gather(scr, a.buf, a.S, a.inst, a.R, a.C, gtile, &o) != hipSuccess) {
+            fprintf(stderr, "[libmarshal] %s: no grouped gather for descriptor %lld\n", who,
+                    (long long)a.desc);
+            return false;
+          }
+          GroupKey gk{9, o.u, o.fc ? 1 : 0, 0};
+          Group &g = back[gk];
+          g.key = gk;
+          g.lds = std::max(g.lds, o.lds);
+          add_item(g, o.p, round8(o.grid));
+        }
+      }
+      auto finish = [&](std::map<GroupKey, Group> &gs, std::vector<Group> &lvl) {
+        for (auto &kv : gs) {
+          Group g = kv.second;
+          switch (g.key.kind) {
+            case 8:
+            case 10:
+              g.func = kg_oop_transpose_func(g.key.a, g.key.b != 0, g.key.c);
+              g.block = kOopGroupThreads;
+              g.grid = uint32_t(std::min<uint64_t>(g.prefix.back(), uint64_t(di.cus) * 4));
+              break;
+            case 9:
+              g.func = kg_oop_gather_func(g.key.a, g.key.b != 0);
+              g.block = kOopGroupThreads;
+              g.grid = uint32_t(std::min<uint64_t>(g.prefix.back(), uint64_t(di.cus) * 4));
+              break;
+            default:  // 11: unpitch copies
+              g.func = kg_unpitch_func(g.key.a != 0);
+              g.block = 256;
+              g.grid = uint32_t(std::min<uint64_t>(g.prefix.back(), uint64_t(di.cus) * 8));
+              break;
+          }
+          if (g.lds > 65536 &&
+              hipFuncSetAttribute(g.func, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(g.lds)) != hipSuccess) {
+            fprintf(stderr, "[libmarshal] %s: LDS attribute failed\n", who);
+            return false;
+          }
+          lvl.push_back(std::move(g));
+        }
+        return true;
+      };
+      std::vector<Group> la, lb;
+      if (cp.prefix.size() > 1) {
+        cp.func = kg_copy_func();
+        cp.block = 256;
+        cp.grid = uint32_t(std::min<uint64_t>(cp.prefix.back(), uint64_t(di.cus) * 8));
+        la.push_back(std::move(cp));
+      }
+      if (!finish(fwd, la) || !finish(back, lb)) return false;
+      b.levels.push_back(std::move(la));
+      b.levels.push_back(std::move(lb));
+    }
+  }
   // 4. layout of every work list in one parameter buffer
   b.param_bytes = 0;
   for (auto &lvl : b.levels)
     for (Group &g : lvl) {
       g.items_off = b.param_bytes;
       b.param_bytes += (g.items.size() + 255) / 256 * 256;
+      // find_item's table: the owner of unit j * kFindStep, for j up to one past the end
+      const uint64_t total = g.prefix.back();
+      const size_t n = g.prefix.size() - 1;
+      g.lut.assign(size_t(total / kFindStep) + 2, uint64_t(n - 1));
+      size_t it = 0;
+      for (size_t j = 0; j < g.lut.size() && j * kFindStep < total; j++) {
+        while (g.prefix[it + 1] <= j * kFindStep) it++;
+        g.lut[j] = it;
+      }
       g.prefix_off = b.param_bytes;
-      b.param_bytes += (g.prefix.size() * 8 + 255) / 256 * 256;
+      b.param_bytes += ((g.prefix.size() + g.lut.size()) * 8 + 255) / 256 * 256;
     }
   return true;
 }
@@ -610,6 +839,7 @@ struct hip_marshal_batch {
   BatchPlan b;
   void *params = nullptr;     // device copy of all item arrays and prefix sums
   uint32_t *flags = nullptr;  // visited bits of every global-bit stage
+  unsigned char *arena = nullptr;  // through-scratch rounds
   SoloExec solo;  // side streams, events and scratch of the stripe-route matrices
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -683,11 +913,13 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
              hipSuccess;
     }
   }
-  if (b.flag_words) ok = hipMalloc(&hb->flags, b.flag_words * 4) == hipSuccess;
+  if (ok && b.flag_words) ok = hipMalloc(&hb->flags, b.flag_words * 4) == hipSuccess;
+  if (ok && b.arena_bytes)
+    ok = hipMalloc(reinterpret_cast<void **>(&hb->arena), b.arena_bytes) == hipSuccess;
   if (ok && b.param_bytes) ok = hipMalloc(&hb->params, b.param_bytes) == hipSuccess;
   if (ok && b.param_bytes) {
     std::vector<unsigned char> host;
-    build_image(b, hb->flags, host);
+    build_image(b, hb->flags, host, hb->arena);
     ok = hipMemcpy(hb->params, host.data(), b.param_bytes, hipMemcpyHostToDevice) == hipSuccess;
   }
   if (!ok) {
@@ -749,6 +981,9 @@ int64_t hip_marshal_batch_info(const hip_marshal_batch *hb, const char *key) {
   if (!strcmp(key, "flag_bytes")) return int64_t(b.flag_words * 4);
   if (!strcmp(key, "bytes")) return int64_t(b.bytes);
   if (!strcmp(key, "solo")) return int64_t(b.solo.size());
+  if (!strcmp(key, "arena_matrices")) return b.arena_matrices;
+  if (!strcmp(key, "arena_rounds")) return int64_t(b.arena_rounds);
+  if (!strcmp(key, "arena_bytes")) return int64_t(b.arena_bytes);
   if (!strcmp(key, "kernels")) {
     int64_t k = b.flag_words ? 1 : 0;
     for (auto &l : b.levels) k += int64_t(l.size());
@@ -768,6 +1003,7 @@ void hip_marshal_batch_destroy(hip_marshal_batch *hb) {
   if (hb->graph) hipGraphDestroy(hb->graph);
   if (hb->params) hipFree(hb->params);
   if (hb->flags) hipFree(hb->flags);
+  if (hb->arena) hipFree(hb->arena);
   for (int l = 0; l < kSoloLanes; l++) {
     if (hb->solo.st[l]) hipStreamSynchronize(hb->solo.st[l]);
     if (hb->solo.scr[l]) hipFree(hb->solo.scr[l]);

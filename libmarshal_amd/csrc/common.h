@@ -145,4 +145,22 @@ __device__ __forceinline__ U128 load_window16(const unsigned char *base, uint64_
   return r;
 }
 
+// Work lists of grouped kernels: item i owns units [prefix[i], prefix[i + 1]) (every item at
+// least one), and right after prefix[n] a lookup table holds, for every kFindStep units, the
+// item owning unit j * kFindStep (the last item past the end). find_item returns the item
+// that owns unit q (q < prefix[n]): two table reads bound a binary search that is empty
+// whenever the items are longer than kFindStep units.
+constexpr uint32_t kFindLog2 = 6, kFindStep = 1u << kFindLog2;
+__device__ __forceinline__ uint32_t find_item(const uint64_t *prefix, uint32_t n, uint64_t q) {
+  const uint64_t *lut = prefix + n + 1;
+  const uint64_t j = q >> kFindLog2;
+  uint32_t lo = uint32_t(lut[j]), hi = uint32_t(lut[j + 1]) + 1;  // prefix[lo] <= q < prefix[hi]
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (prefix[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 }  // namespace lm

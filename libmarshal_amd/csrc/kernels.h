@@ -160,6 +160,94 @@ hipError_t launch_copy2d(unsigned char *dst, uint64_t dpitch, const unsigned cha
 // chunks whatever its alignment.
 hipError_t launch_unpitch(unsigned char *dst, const unsigned char *src, uint64_t RB,
                           uint64_t P, uint64_t rows, hipStream_t st, int nt_store = -1);
+// Parameters of one out-of-place transpose launch (k_oop_transpose) and of one out-of-place
+// super-element gather (k_oop_gather); both kernels take a block index, so a grouped kernel
+// can run many such launches as one (every matrix's blocks in one work list).
+struct OopParams {
+  const unsigned char *src;         // 16-byte aligned base; the first block starts at src_off
+  unsigned char *dst;
+  uint64_t src_off;
+  uint64_t src_stride, dst_stride;  // bytes between consecutive blocks
+  uint64_t src_pitch, dst_pitch;    // bytes between rows (source rows of C, destination of R)
+  uint64_t src_end;                 // bytes readable from src (bounds the window over-read)
+  uint64_t R, C;                    // each block R x C (source) -> C x R (destination)
+  uint32_t tr_log2, tc_log2;        // tile TR x TC elements, both powers of two
+  uint32_t PB;                      // LDS row pitch in bytes (multiple of 128)
+  uint32_t wstore;                  // 4-/8-byte elements: aligned vector stores on misaligned rows
+  uint64_t ntr, ntc, per_block, tiles, per_xcd;
+  // skewed form: workgroups from per_xcd * 8 on write the chunks holding row boundaries
+  uint64_t fix_rows;                // rows of all blocks (one more chunk: the end)
+  FastDiv64 div_C;
+  uint32_t nt;                      // bit 0: non-temporal source loads; bit 1: stores
+  uint32_t order;                   // tile order in a block (see oop_tile_rc)
+};
+struct GatherParams {
+  const unsigned char *src;  // 16-byte aligned
+  unsigned char *dst;
+  uint64_t inst_bytes;       // R * C * S
+  uint64_t R, C;
+  uint32_t S, TR, TC, LP;    // LP: LDS row pitch (multiple of 16)
+  uint32_t nch, nchd;        // 16-byte chunks per LDS row / per destination segment
+  uint32_t fullcol;          // TR == R: one destination segment per tile (kernel form FC)
+  FastDiv32 div_nch, div_nchd, div_S, div_RS;
+  uint64_t ntc, per_inst, tiles, per_xcd;
+};
+struct OopLaunch {
+  OopParams p{};
+  size_t lds = 0;
+  uint64_t grid = 0;  // workgroups of the single launch (tiles, plus fix-up units when out = 2)
+  int es = 0;
+  bool vin = false;
+  int out = 0;
+};
+struct GatherLaunch {
+  GatherParams p{};
+  size_t lds = 0;
+  uint64_t grid = 0;
+  int u = 1;
+  bool fc = false;
+};
+// One contiguous byte copy of a grouped copy kernel (dst and src 16-byte aligned).
+struct CopyItem {
+  unsigned char *dst;
+  const unsigned char *src;
+  uint64_t bytes;
+};
+constexpr uint64_t kCopyItemUnit = 16384;  // bytes per work unit of the grouped copy
+// Parameters, template and grid of the launch launch_oop_transpose / launch_oop_gather would
+// make (grid 0: nothing to do).
+hipError_t prepare_oop_transpose(const unsigned char *src, unsigned char *dst, size_t es,
+                                 uint64_t blocks, uint64_t R, uint64_t C, uint64_t src_stride,
+                                 uint64_t dst_stride, size_t tile_bytes, uint64_t src_end,
+                                 uint64_t dst_pitch, int nt, OopLaunch *o);
+hipError_t prepare_oop_gather(const unsigned char *src, unsigned char *dst, uint64_t S,
+                              uint64_t inst, uint64_t R, uint64_t C, size_t tile_bytes,
+                              GatherLaunch *o);
+// Grouped kernels (items, prefix of units per item, item count, total units): units of an
+// item are its launch's blocks, padded to a multiple of 8 so that with a grid that is a
+// multiple of 8 a unit keeps its XCD.
+const void *kg_oop_transpose_func(int es, bool vin, int out);
+const void *kg_oop_gather_func(int u, bool fc);
+const void *kg_copy_func();
+constexpr int kOopGroupThreads = 512;
+// One pitched-to-contiguous copy of a grouped unpitch kernel (see launch_unpitch).
+struct UnpitchItem {
+  unsigned char *dst;
+  const unsigned char *src;  // 16-byte aligned, rows of RB bytes at pitch P
+  uint64_t RB, P, total, nb;
+  uint32_t head;
+  FastDiv64 div_rb;
+};
+constexpr uint64_t kUnpitchItemUnit = 1024;  // 16-byte destination chunks per work unit
+UnpitchItem make_unpitch_item(unsigned char *dst, const unsigned char *src, uint64_t RB,
+                              uint64_t P, uint64_t rows);
+const void *kg_unpitch_func(bool nts);
+// Scratch row pitch of a through-scratch step over [inst][R][C] S-byte units of `buf` when
+// it takes the pitched order (transpose into 16-byte-aligned scratch rows, then one
+// contiguous copy back); 0 for the plain order (copy, then transpose back).
+uint64_t oop_pitch_for(const unsigned char *buf, uint64_t inst, uint64_t R, uint64_t C,
+                       uint64_t S);
+
 // Out-of-place transpose of `blocks` R x C matrices (src and dst must not overlap), tiles of
 // at most `tile_bytes` staged in LDS. src_end (0: just the blocks) is how many bytes from
 // src may be read: 16-byte window loads of misaligned rows read up to 31 bytes past a row.

@@ -38,24 +38,6 @@ namespace lm {
 // of one LDS write hit distinct banks, and 16-byte reads along a row stay conflict-free.
 // Source rows are read as 16-byte windows at any byte alignment (two aligned loads and a
 // byte shift), so rows of an odd number of bytes still load one vector per lane.
-struct OopParams {
-  const unsigned char *src;         // 16-byte aligned base; the first block starts at src_off
-  unsigned char *dst;
-  uint64_t src_off;
-  uint64_t src_stride, dst_stride;  // bytes between consecutive blocks
-  uint64_t src_pitch, dst_pitch;    // bytes between rows (source rows of C, destination of R)
-  uint64_t src_end;                 // bytes readable from src (bounds the window over-read)
-  uint64_t R, C;                    // each block R x C (source) -> C x R (destination)
-  uint32_t tr_log2, tc_log2;        // tile TR x TC elements, both powers of two
-  uint32_t PB;                      // LDS row pitch in bytes (multiple of 128)
-  uint32_t wstore;                  // 4-/8-byte elements: aligned vector stores on misaligned rows
-  uint64_t ntr, ntc, per_block, tiles, per_xcd;
-  // skewed form: workgroups from per_xcd * 8 on write the chunks holding row boundaries
-  uint64_t fix_rows;                // rows of all blocks (one more chunk: the end)
-  FastDiv64 div_C;
-  uint32_t nt;                      // bit 0: non-temporal source loads; bit 1: stores
-  uint32_t order;                   // tile order in a block (see oop_tile_rc)
-};
 
 // Tile row and column of the rem-th tile of a block. order 0: row-major over tiles (tiles
 // along a source row come one after another). order k >= 1: the tile columns form groups of
@@ -350,21 +332,23 @@ __device__ __forceinline__ void oop_store(const OopParams &p, const unsigned cha
   }
 }
 
+// One workgroup's share of a launch: block bx of the launch's grid (tile or fix-up unit). The
+// single-matrix kernel passes blockIdx.x; the grouped kernel of a batch passes the block's
+// index within the matrix it serves.
 template <int ES, bool VIN, int OUT, int NT>
-__global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
+__device__ __forceinline__ void oop_block(const OopParams &p, uint64_t bx, unsigned char *lds) {
   using E = typename Elem<ES>::T;
   constexpr uint32_t EPV = 16 / ES;  // elements per 16-byte vector
   constexpr uint32_t EPV_LOG2 = ES == 1 ? 4 : ES == 2 ? 3 : ES == 4 ? 2 : ES == 8 ? 1 : 0;
   constexpr int SW = VIN ? int(EPV_LOG2) : 0;
   constexpr uint32_t KOFS = OUT == 2 ? EPV : 0;  // LDS element index of source row r0
-  extern __shared__ __align__(16) unsigned char lds[];
-  if (OUT == 2 && blockIdx.x >= p.per_xcd * 8) {
-    oop_fix<ES>(p, (blockIdx.x - p.per_xcd * 8) * uint64_t(NT) + threadIdx.x);
+  if (OUT == 2 && bx >= p.per_xcd * 8) {
+    oop_fix<ES>(p, (bx - p.per_xcd * 8) * uint64_t(NT) + threadIdx.x);
     return;
   }
   // Consecutive tiles go to the same XCD (workgroups are dealt round-robin over 8 XCDs), so
   // partial lines shared by neighbouring tiles meet in one L2.
-  const uint64_t t = uint64_t(blockIdx.x & 7u) * p.per_xcd + (blockIdx.x >> 3);
+  const uint64_t t = (bx & 7u) * p.per_xcd + (bx >> 3);
   if (t >= p.tiles) return;
   const uint64_t g = t / p.per_block;
   const uint64_t rem = t - g * p.per_block;
@@ -463,6 +447,12 @@ __global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
   }
   __syncthreads();
   oop_store<ES, SW, OUT, NT>(p, lds, dst, r0, c0, nr, nc);
+}
+
+template <int ES, bool VIN, int OUT, int NT>
+__global__ __launch_bounds__(NT) void k_oop_transpose(OopParams p) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  oop_block<ES, VIN, OUT, NT>(p, blockIdx.x, lds);
 }
 
 // ---- square in-place transpose: swap tile pairs ------------------------------------------
@@ -748,15 +738,15 @@ __device__ __forceinline__ uint64_t unpitch_src(uint64_t d, uint64_t RB, uint64_
   return row * P + (d - row * RB);
 }
 
+// Destination chunks [rd * 256 * kCopyUnroll, +256 * kCopyUnroll) (round rd).
 template <bool NTS>
-__global__ __launch_bounds__(256) void k_unpitch(unsigned char *dst,
-                                                 const unsigned char *src, uint64_t RB,
-                                                 uint64_t P, uint64_t total, uint32_t head,
-                                                 uint64_t nb, FastDiv64 div_rb) {
-  const uint64_t stride = uint64_t(gridDim.x) * 256;
+__device__ __forceinline__ void unpitch_round(unsigned char *dst, const unsigned char *src,
+                                              uint64_t RB, uint64_t P, uint32_t head,
+                                              uint64_t nb, const FastDiv64 &div_rb,
+                                              uint64_t rd) {
   U128 *d16 = reinterpret_cast<U128 *>(dst + head);  // 16-byte aligned
-  for (uint64_t b = uint64_t(blockIdx.x) * 256 * kCopyUnroll + threadIdx.x; b < nb;
-       b += stride * kCopyUnroll) {
+  {
+    const uint64_t b = rd * 256 * kCopyUnroll + threadIdx.x;
     U128 lo[kCopyUnroll], hi[kCopyUnroll];
     uint64_t s[kCopyUnroll];
     uint32_t split[kCopyUnroll];  // bytes of the chunk before a row boundary (16: none)
@@ -805,11 +795,26 @@ __global__ __launch_bounds__(256) void k_unpitch(unsigned char *dst,
         d16[i] = v;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    for (uint64_t x = 0; x < head && x < total; x++) dst[x] = src[unpitch_src(x, RB, P, div_rb)];
-    for (uint64_t x = head + 16 * nb; x < total; x++)
-      dst[x] = src[unpitch_src(x, RB, P, div_rb)];
-  }
+}
+
+// Bytes before the destination's first aligned address and after its last whole chunk.
+__device__ __forceinline__ void unpitch_ends(unsigned char *dst, const unsigned char *src,
+                                             uint64_t RB, uint64_t P, uint64_t total,
+                                             uint32_t head, uint64_t nb,
+                                             const FastDiv64 &div_rb) {
+  for (uint64_t x = 0; x < head && x < total; x++) dst[x] = src[unpitch_src(x, RB, P, div_rb)];
+  for (uint64_t x = head + 16 * nb; x < total; x++) dst[x] = src[unpitch_src(x, RB, P, div_rb)];
+}
+
+template <bool NTS>
+__global__ __launch_bounds__(256) void k_unpitch(unsigned char *dst,
+                                                 const unsigned char *src, uint64_t RB,
+                                                 uint64_t P, uint64_t total, uint32_t head,
+                                                 uint64_t nb, FastDiv64 div_rb) {
+  const uint64_t rounds = (nb + 256 * kCopyUnroll - 1) / (256 * kCopyUnroll);
+  for (uint64_t rd = blockIdx.x; rd < rounds; rd += gridDim.x)
+    unpitch_round<NTS>(dst, src, RB, P, head, nb, div_rb, rd);
+  if (blockIdx.x == 0 && threadIdx.x == 0) unpitch_ends(dst, src, RB, P, total, head, nb, div_rb);
 }
 
 hipError_t launch_unpitch(unsigned char *dst, const unsigned char *src, uint64_t RB,
@@ -892,14 +897,15 @@ hipError_t launch_oop_es(const OopParams &p, bool vin, int out, size_t lds, hipS
 }
 }  // namespace
 
-hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, size_t es,
-                                uint64_t blocks, uint64_t R, uint64_t C, uint64_t src_stride,
-                                uint64_t dst_stride, size_t tile_bytes, hipStream_t st,
-                                uint64_t src_end, uint64_t dst_pitch, int nt) {
+hipError_t prepare_oop_transpose(const unsigned char *src, unsigned char *dst, size_t es,
+                                 uint64_t blocks, uint64_t R, uint64_t C, uint64_t src_stride,
+                                 uint64_t dst_stride, size_t tile_bytes, uint64_t src_end,
+                                 uint64_t dst_pitch, int nt, OopLaunch *o) {
+  *o = OopLaunch{};
   if (blocks == 0 || R == 0 || C == 0) return hipSuccess;
+  if (!(es == 1 || es == 2 || es == 4 || es == 8 || es == 16)) return hipErrorInvalidValue;
   if (dst_pitch == 0) dst_pitch = R * es;
   if (dst_pitch < R * es) return hipErrorInvalidValue;
-  KernelSpan span(st, "k_oop_transpose", 2.0 * double(blocks) * double(R) * double(C) * double(es));
   auto al = [](uint64_t v) { return (v & 15u) == 0; };
   const bool dst_al = al(reinterpret_cast<uintptr_t>(dst)) && al(dst_pitch) && al(dst_stride);
   const int64_t skew_knob = tuning("st_skew");
@@ -1013,13 +1019,35 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
     p.fix_rows = blocks * C;
     p.div_C = FastDiv64(C);
   }
-  hipError_t e = hipErrorInvalidValue;
+  if (out == 2 && es == 16) return hipErrorInvalidValue;  // 16-byte rows are always aligned
+  const uint64_t fix_blocks = out == 2 ? (p.fix_rows + 1 + kOopThreads - 1) / kOopThreads : 0;
+  if (p.per_xcd * 8 + fix_blocks >= (1ull << 32)) return hipErrorInvalidValue;
+  o->p = p;
+  o->lds = lds;
+  o->grid = p.per_xcd * 8 + fix_blocks;
+  o->es = int(es);
+  o->vin = vin;
+  o->out = out;
+  return hipSuccess;
+}
+
+hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, size_t es,
+                                uint64_t blocks, uint64_t R, uint64_t C, uint64_t src_stride,
+                                uint64_t dst_stride, size_t tile_bytes, hipStream_t st,
+                                uint64_t src_end, uint64_t dst_pitch, int nt) {
+  if (blocks == 0 || R == 0 || C == 0) return hipSuccess;
+  OopLaunch o;
+  hipError_t e = prepare_oop_transpose(src, dst, es, blocks, R, C, src_stride, dst_stride,
+                                       tile_bytes, src_end, dst_pitch, nt, &o);
+  if (e != hipSuccess) return e;
+  KernelSpan span(st, "k_oop_transpose", 2.0 * double(blocks) * double(R) * double(C) * double(es));
+  e = hipErrorInvalidValue;
   switch (es) {
-    case 1: e = launch_oop_es<1>(p, vin, out, lds, st); break;
-    case 2: e = launch_oop_es<2>(p, vin, out, lds, st); break;
-    case 4: e = launch_oop_es<4>(p, vin, out, lds, st); break;
-    case 8: e = launch_oop_es<8>(p, vin, out, lds, st); break;
-    case 16: e = launch_oop_es<16>(p, vin, out, lds, st); break;
+    case 1: e = launch_oop_es<1>(o.p, o.vin, o.out, o.lds, st); break;
+    case 2: e = launch_oop_es<2>(o.p, o.vin, o.out, o.lds, st); break;
+    case 4: e = launch_oop_es<4>(o.p, o.vin, o.out, o.lds, st); break;
+    case 8: e = launch_oop_es<8>(o.p, o.vin, o.out, o.lds, st); break;
+    case 16: e = launch_oop_es<16>(o.p, o.vin, o.out, o.lds, st); break;
   }
   return e;
 }
@@ -1036,23 +1064,13 @@ hipError_t launch_oop_transpose(const unsigned char *src, unsigned char *dst, si
 // Unit x of a segment is byte (x mod S) of super-element (row, column) = (x / S, c), or in
 // whole-column tiles ((x mod RS) / S, x / RS). A chunk only partly inside its segment (the
 // rest belongs to a neighbouring tile) is stored in aligned pieces of 8, 4, 2 and 1 bytes.
-struct GatherParams {
-  const unsigned char *src;  // 16-byte aligned
-  unsigned char *dst;
-  uint64_t inst_bytes;       // R * C * S
-  uint64_t R, C;
-  uint32_t S, TR, TC, LP;    // LP: LDS row pitch (multiple of 16)
-  uint32_t nch, nchd;        // 16-byte chunks per LDS row / per destination segment
-  uint32_t fullcol;          // TR == R: one destination segment per tile (kernel form FC)
-  FastDiv32 div_nch, div_nchd, div_S, div_RS;
-  uint64_t ntc, per_inst, tiles, per_xcd;
-};
 
+// Block bx of the launch's grid (see oop_block).
 template <int U, bool FC, int NT>
-__global__ __launch_bounds__(NT) void k_oop_gather(GatherParams p) {
+__device__ __forceinline__ void gather_block(const GatherParams &p, uint64_t bx,
+                                             unsigned char *lds) {
   using T = typename Elem<U>::T;
-  extern __shared__ __align__(16) unsigned char lds[];
-  const uint64_t t = uint64_t(blockIdx.x & 7u) * p.per_xcd + (blockIdx.x >> 3);
+  const uint64_t t = (bx & 7u) * p.per_xcd + (bx >> 3);
   if (t >= p.tiles) return;
   const uint64_t g = t / p.per_inst;
   const uint64_t rem = t - g * p.per_inst;
@@ -1153,6 +1171,12 @@ __global__ __launch_bounds__(NT) void k_oop_gather(GatherParams p) {
   }
 }
 
+template <int U, bool FC, int NT>
+__global__ __launch_bounds__(NT) void k_oop_gather(GatherParams p) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  gather_block<U, FC, NT>(p, blockIdx.x, lds);
+}
+
 namespace {
 constexpr int kGatherThreads = 512;
 
@@ -1174,12 +1198,12 @@ bool oop_gather_fits(uint64_t S) {
   return S >= 1 && S + 48 <= device_info().lds_per_block && S < (1ull << 24);
 }
 
-hipError_t launch_oop_gather(const unsigned char *src, unsigned char *dst, uint64_t S,
-                             uint64_t inst, uint64_t R, uint64_t C, size_t tile_bytes,
-                             hipStream_t st) {
+hipError_t prepare_oop_gather(const unsigned char *src, unsigned char *dst, uint64_t S,
+                              uint64_t inst, uint64_t R, uint64_t C, size_t tile_bytes,
+                              GatherLaunch *o) {
+  *o = GatherLaunch{};
   if (inst == 0 || R == 0 || C == 0) return hipSuccess;
   if ((reinterpret_cast<uintptr_t>(src) & 15u) || !oop_gather_fits(S)) return hipErrorInvalidValue;
-  KernelSpan span(st, "k_oop_gather", 2.0 * double(inst) * double(R) * double(C) * double(S));
   GatherParams p{};
   p.src = src;
   p.dst = dst;
@@ -1209,18 +1233,36 @@ hipError_t launch_oop_gather(const unsigned char *src, unsigned char *dst, uint6
   if (p.per_xcd * 8 >= (1ull << 32)) return hipErrorInvalidValue;
   uint64_t u = 16;
   while (u > 1 && ((S | reinterpret_cast<uintptr_t>(dst)) & (u - 1))) u >>= 1;
+  o->p = p;
+  o->lds = lds;
+  o->grid = p.per_xcd * 8;
+  o->u = int(u);
+  o->fc = p.fullcol != 0;
+  return hipSuccess;
+}
+
+hipError_t launch_oop_gather(const unsigned char *src, unsigned char *dst, uint64_t S,
+                             uint64_t inst, uint64_t R, uint64_t C, size_t tile_bytes,
+                             hipStream_t st) {
+  if (inst == 0 || R == 0 || C == 0) return hipSuccess;
+  GatherLaunch o;
+  hipError_t e0 = prepare_oop_gather(src, dst, S, inst, R, C, tile_bytes, &o);
+  if (e0 != hipSuccess) return e0;
+  KernelSpan span(st, "k_oop_gather", 2.0 * double(inst) * double(R) * double(C) * double(S));
+  const GatherParams &p = o.p;
+  const size_t lds = o.lds;
   auto go = [&](auto kern) -> hipError_t {
     if (lds > 65536) {
       hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
       if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(uint32_t(p.per_xcd * 8)), dim3(kGatherThreads), lds, st, p);
+    hipLaunchKernelGGL(kern, dim3(uint32_t(o.grid)), dim3(kGatherThreads), lds, st, p);
     return hipGetLastError();
   };
   auto by_u = [&](auto fc) -> hipError_t {
     constexpr bool F = decltype(fc)::value;
-    switch (u) {
+    switch (o.u) {
       case 16: return go(k_oop_gather<16, F, kGatherThreads>);
       case 8: return go(k_oop_gather<8, F, kGatherThreads>);
       case 4: return go(k_oop_gather<4, F, kGatherThreads>);
@@ -1228,7 +1270,7 @@ hipError_t launch_oop_gather(const unsigned char *src, unsigned char *dst, uint6
       default: return go(k_oop_gather<1, F, kGatherThreads>);
     }
   };
-  return p.fullcol ? by_u(std::true_type{}) : by_u(std::false_type{});
+  return o.fc ? by_u(std::true_type{}) : by_u(std::false_type{});
 }
 
 // ---- host: planning ---------------------------------------------------------------------
@@ -1363,5 +1405,145 @@ hipError_t run_st(unsigned char *buf, const StPlan &p, unsigned char *scratch, s
 #undef ST_TRY
   return hipSuccess;
 }
+
+// ---- grouped forms (a batch's through-scratch rounds) -------------------------------------
+// Many launches of the kernels above as one: every workgroup walks the units of the work
+// list (each item's launch grid, padded to a multiple of 8), finds its item by binary search
+// and runs that launch's block. The grid is a multiple of 8, so a unit keeps the XCD it
+// would have had in its own launch.
+template <int ES, bool VIN, int OUT>
+__global__ __launch_bounds__(kOopGroupThreads) void kg_oop_transpose(const OopParams *__restrict__ items,
+                                                                     const uint64_t *__restrict__ prefix,
+                                                                     uint32_t n, uint64_t total) {
+  static_assert(kOopGroupThreads == kOopThreads, "fix-up units are sized for kOopThreads");
+  extern __shared__ __align__(16) unsigned char lds[];
+  for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
+    const uint32_t it = find_item(prefix, n, q);
+    const OopParams p = items[it];
+    oop_block<ES, VIN, OUT, kOopGroupThreads>(p, q - prefix[it], lds);
+    __syncthreads();  // the next unit refills LDS
+  }
+}
+
+template <int U, bool FC>
+__global__ __launch_bounds__(kOopGroupThreads) void kg_oop_gather(const GatherParams *__restrict__ items,
+                                                                  const uint64_t *__restrict__ prefix,
+                                                                  uint32_t n, uint64_t total) {
+  static_assert(kOopGroupThreads == kGatherThreads, "one block shape for both");
+  extern __shared__ __align__(16) unsigned char lds[];
+  for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
+    const uint32_t it = find_item(prefix, n, q);
+    const GatherParams p = items[it];
+    gather_block<U, FC, kOopGroupThreads>(p, q - prefix[it], lds);
+    __syncthreads();
+  }
+}
+
+// Unit u of an item copies bytes [u * kCopyItemUnit, ...) of it: 16-byte vectors, four per
+// lane in flight (non-temporal loads, cached stores: the arena is read back next), the last
+// unit's bytes past its last whole vector one per lane.
+__global__ __launch_bounds__(256) void kg_copy(const CopyItem *__restrict__ items, const uint64_t *__restrict__ prefix,
+                                               uint32_t n, uint64_t total) {
+  static_assert(kCopyItemUnit == 256 * 16 * kCopyUnroll, "one vector per lane per round");
+  for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
+    const uint32_t it = find_item(prefix, n, q);
+    const CopyItem c = items[it];
+    const uint64_t b0 = (q - prefix[it]) * kCopyItemUnit;
+    if (b0 >= c.bytes) continue;
+    const uint64_t len = std::min<uint64_t>(kCopyItemUnit, c.bytes - b0);
+    const uint32_t nv = uint32_t(len / 16);
+    const U128 *s = reinterpret_cast<const U128 *>(c.src + b0);
+    U128 *d = reinterpret_cast<U128 *>(c.dst + b0);
+    U128 v[kCopyUnroll];
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; u++) {
+      const uint32_t i = threadIdx.x + 256u * u;
+      if (i < nv) v[u] = __builtin_nontemporal_load(s + i);  // the matrix is read once
+    }
+#pragma unroll
+    for (int u = 0; u < kCopyUnroll; u++) {
+      const uint32_t i = threadIdx.x + 256u * u;
+      if (i < nv) d[i] = v[u];
+    }
+    const uint32_t tail = uint32_t(len) - 16u * nv;
+    if (threadIdx.x < tail) c.dst[b0 + 16u * nv + threadIdx.x] = c.src[b0 + 16u * nv + threadIdx.x];
+  }
+}
+
+// Unit u of an item: round u of its unpitch copy; unit 0 also copies the end bytes.
+template <bool NTS>
+__global__ __launch_bounds__(256) void kg_unpitch(const UnpitchItem *__restrict__ items,
+                                                  const uint64_t *__restrict__ prefix, uint32_t n,
+                                                  uint64_t total) {
+  static_assert(kUnpitchItemUnit == 256 * kCopyUnroll, "one round per unit");
+  for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
+    const uint32_t it = find_item(prefix, n, q);
+    const UnpitchItem c = items[it];
+    const uint64_t u = q - prefix[it];
+    if (u * kUnpitchItemUnit < c.nb)
+      unpitch_round<NTS>(c.dst, c.src, c.RB, c.P, c.head, c.nb, c.div_rb, u);
+    if (u == 0 && threadIdx.x == 0)
+      unpitch_ends(c.dst, c.src, c.RB, c.P, c.total, c.head, c.nb, c.div_rb);
+  }
+}
+
+UnpitchItem make_unpitch_item(unsigned char *dst, const unsigned char *src, uint64_t RB,
+                              uint64_t P, uint64_t rows) {
+  UnpitchItem c{};
+  c.dst = dst;
+  c.src = src;
+  c.RB = RB;
+  c.P = P;
+  c.total = RB * rows;
+  c.head = uint32_t(std::min<uint64_t>((16u - (reinterpret_cast<uintptr_t>(dst) & 15u)) & 15u,
+                                       c.total));
+  c.nb = (c.total - c.head) / 16;
+  c.div_rb = FastDiv64(RB);
+  return c;
+}
+
+const void *kg_unpitch_func(bool nts) {
+  return nts ? reinterpret_cast<const void *>(kg_unpitch<true>)
+             : reinterpret_cast<const void *>(kg_unpitch<false>);
+}
+
+namespace {
+template <int ES>
+const void *kg_oop_es(bool vin, int out) {
+  if (out == 2)
+    return ES == 16 ? nullptr
+           : vin    ? reinterpret_cast<const void *>(kg_oop_transpose<ES, true, 2>)
+                    : reinterpret_cast<const void *>(kg_oop_transpose<ES, false, 2>);
+  if (vin && out) return reinterpret_cast<const void *>(kg_oop_transpose<ES, true, 1>);
+  if (vin) return reinterpret_cast<const void *>(kg_oop_transpose<ES, true, 0>);
+  if (out) return reinterpret_cast<const void *>(kg_oop_transpose<ES, false, 1>);
+  return reinterpret_cast<const void *>(kg_oop_transpose<ES, false, 0>);
+}
+template <bool FC>
+const void *kg_gather_fc(int u) {
+  switch (u) {
+    case 16: return reinterpret_cast<const void *>(kg_oop_gather<16, FC>);
+    case 8: return reinterpret_cast<const void *>(kg_oop_gather<8, FC>);
+    case 4: return reinterpret_cast<const void *>(kg_oop_gather<4, FC>);
+    case 2: return reinterpret_cast<const void *>(kg_oop_gather<2, FC>);
+    default: return reinterpret_cast<const void *>(kg_oop_gather<1, FC>);
+  }
+}
+}  // namespace
+
+const void *kg_oop_transpose_func(int es, bool vin, int out) {
+  switch (es) {
+    case 1: return kg_oop_es<1>(vin, out);
+    case 2: return kg_oop_es<2>(vin, out);
+    case 4: return kg_oop_es<4>(vin, out);
+    case 8: return kg_oop_es<8>(vin, out);
+    case 16: return kg_oop_es<16>(vin, out);
+    default: return nullptr;
+  }
+}
+const void *kg_oop_gather_func(int u, bool fc) {
+  return fc ? kg_gather_fc<true>(u) : kg_gather_fc<false>(u);
+}
+const void *kg_copy_func() { return reinterpret_cast<const void *>(kg_copy); }
 
 }  // namespace lm

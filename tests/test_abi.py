@@ -118,7 +118,8 @@ def test_tuning_keys_defaults_and_limits():
                 "copy_blocks_per_cu": 64, "copy_nt_store_mb": 256, "oop_pitched": 1, "oop_min_rows": 256, "oop_min_tiles": 512, "st_super_bytes": 8192,
                 "st_max_height": 4096,
                 "autotune": 0, "oop_order": 2, "cl_max_wgs": 16, "cl_min_bytes": 256 << 10,
-                "cl_min_inst": 64, "cl_fuse": 1, "cl_spin": 1 << 22}
+                "cl_min_inst": 64, "cl_fuse": 1, "cl_spin": 1 << 22,
+                "batch_arena_mb": 1024, "batch_arena_max_mb": 16}
     for k, v in defaults.items():
         assert lm.get_tuning(k) == v, k
     assert lm.get_tuning("no_such_key") == 0
@@ -134,7 +135,7 @@ def test_tuning_keys_defaults_and_limits():
     lm.set_tuning("oop_elem_tile_kb", 8)
     with pytest.raises(ValueError):
         lm.set_tuning("copy_blocks_per_cu", 0)
-    for k in ("cl_max_wgs", "cl_fuse", "oop_order"):  # 0 switches these off
+    for k in ("cl_max_wgs", "cl_fuse", "oop_order", "batch_arena_mb"):  # 0 switches these off
         keep = lm.get_tuning(k)
         lm.set_tuning(k, 0)
         assert lm.get_tuning(k) == 0

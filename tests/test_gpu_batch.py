@@ -169,3 +169,44 @@ def test_batch_with_stripe_route_matrices(hip, orc, use_graph):
         for i, (b, w) in enumerate(zip(bufs, want)):
             assert np.array_equal(b.cpu().numpy(), w), (replay, i, items[i][1:], batch.route(i))
     batch.close()
+
+
+@pytest.mark.parametrize("arena_mb", [1024, 1])
+def test_batch_through_scratch_rounds(hip, orc, arena_mb):
+    """Matrices whose best route is not one streaming pass go through the batch's arena: one
+    grouped copy per round, then grouped out-of-place transposes (aligned, misaligned and
+    skewed destination rows) or super-element gathers (SoA tiles of 3 to 64 elements) back.
+    With a 1 MiB round the same matrices take many rounds (and the larger ones other paths);
+    every buffer must match the oracle on every replay."""
+    import torch
+    shapes = [(0, 1, 333, 517, 0), (0, 2, 1001, 77, 0), (0, 4, 129, 4099, 0), (0, 8, 700, 31, 0),
+              (0, 16, 97, 211, 0), (0, 1, 2048, 1500, 0), (0, 2, 17, 3000, 0),
+              (2, 4, 96 * 3, 50, 3), (2, 1, 640, 33, 5), (2, 8, 1280, 7, 64), (2, 2, 330, 19, 11),
+              (1, 1, 1536, 700, 3), (1, 16, 640, 40, 5), (1, 2, 4096, 300, 64),
+              (0, 8, 16, 8192, 0), (0, 1, 5000, 16, 0)]
+    items = []
+    for k, (op, es, r, c, t) in enumerate(shapes):
+        items.append((orc.tagged(r * c, es, seed=700 + k), op, es, r, c, t))
+    want = [expected(orc, it) for it in items]
+    bufs = upload(items)
+    keep = hip.get_tuning("batch_arena_mb")
+    hip.set_tuning("batch_arena_mb", arena_mb)
+    try:
+        batch = hip.Batch([(b, op, es, r, c, t) for b, (_, op, es, r, c, t) in zip(bufs, items)])
+    finally:
+        hip.set_tuning("batch_arena_mb", keep)
+    say("arena", arena_mb, batch.info("arena_matrices"), batch.info("arena_rounds"),
+        batch.info("arena_bytes"), [batch.route(i) for i in range(len(items))])
+    assert batch.info("arena_matrices") >= (8 if arena_mb > 1 else 1)
+    if arena_mb == 1:
+        assert batch.info("arena_rounds") >= 2
+    for replay in range(3):
+        if replay:
+            for b, it in zip(bufs, items):
+                b.copy_(torch.from_numpy(it[0]).cuda())
+            torch.cuda.synchronize()
+        assert batch.run() is False
+        torch.cuda.synchronize()
+        for i, (b, w) in enumerate(zip(bufs, want)):
+            assert np.array_equal(b.cpu().numpy(), w), (replay, i, items[i][1:], batch.route(i))
+    batch.close()
