@@ -135,6 +135,7 @@ Knob g_knobs[] = {
     {"batch_solo_lanes", {4}},
     {"batch_arena_mb", {1024}, 0},    // a batch's through-scratch round (0: no rounds)
     {"batch_arena_max_mb", {16}, 0},  // largest matrix taken into them
+    {"batch_level_lanes", {4}},       // branch streams for the groups of one level (1: none)
     {"serve_scratch_mb", {8192}, 0},  // per-request scratch cap of server requests
     {"seg_len", {16}},           // slots per segment of the segmented cycle walk
     {"seg_min_bytes", {64}},     // super-elements of at least this many bytes take it

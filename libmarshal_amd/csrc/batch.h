@@ -56,12 +56,18 @@ struct BatchPlan {
 };
 constexpr int kSoloLanes = 8;  // at most; the "batch_solo_lanes" knob picks how many
 
-// Side streams, events and scratch for the solo lists of a batch.
+constexpr int kLevelLanes = 8;  // at most; the "batch_level_lanes" knob picks how many
+
+// Side streams, events and scratch for the solo lists of a batch, and the branch streams on
+// which the independent groups of one level run side by side (joined before the next level).
 struct SoloExec {
   hipStream_t st[kSoloLanes] = {};
   hipEvent_t join[kSoloLanes] = {};
   unsigned char *scr[kSoloLanes] = {};
   hipEvent_t fork = nullptr;
+  int level_lanes = 0;  // branch streams created (0: every group on the run's stream)
+  hipStream_t lst[kLevelLanes] = {};
+  hipEvent_t lfork[kLevelLanes] = {}, ljoin[kLevelLanes] = {};
 };
 
 // allow_solo: matrices best served by the stripe route go to `solo` (the batched form);

@@ -334,14 +334,31 @@ bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_
   }
   if (b.flag_words) LM_CHECK_HIP(launch_fill_u32(flags, b.flag_words, 0, st));
   unsigned char *base = static_cast<unsigned char *>(params);
+  const int lanes = solo ? solo->level_lanes : 0;
   for (auto &level : b.levels) {
-    for (const Group &g : level) {
+    // The groups of one level touch disjoint matrices: with branch streams they run side by
+    // side (a small group's few workgroups no longer wait for the groups before it), and the
+    // level ends when every branch has joined the run's stream again.
+    const int used = std::min<int>(lanes, int(level.size()));
+    if (used > 1) {
+      LM_CHECK_HIP(hipEventRecord(solo->lfork[0], st));
+      for (int l = 0; l < used; l++) LM_CHECK_HIP(hipStreamWaitEvent(solo->lst[l], solo->lfork[0], 0));
+    }
+    for (size_t gi = 0; gi < level.size(); gi++) {
+      const Group &g = level[gi];
       const void *items = base + g.items_off;
       const uint64_t *prefix = reinterpret_cast<const uint64_t *>(base + g.prefix_off);
       uint32_t n = uint32_t(g.prefix.size() - 1);
       uint64_t total = g.prefix.back();
       void *args[] = {&items, &prefix, &n, &total};
-      LM_CHECK_HIP(hipLaunchKernel(g.func, dim3(g.grid), dim3(g.block), args, g.lds, st));
+      hipStream_t gs = used > 1 ? solo->lst[gi % size_t(used)] : st;
+      LM_CHECK_HIP(hipLaunchKernel(g.func, dim3(g.grid), dim3(g.block), args, g.lds, gs));
+    }
+    if (used > 1) {
+      for (int l = 0; l < used; l++) {
+        LM_CHECK_HIP(hipEventRecord(solo->ljoin[l], solo->lst[l]));
+        LM_CHECK_HIP(hipStreamWaitEvent(st, solo->ljoin[l], 0));
+      }
     }
   }
   if (fork)
@@ -723,13 +740,13 @@ bool plan_batch(const hip_marshal_desc *descs, int64_t count, BatchPlan &b, cons
                     (long long)a.desc);
             return false;
           }
+          const UnpitchItem u = make_unpitch_item(a.buf, scr, row_bytes, a.P, a.inst * a.C);
+          const int nts = (tuning("oop_nt") & 2) ? 1 : 0;
           GroupKey gk{10, o.es, o.vin ? 1 : 0, o.out};
           Group &g = fwd[gk];
           g.key = gk;
           g.lds = std::max(g.lds, o.lds);
           add_item(g, o.p, round8(o.grid));
-          const UnpitchItem u = make_unpitch_item(a.buf, scr, row_bytes, a.P, a.inst * a.C);
-          const int nts = (tuning("oop_nt") & 2) ? 1 : 0;
           GroupKey uk{11, nts, 0, 0};
           Group &ug = back[uk];
           ug.key = uk;
@@ -913,6 +930,18 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
              hipSuccess;
     }
   }
+  // branch streams for levels of more than one group
+  size_t widest = 0;
+  for (auto &l : b.levels) widest = std::max(widest, l.size());
+  const int want = int(std::min<int64_t>(kLevelLanes, tuning("batch_level_lanes")));
+  if (ok && widest > 1 && want > 1) {
+    const int n = int(std::min<size_t>(size_t(want), widest));
+    for (int l = 0; ok && l < n; l++)
+      ok = hipStreamCreateWithFlags(&hb->solo.lst[l], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&hb->solo.lfork[l], hipEventDisableTiming) == hipSuccess &&
+           hipEventCreateWithFlags(&hb->solo.ljoin[l], hipEventDisableTiming) == hipSuccess;
+    if (ok) hb->solo.level_lanes = n;
+  }
   if (ok && b.flag_words) ok = hipMalloc(&hb->flags, b.flag_words * 4) == hipSuccess;
   if (ok && b.arena_bytes)
     ok = hipMalloc(reinterpret_cast<void **>(&hb->arena), b.arena_bytes) == hipSuccess;
@@ -1011,6 +1040,12 @@ void hip_marshal_batch_destroy(hip_marshal_batch *hb) {
     if (hb->solo.st[l]) hipStreamDestroy(hb->solo.st[l]);
   }
   if (hb->solo.fork) hipEventDestroy(hb->solo.fork);
+  for (int l = 0; l < kLevelLanes; l++) {
+    if (hb->solo.lst[l]) hipStreamSynchronize(hb->solo.lst[l]);
+    if (hb->solo.lfork[l]) hipEventDestroy(hb->solo.lfork[l]);
+    if (hb->solo.ljoin[l]) hipEventDestroy(hb->solo.ljoin[l]);
+    if (hb->solo.lst[l]) hipStreamDestroy(hb->solo.lst[l]);
+  }
   delete hb;
 }
 

@@ -1439,34 +1439,40 @@ __global__ __launch_bounds__(kOopGroupThreads) void kg_oop_gather(const GatherPa
   }
 }
 
-// Unit u of an item copies bytes [u * kCopyItemUnit, ...) of it: 16-byte vectors, four per
-// lane in flight (non-temporal loads, cached stores: the arena is read back next), the last
-// unit's bytes past its last whole vector one per lane.
+// Unit u of a copy item: bytes [u * kCopyItemUnit, ...) of it as 16-byte vectors, all of a
+// lane's in flight together (non-temporal loads, cached stores: the arena is read back next),
+// the last unit's bytes past its last whole vector one per lane. NT lanes.
+template <int NT>
+__device__ __forceinline__ void copy_unit(const CopyItem &c, uint64_t u) {
+  constexpr int UN = int(kCopyItemUnit / 16) / NT;
+  static_assert(uint64_t(UN) * NT * 16 == kCopyItemUnit, "whole vectors per lane");
+  const uint64_t b0 = u * kCopyItemUnit;
+  if (b0 >= c.bytes) return;
+  const uint64_t len = std::min<uint64_t>(kCopyItemUnit, c.bytes - b0);
+  const uint32_t nv = uint32_t(len / 16);
+  const U128 *s = reinterpret_cast<const U128 *>(c.src + b0);
+  U128 *d = reinterpret_cast<U128 *>(c.dst + b0);
+  U128 v[UN];
+#pragma unroll
+  for (int k = 0; k < UN; k++) {
+    const uint32_t i = threadIdx.x + uint32_t(NT) * k;
+    if (i < nv) v[k] = __builtin_nontemporal_load(s + i);  // the matrix is read once
+  }
+#pragma unroll
+  for (int k = 0; k < UN; k++) {
+    const uint32_t i = threadIdx.x + uint32_t(NT) * k;
+    if (i < nv) d[i] = v[k];
+  }
+  const uint32_t tail = uint32_t(len) - 16u * nv;
+  if (threadIdx.x < tail) c.dst[b0 + 16u * nv + threadIdx.x] = c.src[b0 + 16u * nv + threadIdx.x];
+}
+
 __global__ __launch_bounds__(256) void kg_copy(const CopyItem *__restrict__ items, const uint64_t *__restrict__ prefix,
                                                uint32_t n, uint64_t total) {
-  static_assert(kCopyItemUnit == 256 * 16 * kCopyUnroll, "one vector per lane per round");
   for (uint64_t q = blockIdx.x; q < total; q += gridDim.x) {
     const uint32_t it = find_item(prefix, n, q);
     const CopyItem c = items[it];
-    const uint64_t b0 = (q - prefix[it]) * kCopyItemUnit;
-    if (b0 >= c.bytes) continue;
-    const uint64_t len = std::min<uint64_t>(kCopyItemUnit, c.bytes - b0);
-    const uint32_t nv = uint32_t(len / 16);
-    const U128 *s = reinterpret_cast<const U128 *>(c.src + b0);
-    U128 *d = reinterpret_cast<U128 *>(c.dst + b0);
-    U128 v[kCopyUnroll];
-#pragma unroll
-    for (int u = 0; u < kCopyUnroll; u++) {
-      const uint32_t i = threadIdx.x + 256u * u;
-      if (i < nv) v[u] = __builtin_nontemporal_load(s + i);  // the matrix is read once
-    }
-#pragma unroll
-    for (int u = 0; u < kCopyUnroll; u++) {
-      const uint32_t i = threadIdx.x + 256u * u;
-      if (i < nv) d[i] = v[u];
-    }
-    const uint32_t tail = uint32_t(len) - 16u * nv;
-    if (threadIdx.x < tail) c.dst[b0 + 16u * nv + threadIdx.x] = c.src[b0 + 16u * nv + threadIdx.x];
+    copy_unit<256>(c, q - prefix[it]);
   }
 }
 

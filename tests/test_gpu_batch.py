@@ -171,13 +171,14 @@ def test_batch_with_stripe_route_matrices(hip, orc, use_graph):
     batch.close()
 
 
-@pytest.mark.parametrize("arena_mb", [1024, 1])
-def test_batch_through_scratch_rounds(hip, orc, arena_mb):
+@pytest.mark.parametrize("arena_mb,lanes", [(1024, 4), (1, 4), (1024, 1), (1, 1)])
+def test_batch_through_scratch_rounds(hip, orc, arena_mb, lanes):
     """Matrices whose best route is not one streaming pass go through the batch's arena: one
     grouped copy per round, then grouped out-of-place transposes (aligned, misaligned and
     skewed destination rows) or super-element gathers (SoA tiles of 3 to 64 elements) back.
     With a 1 MiB round the same matrices take many rounds (and the larger ones other paths);
-    every buffer must match the oracle on every replay."""
+    every buffer must match the oracle on every replay, with the groups of a level on branch
+    streams and all on one stream."""
     import torch
     shapes = [(0, 1, 333, 517, 0), (0, 2, 1001, 77, 0), (0, 4, 129, 4099, 0), (0, 8, 700, 31, 0),
               (0, 16, 97, 211, 0), (0, 1, 2048, 1500, 0), (0, 2, 17, 3000, 0),
@@ -189,12 +190,14 @@ def test_batch_through_scratch_rounds(hip, orc, arena_mb):
         items.append((orc.tagged(r * c, es, seed=700 + k), op, es, r, c, t))
     want = [expected(orc, it) for it in items]
     bufs = upload(items)
-    keep = hip.get_tuning("batch_arena_mb")
+    keep = {k: hip.get_tuning(k) for k in ("batch_arena_mb", "batch_level_lanes")}
     hip.set_tuning("batch_arena_mb", arena_mb)
+    hip.set_tuning("batch_level_lanes", lanes)
     try:
         batch = hip.Batch([(b, op, es, r, c, t) for b, (_, op, es, r, c, t) in zip(bufs, items)])
     finally:
-        hip.set_tuning("batch_arena_mb", keep)
+        for k, v in keep.items():
+            hip.set_tuning(k, v)
     say("arena", arena_mb, batch.info("arena_matrices"), batch.info("arena_rounds"),
         batch.info("arena_bytes"), [batch.route(i) for i in range(len(items))])
     assert batch.info("arena_matrices") >= (8 if arena_mb > 1 else 1)
