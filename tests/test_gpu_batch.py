@@ -190,6 +190,13 @@ def test_batch_through_scratch_rounds(hip, orc, arena_mb, lanes):
         items.append((orc.tagged(r * c, es, seed=700 + k), op, es, r, c, t))
     want = [expected(orc, it) for it in items]
     bufs = upload(items)
+    # one buffer 8 bytes past a 16-byte boundary: never taken into the arena
+    x = orc.tagged(300 * 257, 8, seed=799)
+    holder = torch.zeros(300 * 257 * 8 + 8, dtype=torch.uint8, device="cuda")
+    holder[8:].copy_(torch.from_numpy(x.view(np.uint8)).cuda())
+    items.append((x, 0, 8, 300, 257, 0))
+    want.append(orc.op_transpose(x, 300, 257, es=8))
+    bufs.append(holder[8:])
     keep = {k: hip.get_tuning(k) for k in ("batch_arena_mb", "batch_level_lanes")}
     hip.set_tuning("batch_arena_mb", arena_mb)
     hip.set_tuning("batch_level_lanes", lanes)
@@ -201,6 +208,7 @@ def test_batch_through_scratch_rounds(hip, orc, arena_mb, lanes):
     say("arena", arena_mb, batch.info("arena_matrices"), batch.info("arena_rounds"),
         batch.info("arena_bytes"), [batch.route(i) for i in range(len(items))])
     assert batch.info("arena_matrices") >= (8 if arena_mb > 1 else 1)
+    assert batch.route(len(items) - 1) != "ARENA-"
     if arena_mb == 1:
         assert batch.info("arena_rounds") >= 2
     for replay in range(3):
