@@ -144,9 +144,12 @@ def test_several_scratch_fills_per_call(hip, orc):
             assert np.array_equal(host(d), orc.op_0100(x, A, a, B, b, es=es)), (es, A, a, B, b)
 
 
-def test_batch_runs_scratch_steps_on_side_streams(hip, orc):
+@pytest.mark.parametrize("arena_mb", [0, 1024])
+def test_batch_runs_scratch_steps_on_side_streams(hip, orc, arena_mb):
     """Large AoS / SoA conversions whose plain route goes through scratch run beside the
-    grouped small ones; every buffer matches the oracle on every replay."""
+    grouped small ones: on side streams with the batch's through-scratch rounds off, in those
+    rounds with them on (the four are at most 10 MB); every buffer matches the oracle on every
+    replay."""
     import torch
     items = []
     for k, (op, es, r, c, t) in enumerate([(hip.OP_AOS_ASTA, 4, 640, 4096, 64),
@@ -168,12 +171,20 @@ def test_batch_runs_scratch_steps_on_side_streams(hip, orc):
 
     want = [expected(it) for it in items]
     bufs = [dev(it[0]) for it in items]
-    batch = hip.Batch([(b, op, es, r, c, t) for b, (_, op, es, r, c, t) in zip(bufs, items)],
-                      use_graph=True)
+    keep = hip.get_tuning("batch_arena_mb")
+    hip.set_tuning("batch_arena_mb", arena_mb)
+    try:
+        batch = hip.Batch([(b, op, es, r, c, t) for b, (_, op, es, r, c, t) in zip(bufs, items)],
+                          use_graph=True)
+    finally:
+        hip.set_tuning("batch_arena_mb", keep)
     routes = [batch.route(i) for i in range(len(items))]
-    say("routes", routes, "solo", batch.info("solo"))
-    assert batch.info("solo") >= 4
-    assert all("OOP" in routes[i] for i in range(4)), routes
+    say("routes", routes, "solo", batch.info("solo"), "arena", batch.info("arena_matrices"))
+    if arena_mb:
+        assert all(routes[i] == "ARENA-" for i in range(4)), routes
+    else:
+        assert batch.info("solo") >= 4
+        assert all("OOP" in routes[i] for i in range(4)), routes
     for replay in range(3):
         if replay:
             for b, it in zip(bufs, items):
