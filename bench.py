@@ -56,6 +56,10 @@ def parse():
                         "the cycle-following 010 kernel, after one untimed round trip")
     p.add_argument("--no-large", action="store_true",
                    help="skip the 103 GB and 50 GB prime-dimension transposes")
+    p.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                   help="library tuning knob for a profiling run (repeatable), e.g. "
+                        "pt010_lead_order=0 to take the visited-bit form of the "
+                        "cycle-following 010")
     p.add_argument("--profile", type=int, default=0,
                    help="only run N AoS->ASTA->AoS round trips (2N LDS-staged launches, as "
                         "in the timed step) and N plain 1 GiB device copies (the calibration "
@@ -364,6 +368,9 @@ def main():
 
     torch.cuda.set_device(0)
     lm.set_profiling(False)  # asynchronous calls: no per-call synchronization
+    for kv in args.set:
+        key, _, value = kv.partition("=")
+        lm.set_tuning(key, int(value))
     if args.profile_st or args.profile_st_u8:
         c1 = torch.rand(1 << 28, device="cuda")  # 1 GiB
         c2 = torch.empty_like(c1)

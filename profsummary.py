@@ -7,6 +7,13 @@
     python profsummary.py route  <fetch.csv> <write.csv> <copy-bytes> <transposes>
                                  <workload> <out.json> <command>...
 
+    python profsummary.py counters <counter_collection.csv> <out.txt> <command...>
+
+`counters` averages every collected counter per dispatch of each library kernel (and of the
+calibration copy) and adds the LDS bank-conflict ratio
+SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE - SQ_LDS_BANK_CONFLICT) (stall cycles per cycle of
+useful LDS work) and the stall cycles per LDS instruction, where those counters are present.
+
 `pmc` scales the FETCH_SIZE / WRITE_SIZE readings (KiB per dispatch) of the named kernel by
 the factor that maps the same counter on the calibration copy kernel (__amd_rocclr_copyBuffer,
 a device copy of <copy-bytes> bytes: it reads and writes exactly that many) to its known
@@ -17,6 +24,36 @@ import csv
 import json
 import statistics
 import sys
+from collections import defaultdict
+
+
+def counters(csv_path, out_path, command):
+    rows = list(csv.DictReader(open(csv_path)))
+    # (kernel, dispatch) -> counter -> value (a counter can come as several rows: sum them)
+    per = defaultdict(lambda: defaultdict(float))
+    for r in rows:
+        name = r["Kernel_Name"]
+        if "lm::" not in name and "copyBuffer" not in name:
+            continue
+        per[(name, r["Dispatch_Id"])][r["Counter_Name"]] += float(r["Counter_Value"])
+    by_kernel = defaultdict(list)
+    for (name, _), vals in per.items():
+        by_kernel[name].append(vals)
+    lines = ["# rocprofv3 counters, average per dispatch", f"# command: {command}", ""]
+    for name, ds in sorted(by_kernel.items(), key=lambda kv: -len(kv[1])):
+        keys = sorted({k for d in ds for k in d})
+        avg = {k: statistics.mean(d.get(k, 0.0) for d in ds) for k in keys}
+        lines.append(f"{name[:110]}  ({len(ds)} dispatches)")
+        for k in keys:
+            lines.append(f"    {k:32s} {avg[k]:16.1f}")
+        bc, act = avg.get("SQ_LDS_BANK_CONFLICT"), avg.get("SQ_LDS_IDX_ACTIVE")
+        if bc is not None and act and act > bc:
+            lines.append(f"    {'bank-conflict stall / LDS work':32s} {bc / (act - bc):16.4f}")
+        n = avg.get("SQ_INSTS_LDS")
+        if bc is not None and n:
+            lines.append(f"    {'stall cycles per LDS instruction':32s} {bc / n:16.3f}")
+        lines.append("")
+    open(out_path, "w").write("\n".join(lines))
 
 
 def stats(csv_path, out_path, command):
@@ -98,6 +135,8 @@ if __name__ == "__main__":
         stats(sys.argv[2], sys.argv[3], " ".join(sys.argv[4:]))
     elif sys.argv[1] == "pmc":
         print(pmc(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]), sys.argv[6]))
+    elif sys.argv[1] == "counters":
+        counters(sys.argv[2], sys.argv[3], " ".join(sys.argv[4:]))
     elif sys.argv[1] == "route":
         # route <fetch.csv> <write.csv> <copy-bytes> <transposes> <workload> <out.json> <cmd>...
         print(route(sys.argv[2], sys.argv[3], int(sys.argv[4]), int(sys.argv[5]), sys.argv[6],
