@@ -5,10 +5,21 @@
 // level, stages that use the same kernel form (and template instantiation) form a group,
 // executed by ONE persistent kernel over a work list: each workgroup (or wave) claims work
 // units (chunks, tiles, instances, walker sets) of any matrix in the group, looks up its
-// matrix by binary search over a prefix sum of units, and runs the same device body as the
-// single-matrix kernel. Levels are ordered; groups inside a level are independent. The whole
-// sequence - clearing every matrix's visited bits, then the group kernels level by level - is
-// captured once into a hipGraph, so a run is one graph launch.
+// matrix from a prefix sum of units (find_item), and runs the same device body as the
+// single-matrix kernel. Levels are ordered; groups inside a level are independent and run on
+// branch streams side by side. The whole sequence - clearing every matrix's visited bits,
+// then the group kernels level by level - is captured once into a hipGraph, so a run is one
+// graph launch.
+//
+// Three kinds of matrix share a run:
+// - grouped in-place stages (the levels above);
+// - through-scratch rounds: matrices of up to "batch_arena_max_mb" whose best route is not a
+//   single streaming pass are copied (or, pitched, transposed) into a batch-owned arena by
+//   one grouped launch per round, then moved back out of place by grouped launches; rounds
+//   of at most "batch_arena_mb" follow one another on the run's stream;
+// - side-stream matrices (stripe route, and from "batch_solo_kb" the square, cycle-leader,
+//   cluster, segmented and larger through-scratch forms) run their plain stage lists on
+//   side streams forked from and joined to the run's stream inside the same graph.
 //
 // No counterpart in the reference, which launches one kernel per stage per matrix and waits
 // after each (src/cl_marshal.cc:561-776, include/cl_profile.h:23-24).
