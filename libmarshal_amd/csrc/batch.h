@@ -10,7 +10,10 @@
 
 namespace lm {
 
-// Kernel form + template instantiation of a group.
+// Kernel form + template instantiation of a group. kind: 0 LDS-staged 010, 1 010 cycle
+// following with LDS bits, 2 100 with global bits, 3 0100 with LDS bits, 4-6 row/column
+// passes; through-scratch rounds: 7 copy into the arena, 8 out-of-place transpose from it,
+// 9 gather from it, 10 pitched transpose into it, 11 unpitch copy from it.
 struct GroupKey {
   int kind, a, b, c;
   bool operator<(const GroupKey &o) const {
@@ -67,7 +70,7 @@ struct SoloExec {
   hipEvent_t fork = nullptr;
   int level_lanes = 0;  // branch streams created (0: every group on the run's stream)
   hipStream_t lst[kLevelLanes] = {};
-  hipEvent_t lfork[kLevelLanes] = {}, ljoin[kLevelLanes] = {};
+  hipEvent_t lfork = nullptr, ljoin[kLevelLanes] = {};
 };
 
 // allow_solo: matrices best served by the stripe route go to `solo` (the batched form);

@@ -352,8 +352,8 @@ bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_
     // level ends when every branch has joined the run's stream again.
     const int used = std::min<int>(lanes, int(level.size()));
     if (used > 1) {
-      LM_CHECK_HIP(hipEventRecord(solo->lfork[0], st));
-      for (int l = 0; l < used; l++) LM_CHECK_HIP(hipStreamWaitEvent(solo->lst[l], solo->lfork[0], 0));
+      LM_CHECK_HIP(hipEventRecord(solo->lfork, st));
+      for (int l = 0; l < used; l++) LM_CHECK_HIP(hipStreamWaitEvent(solo->lst[l], solo->lfork, 0));
     }
     for (size_t gi = 0; gi < level.size(); gi++) {
       const Group &g = level[gi];
@@ -947,9 +947,9 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
   const int want = int(std::min<int64_t>(kLevelLanes, tuning("batch_level_lanes")));
   if (ok && widest > 1 && want > 1) {
     const int n = int(std::min<size_t>(size_t(want), widest));
+    ok = hipEventCreateWithFlags(&hb->solo.lfork, hipEventDisableTiming) == hipSuccess;
     for (int l = 0; ok && l < n; l++)
       ok = hipStreamCreateWithFlags(&hb->solo.lst[l], hipStreamNonBlocking) == hipSuccess &&
-           hipEventCreateWithFlags(&hb->solo.lfork[l], hipEventDisableTiming) == hipSuccess &&
            hipEventCreateWithFlags(&hb->solo.ljoin[l], hipEventDisableTiming) == hipSuccess;
     if (ok) hb->solo.level_lanes = n;
   }
@@ -1053,10 +1053,10 @@ void hip_marshal_batch_destroy(hip_marshal_batch *hb) {
   if (hb->solo.fork) hipEventDestroy(hb->solo.fork);
   for (int l = 0; l < kLevelLanes; l++) {
     if (hb->solo.lst[l]) hipStreamSynchronize(hb->solo.lst[l]);
-    if (hb->solo.lfork[l]) hipEventDestroy(hb->solo.lfork[l]);
     if (hb->solo.ljoin[l]) hipEventDestroy(hb->solo.ljoin[l]);
     if (hb->solo.lst[l]) hipStreamDestroy(hb->solo.lst[l]);
   }
+  if (hb->solo.lfork) hipEventDestroy(hb->solo.lfork);
   delete hb;
 }
 
