@@ -94,6 +94,7 @@ Knob g_knobs[] = {
     {"pt010_wave", {1}, 0},      // 1: the 010 cycle-following form owns one tile per wave
     {"pt010_lead_order", {64}, 0},  // its leader-only form when every cycle is this short
     {"pt010_lead_blocks_per_cu", {256}},
+    {"pt010_lead_cached", {1}, 0},  // its leaders copied from the cached list of the a x B grid
     {"pt100_vw_max", {8}},
     {"pt100_blocks_per_cu", {32}},
     {"lds0100_min_inst", {32}},
@@ -222,6 +223,8 @@ std::map<std::tuple<int, uint64_t, uint64_t>, LeaderEntry> g_lc;
 size_t g_lc_bytes = 0;
 constexpr size_t kLeaderCacheBytes = size_t(256) << 20;
 
+}  // namespace
+
 const uint32_t *cached_leaders(uint64_t R, uint64_t C, const PlPlan &pc, hipStream_t st) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
@@ -255,6 +258,8 @@ const uint32_t *cached_leaders(uint64_t R, uint64_t C, const PlPlan &pc, hipStre
   g_lc_bytes += bytes;
   return e.list;
 }
+
+namespace {
 
 struct Lease {
   Pool &p;

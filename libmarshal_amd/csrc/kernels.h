@@ -102,6 +102,10 @@ struct PlPlan {
 };
 // False unless R, C >= 2, R*C < 2^32 and the order is at most max_order.
 bool plan_100_leaders(uint64_t R, uint64_t C, uint32_t max_order, PlPlan *out);
+// The device list of cycle leaders of the R x C grid's map (pc from plan_100_leaders), kept
+// per (device, R, C) and ordered after its search on `st`; null when it cannot be had now
+// (allocation failed, cache full, or first use under stream capture).
+const uint32_t *cached_leaders(uint64_t R, uint64_t C, const PlPlan &pc, hipStream_t st);
 // scratch: cycles + 1 32-bit words (the leader list and its counter).
 hipError_t launch_100_leaders(unsigned char *base, uint32_t *scratch, uint64_t inst,
                               uint64_t R, uint64_t C, uint64_t S, const Pt100Plan &pl,

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(256) void k_010_pt_wave(Pt010Params p, uint32_t sh_
 // touch the same element.
 template <int ES, int W>
 __global__ __launch_bounds__(256) void k_010_pt_lead(Pt010Params p, uint32_t order,
-                                                      uint32_t max_leaders, FastDiv32 div_nl) {
+                                                      uint32_t max_leaders, FastDiv32 div_nl,
+                                                      const uint32_t *cached) {
   using E = typename Elem<ES>::T;
   extern __shared__ __align__(16) uint32_t lds[];
   uint32_t *count = lds;      // [0]: leaders found
@@ -108,9 +109,12 @@ __global__ __launch_bounds__(256) void k_010_pt_lead(Pt010Params p, uint32_t ord
     const uint32_t q = p.div_B.div(s);
     return (s - q * B) * a + q;
   };
-  if (tid == 0) count[0] = 0;
+  if (tid == 0) count[0] = cached ? max_leaders : 0;
   __syncthreads();
-  for (uint32_t s0 = 0; s0 < last; s0 += 256) {  // uniform over the workgroup
+  // The tile map is the 100 map of the a x B grid: when that grid's leader list is cached,
+  // it is copied in instead of searched for.
+  for (uint32_t k = tid; cached && k < max_leaders; k += 256) list[k] = cached[k];
+  for (uint32_t s0 = 0; !cached && s0 < last; s0 += 256) {  // uniform over the workgroup
     const uint32_t s = s0 + tid;
     bool lead = false;
     if (s < last) {
@@ -227,7 +231,7 @@ hipError_t launch_ptw_t(const Pt010Params &p, uint32_t stride, uint32_t blocks, 
 
 template <int ES>
 hipError_t launch_ptl_t(const Pt010Params &p, uint32_t order, uint32_t max_leaders,
-                        uint32_t blocks, hipStream_t st) {
+                        uint32_t blocks, const uint32_t *cached, hipStream_t st) {
   constexpr int W = ES >= 16 ? 4 : 8;
   auto k = k_010_pt_lead<ES, W>;
   const size_t lds = (size_t(max_leaders) + 4) * 4;
@@ -237,7 +241,7 @@ hipError_t launch_ptl_t(const Pt010Params &p, uint32_t order, uint32_t max_leade
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, st, p, order, max_leaders,
-                     FastDiv32(max_leaders));
+                     FastDiv32(max_leaders), cached);
   return hipGetLastError();
 }
 
@@ -403,12 +407,13 @@ hipError_t launch_010_pttwac(unsigned char *base, size_t es, uint64_t A, uint64_
     const uint64_t nw = uint64_t(lblocks) * 4;
     if ((A + nw - 1) / nw * pc.cycles >= (uint64_t(1) << 32)) return hipErrorInvalidValue;
     const uint32_t ml = uint32_t(pc.cycles);
+    const uint32_t *cached = tuning("pt010_lead_cached") ? cached_leaders(a, B, pc, st) : nullptr;
     switch (es) {
-      case 1: return launch_ptl_t<1>(p, pc.order, ml, lblocks, st);
-      case 2: return launch_ptl_t<2>(p, pc.order, ml, lblocks, st);
-      case 4: return launch_ptl_t<4>(p, pc.order, ml, lblocks, st);
-      case 8: return launch_ptl_t<8>(p, pc.order, ml, lblocks, st);
-      case 16: return launch_ptl_t<16>(p, pc.order, ml, lblocks, st);
+      case 1: return launch_ptl_t<1>(p, pc.order, ml, lblocks, cached, st);
+      case 2: return launch_ptl_t<2>(p, pc.order, ml, lblocks, cached, st);
+      case 4: return launch_ptl_t<4>(p, pc.order, ml, lblocks, cached, st);
+      case 8: return launch_ptl_t<8>(p, pc.order, ml, lblocks, cached, st);
+      case 16: return launch_ptl_t<16>(p, pc.order, ml, lblocks, cached, st);
     }
     return hipErrorInvalidValue;
   }
