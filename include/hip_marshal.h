@@ -267,6 +267,14 @@ void hip_marshal_set_lds_budget(size_t bytes);
 size_t hip_marshal_get_lds_budget(void);
 /* 1 = time calls with events and synchronize (default), 0 = asynchronous. */
 void hip_marshal_set_profiling(int on);
+/* Cluster-staged steps (route tags 0100_CL / 0100_CLF) hold an instance in a few
+ * workgroups that wait for each other at a barrier. The wait is bounded ("cl_spin"): a
+ * cluster whose workgroups cannot all become resident in time (other work holding the CUs)
+ * gives up and leaves its instance unconverted. Calls with profiling on (they synchronize)
+ * then return true with a message; asynchronous callers, batches and servers ask here.
+ * Waits for the current device, returns the clusters that gave up on it since the last
+ * check (0 normally). No counterpart in the reference. */
+uint64_t hip_marshal_cluster_aborts(void);
 /* Launch-shape knobs: "bs_upt" (4, 8, 16 units per lane per round of the LDS-staged 010),
  * "bs_blocks_per_cu", "pt010_blocks_per_cu", "pt100_blocks_per_cu", "rc_row_blocks_per_cu",
  * "rc_col_blocks_per_cu" (grid caps), "pt100_vw_max" (lanes per cycle-following walker),

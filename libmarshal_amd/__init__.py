@@ -93,6 +93,7 @@ def lib():
         "hip_marshal_set_lds_budget": (None, [sz]),
         "hip_marshal_get_lds_budget": (sz, []),
         "hip_marshal_set_profiling": (None, [i32]),
+        "hip_marshal_cluster_aborts": (ctypes.c_uint64, []),
         "hip_marshal_last_route": (ctypes.c_char_p, []),
         "hip_marshal_reserve": (b, [sz]),
         "hip_marshal_flag_bytes_0100": (sz, [i64, i64, i64, i64, sz]),
@@ -148,7 +149,8 @@ EXPORTED = [
     "hip_transpose_0100_es", "hip_transpose_0100_noalloc_es", "hip_transpose_es", "hip_transpose_full_es", "hip_aos_asta_es",
     "hip_asta_aos_es", "hip_soa_asta_es", "hip_asta_soa_es", "hip_padding_es",
     "hip_unpadding_es", "hip_marshal_set_lds_budget", "hip_marshal_get_lds_budget",
-    "hip_marshal_set_profiling", "hip_marshal_last_route", "hip_marshal_reserve",
+    "hip_marshal_set_profiling", "hip_marshal_cluster_aborts", "hip_marshal_last_route",
+    "hip_marshal_reserve",
     "hip_marshal_flag_bytes_0100", "hip_marshal_cycle_count", "hip_marshal_version",
     "hip_marshal_set_tuning",
     "hip_marshal_get_tuning", "hip_marshal_plan_transpose", "hip_marshal_autotune",
@@ -309,6 +311,12 @@ def set_lds_budget(nbytes):
 
 def lds_budget():
     return lib().hip_marshal_get_lds_budget()
+
+
+def cluster_aborts():
+    """Waits for the device; cluster-staged steps that gave up their barrier wait since the
+    last check (their instances left unconverted; 0 normally)."""
+    return int(lib().hip_marshal_cluster_aborts())
 
 
 def set_profiling(on):

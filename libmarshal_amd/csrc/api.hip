@@ -326,6 +326,13 @@ struct Timer {
     if (!on) return err;
     bool ok = !err && hipEventRecord(e1, st) == hipSuccess &&
               hipEventSynchronize(e1) == hipSuccess;
+    // The call's work has finished: a cluster-staged step that gave up its bounded barrier
+    // wait (since the last check) left its instances unconverted, which is an error.
+    if (ok && cluster_gave_up_new() != 0) {
+      fprintf(stderr, "[libmarshal] %s: a cluster-staged step gave up its barrier wait (its "
+                      "workgroups could not all be resident); instances left unconverted\n", id);
+      err = true;
+    }
     float ms = 0.f;
     if (ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
       uint64_t ns = uint64_t(double(ms) * 1e6 + 0.5);
@@ -1748,6 +1755,11 @@ bool hip_marshal_set_tuning(const char *key, int64_t value) {
 int64_t hip_marshal_get_tuning(const char *key) { return tuning(key); }
 size_t hip_marshal_get_lds_budget(void) { return lds_budget(); }
 void hip_marshal_set_profiling(int on) { g_profiling.store(on ? 1 : 0); }
+
+uint64_t hip_marshal_cluster_aborts(void) {
+  if (hipDeviceSynchronize() != hipSuccess) return 0;
+  return cluster_gave_up_new();
+}
 const char *hip_marshal_last_route(void) { return t_route.c_str(); }
 
 bool hip_marshal_reserve(size_t bytes) {

@@ -362,6 +362,10 @@ bool plan_100_cluster(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64_
 bool plan_cluster_fused(uint64_t inst, uint64_t R, uint64_t a, uint64_t b, size_t es,
                         uint64_t base_addr, uint64_t max_wgs, ClPlan *out);
 uint64_t cluster_flag_words(uint64_t inst);
+// Clusters of the cluster-staged form that gave up their bounded barrier wait on the current
+// device since the last call (their instances were left unconverted); reads a device word,
+// so the work in question must have finished. 0 when no cluster launch ran on the device.
+uint64_t cluster_gave_up_new();
 hipError_t launch_100_cluster(unsigned char *base, uint32_t *flags, uint64_t inst, uint64_t R,
                               uint64_t C, uint64_t S, const ClPlan &pl, hipStream_t st);
 void free_cluster_flags();

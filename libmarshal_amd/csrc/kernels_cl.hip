@@ -48,6 +48,7 @@ struct ClParams {
   unsigned char *base;
   uint32_t *flags;      // [0]: dynamic workgroup ids, [1]: workgroups done, [2 + c]: arrivals
                         // at cluster c's barrier, [2 + inst + c]: members past it
+  uint32_t *gave_up;    // the device's count of clusters that gave up their wait (or null)
   uint64_t inst_bytes;  // R * C * S
   uint64_t wgs;         // instances * G (the grid)
   uint64_t inst;
@@ -186,8 +187,12 @@ __global__ __launch_bounds__(NT) void k_100_cluster(ClParams p) {
         break;
       } else if (n >= p.spin) {
         if (__hip_atomic_compare_exchange_strong(w, &v, v | kClAbort, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
           go = 0;
+          // counted once per cluster (by the member that set the bit), for the host to report
+          if (p.gave_up)
+            __hip_atomic_fetch_add(p.gave_up, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       } else {
         n++;
         __builtin_amdgcn_s_sleep(2);
@@ -212,6 +217,30 @@ __global__ __launch_bounds__(NT) void k_100_cluster(ClParams p) {
   }
 }
 
+
+// Clusters that gave up their barrier wait, counted per device since the library was loaded
+// (the instances they held were left unconverted). Its address is looked up once per device,
+// with that device current; cluster_gave_up_new reports the growth since its last call.
+static __device__ uint32_t cl_gave_up;
+std::mutex g_gu_mu;
+uint32_t *g_gu_addr[64] = {};
+uint32_t g_gu_seen[64] = {};
+bool g_gu_used[64] = {};
+
+uint32_t *gave_up_here(int dev) {
+  std::lock_guard<std::mutex> lk(g_gu_mu);
+  if (dev < 0 || dev >= 64) return nullptr;
+  if (!g_gu_addr[dev]) {
+    void *a = nullptr;
+    if (hipGetSymbolAddress(&a, HIP_SYMBOL(cl_gave_up)) != hipSuccess || !a) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_gu_addr[dev] = static_cast<uint32_t *>(a);
+  }
+  g_gu_used[dev] = true;
+  return g_gu_addr[dev];
+}
 
 size_t cl_lds_bytes() {
   // LDS per workgroup: the device's limit, less room for the two static words, rounded down
@@ -281,6 +310,23 @@ bool plan_cluster_fused(uint64_t inst, uint64_t R, uint64_t a, uint64_t b, size_
 }
 
 uint64_t cluster_flag_words(uint64_t inst) { return 2 + 2 * inst; }
+
+uint64_t cluster_gave_up_new() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  uint32_t *a = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_gu_mu);
+    if (!g_gu_used[dev]) return 0;  // no cluster launch on this device yet
+    a = g_gu_addr[dev];
+  }
+  uint32_t v = 0;
+  if (!a || hipMemcpy(&v, a, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lk(g_gu_mu);
+  const uint32_t d = v - g_gu_seen[dev];
+  g_gu_seen[dev] = v;
+  return d;
+}
 
 // Barrier words kept per (device, stream): zero whenever no launch of this file runs on the
 // stream, since every launch leaves them zero. A larger need replaces the buffer (the old one
@@ -358,6 +404,8 @@ hipError_t launch_100_cluster(unsigned char *base, uint32_t *flags, uint64_t ins
   ClParams p{};
   p.base = base;
   p.flags = kept ? kept : flags;
+  int dev = 0;
+  p.gave_up = hipGetDevice(&dev) == hipSuccess ? gave_up_here(dev) : nullptr;
   p.inst = inst;
   p.inst_bytes = R * C * S;
   p.wgs = inst * pl.G;

@@ -190,3 +190,54 @@ def test_three_stage_route_fusion_edges(hip, orc, case):
     assert np.array_equal(got[off * es:(off + M * N) * es], want.view(np.uint8)), case
     assert not got[:off * es].any() and not got[(off + M * N) * es:].any()
     say("fusion edge", case, hip.last_route())
+
+
+def test_cluster_that_gives_up_is_reported(hip, orc, anywhere):
+    """A cluster whose workgroups cannot all arrive within the barrier's bounded wait gives up
+    and stores nothing, so its instance stays unconverted; that must be reported, never
+    silent. Forced from the host with a one-poll bound ("cl_spin" = 1: every wait still
+    ends): a synchronizing call returns an error exactly when an instance was left untouched,
+    and asynchronous calls report the clusters that gave up through cluster_aborts (one
+    cluster per instance here, so the count equals the untouched instances). Every instance
+    is either converted or untouched."""
+    import torch
+    inst, R, C, b, es = 100, 100, 72, 18, 4  # 518 KB instances, four workgroups each
+    n = inst * R * C * b
+    x = orc.tagged(n, es, seed=31)
+    want = orc.op_0100(x, inst, R, C, b, es=es).view(np.uint8).reshape(inst, -1)
+    orig = x.view(np.uint8).reshape(inst, -1)
+    assert hip.cluster_aborts() == 0
+
+    def outcome(d):
+        got = d.cpu().numpy().reshape(inst, -1)
+        done = [np.array_equal(got[k], want[k]) for k in range(inst)]
+        kept = [np.array_equal(got[k], orig[k]) for k in range(inst)]
+        assert all(a or k for a, k in zip(done, kept)), "an instance was partly written"
+        return sum(1 for a in done if not a)
+
+    keep = hip.get_tuning("cl_spin")
+    hip.set_tuning("cl_spin", 1)
+    try:
+        d = torch.from_numpy(x.copy()).cuda()
+        et = hip.Elapsed()
+        r = hip.transpose_0100(d, inst, R, C, b, elem_bytes=es, elapsed=et)  # synchronizing
+        assert hip.last_route() == "0100_CL-"
+        untouched = outcome(d)
+        say("synchronizing call:", r, "untouched", untouched)
+        assert r is (untouched > 0)
+        d = torch.from_numpy(x.copy()).cuda()
+        hip.set_profiling(False)
+        try:
+            assert hip.transpose_0100(d, inst, R, C, b, elem_bytes=es) is False
+            aborted = hip.cluster_aborts()
+        finally:
+            hip.set_profiling(True)
+        untouched = outcome(d)
+        say("asynchronous call: clusters that gave up", aborted, "untouched", untouched)
+        assert aborted == untouched
+    finally:
+        hip.set_tuning("cl_spin", keep)
+    # with the default bound the same step converts every instance again
+    d = torch.from_numpy(x.copy()).cuda()
+    assert hip.transpose_0100(d, inst, R, C, b, elem_bytes=es) is False
+    assert outcome(d) == 0 and hip.cluster_aborts() == 0
