@@ -14,8 +14,8 @@
  * (1, 2, 4, 8 or 16; elements are opaque bit patterns) and 64-bit dimensions.
  *
  * Timing: when profiling is on (the default, like LIBMARSHAL_OCL_PROFILE in the reference's
- * src/CMakeLists.txt:2,8-10) a non-NULL `elapsed_ns` receives the wall time of the whole call
- * measured with HIP events (the call then synchronizes its stream), and a NULL pointer makes
+ * src/CMakeLists.txt:2,8-10) every call synchronizes its stream: a non-NULL `elapsed_ns`
+ * receives the wall time of the whole call measured with HIP events, and a NULL pointer makes
  * the call print "[Profile] <name> <ns>ns; <GB/s> GB/s" to stderr (include/cl_profile.h:43-56).
  * With profiling off (hip_marshal_set_profiling(0)) calls are fully stream-ordered and never
  * synchronize, and `elapsed_ns` is left untouched.
