@@ -180,13 +180,16 @@ bool hip_transpose_0100_noalloc_es(hipStream_t stream, void *src, size_t elem_by
  * kernel forms), groups the stages of all matrices by level and kernel form, and uploads the
  * work lists; it returns NULL on an invalid descriptor. hip_marshal_batch_run converts every
  * buffer in place: with use_graph the sequence (bit clearing + one persistent kernel per
- * group and level) is captured into a hipGraph on the first run and replayed afterwards.
- * Transposes whose best route is the stripe route (rows and columns beyond LDS) have no
- * grouped form: they run with their plain stage lists, one after another, on a second
- * stream forked from and joined back to the run's stream (captured into the same graph).
+ * group and level) is built into a hipGraph on the first run, node by node (the library
+ * captures no stream, so other threads may use the device meanwhile), and launched
+ * afterwards; without it the same kernels are launched one by one. Matrices whose best route
+ * has no grouped form (the stripe route, and from "batch_solo_kb" the larger in-place and
+ * through-scratch forms) run their plain stage lists on side streams ("batch_solo_lanes",
+ * balanced by bytes) forked from the run's stream as the graph starts and joined back to it.
  * Each buffer ends bit-identical to what the plain entry point gives for it alone. Buffers
- * must not overlap; a batch's own visited-bit space is private to it, so different batches
- * may run concurrently on different streams, but one batch must not run twice at once.
+ * must not overlap; a batch's own visited-bit space and scratch are private to it, so
+ * different batches may run concurrently on different streams, from any threads, but one
+ * batch must not run twice at once.
  * ------------------------------------------------------------------------------------- */
 enum {
   HIP_MARSHAL_OP_TRANSPOSE = 0, /* rows x cols -> cols x rows                     */
@@ -208,7 +211,8 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
                                             int use_graph);
 bool hip_marshal_batch_run(hip_marshal_batch *batch, hipStream_t stream, uint64_t *elapsed_ns);
 /* "levels", "stages", "kernels", "flag_bytes", "bytes" (algorithmic bytes of one run),
- * "solo" (matrices run with their plain stage lists on the second stream). */
+ * "solo" (matrices run with their plain stage lists on the side streams), "arena_matrices",
+ * "arena_rounds", "arena_bytes" (the through-scratch rounds). */
 int64_t hip_marshal_batch_info(const hip_marshal_batch *batch, const char *key);
 const char *hip_marshal_batch_route(const hip_marshal_batch *batch, int64_t index);
 void hip_marshal_batch_destroy(hip_marshal_batch *batch);
@@ -273,7 +277,9 @@ void hip_marshal_set_profiling(int on);
  * gives up and leaves its instance unconverted. Calls with profiling on (they synchronize)
  * then return true with a message; asynchronous callers, batches and servers ask here.
  * Waits for the current device, returns the clusters that gave up on it since the last
- * check (0 normally). No counterpart in the reference. */
+ * check (0 normally). The count is per device, not per call: with several threads on one
+ * device, a give-up is reported by whichever check (this one or a synchronizing call's)
+ * comes next. No counterpart in the reference. */
 uint64_t hip_marshal_cluster_aborts(void);
 /* Launch-shape knobs: "bs_upt" (4, 8, 16 units per lane per round of the LDS-staged 010),
  * "bs_blocks_per_cu", "pt010_blocks_per_cu", "pt100_blocks_per_cu", "rc_row_blocks_per_cu",

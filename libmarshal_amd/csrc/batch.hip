@@ -6,10 +6,10 @@
 // executed by ONE persistent kernel over a work list: each workgroup (or wave) claims work
 // units (chunks, tiles, instances, walker sets) of any matrix in the group, looks up its
 // matrix from a prefix sum of units (find_item), and runs the same device body as the
-// single-matrix kernel. Levels are ordered; groups inside a level are independent and run on
-// branch streams side by side. The whole sequence - clearing every matrix's visited bits,
-// then the group kernels level by level - is captured once into a hipGraph, so a run is one
-// graph launch.
+// single-matrix kernel. Levels are ordered; groups inside a level are independent and run
+// side by side (on branch streams, or as parallel chains of graph nodes). The whole sequence -
+// clearing every matrix's visited bits, then the group kernels level by level - is built once
+// into a hipGraph, node by node (no stream capture), so a run is one graph launch.
 //
 // Three kinds of matrix share a run:
 // - grouped in-place stages (the levels above);
@@ -19,7 +19,8 @@
 //   of at most "batch_arena_mb" follow one another on the run's stream;
 // - side-stream matrices (stripe route, and from "batch_solo_kb" the square, cycle-leader,
 //   cluster, segmented and larger through-scratch forms) run their plain stage lists on
-//   side streams forked from and joined to the run's stream inside the same graph.
+//   side streams forked from the run's stream when the graph is launched and joined to it
+//   after (the run's stream waits for both).
 //
 // No counterpart in the reference, which launches one kernel per stage per matrix and waits
 // after each (src/cl_marshal.cc:561-776, include/cl_profile.h:23-24).
@@ -327,22 +328,38 @@ const void *lds_func(int ub, int upl) {
 
 }  // namespace
 
+// The solo lists on the side streams, ordered after the work already on `st`; each lane's
+// join event is recorded after its last list (wait_solo makes `st` wait for them).
+static bool fork_solo(const BatchPlan &b, hipStream_t st, const SoloExec *solo) {
+  if (b.solo.empty()) return false;
+  if (!solo || !solo->fork) return true;
+  LM_CHECK_HIP(hipEventRecord(solo->fork, st));
+  for (int l = 0; l < b.solo_lanes; l++) {
+    if (!solo->st[l] || !solo->join[l]) return true;
+    LM_CHECK_HIP(hipStreamWaitEvent(solo->st[l], solo->fork, 0));
+  }
+  return false;
+}
+
+static bool run_solo(const BatchPlan &b, const SoloExec *solo) {
+  if (b.solo.empty()) return false;
+  for (size_t i = 0; i < b.solo.size(); i++) {
+    const int l = b.solo_lane[i];
+    if (run_stages_on(b.solo[i], solo->st[l], solo->scr[l])) return true;
+  }
+  for (int l = 0; l < b.solo_lanes; l++) LM_CHECK_HIP(hipEventRecord(solo->join[l], solo->st[l]));
+  return false;
+}
+
+static bool wait_solo(const BatchPlan &b, hipStream_t st, const SoloExec *solo) {
+  if (b.solo.empty()) return false;
+  for (int l = 0; l < b.solo_lanes; l++) LM_CHECK_HIP(hipStreamWaitEvent(st, solo->join[l], 0));
+  return false;
+}
+
 bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_t st,
                    const SoloExec *solo) {
-  const bool fork = !b.solo.empty();
-  if (fork) {
-    if (!solo || !solo->fork) return true;
-    LM_CHECK_HIP(hipEventRecord(solo->fork, st));
-    for (int l = 0; l < b.solo_lanes; l++) {
-      if (!solo->st[l] || !solo->join[l]) return true;
-      LM_CHECK_HIP(hipStreamWaitEvent(solo->st[l], solo->fork, 0));
-    }
-    for (size_t i = 0; i < b.solo.size(); i++) {
-      const int l = b.solo_lane[i];
-      if (run_stages_on(b.solo[i], solo->st[l], solo->scr[l])) return true;
-    }
-    for (int l = 0; l < b.solo_lanes; l++) LM_CHECK_HIP(hipEventRecord(solo->join[l], solo->st[l]));
-  }
+  if (fork_solo(b, st, solo) || run_solo(b, solo)) return true;
   if (b.flag_words) LM_CHECK_HIP(launch_fill_u32(flags, b.flag_words, 0, st));
   unsigned char *base = static_cast<unsigned char *>(params);
   const int lanes = solo ? solo->level_lanes : 0;
@@ -372,8 +389,59 @@ bool enqueue_batch(const BatchPlan &b, void *params, uint32_t *flags, hipStream_
       }
     }
   }
-  if (fork)
-    for (int l = 0; l < b.solo_lanes; l++) LM_CHECK_HIP(hipStreamWaitEvent(st, solo->join[l], 0));
+  return wait_solo(b, st, solo);
+}
+
+// The grouped part of a run (the flag clearing and every group, level by level) as a graph
+// built node by node, with the dependencies the branch streams of enqueue_batch give: with
+// `lanes` > 1 the groups of a level are dealt over that many chains, each chain's first group
+// waits for the whole previous level, and a chain runs its groups one after another. No stream
+// is captured, so building the graph neither restricts nor is disturbed by what other threads
+// do with the device meanwhile (allocations, synchronous copies, their own captures).
+static bool build_group_graph(const BatchPlan &b, void *params, uint32_t *flags, int lanes,
+                              hipGraph_t *out) {
+  hipGraph_t gr = nullptr;
+  LM_CHECK_HIP(hipGraphCreate(&gr, 0));
+  auto fail = [&](hipError_t e, const char *what) {
+    fprintf(stderr, "[libmarshal] batch graph: %s: %s\n", what, hipGetErrorString(e));
+    hipGraphDestroy(gr);
+    return true;
+  };
+  std::vector<hipGraphNode_t> prev;  // nodes the next level waits for
+  if (b.flag_words) {
+    hipGraphNode_t n = nullptr;
+    hipError_t e = add_fill_u32_node(gr, nullptr, 0, flags, b.flag_words, 0, &n);
+    if (e != hipSuccess) return fail(e, "flag clearing node");
+    prev.push_back(n);
+  }
+  unsigned char *base = static_cast<unsigned char *>(params);
+  for (auto &level : b.levels) {
+    if (level.empty()) continue;
+    const size_t used = size_t(std::max(1, std::min<int>(lanes, int(level.size()))));
+    std::vector<hipGraphNode_t> cur(level.size(), nullptr);
+    for (size_t gi = 0; gi < level.size(); gi++) {
+      const Group &g = level[gi];
+      const void *items = base + g.items_off;
+      const uint64_t *prefix = reinterpret_cast<const uint64_t *>(base + g.prefix_off);
+      uint32_t n = uint32_t(g.prefix.size() - 1);
+      uint64_t total = g.prefix.back();
+      void *args[] = {&items, &prefix, &n, &total};  // copied into the node when it is added
+      hipKernelNodeParams kp{};
+      kp.func = const_cast<void *>(g.func);
+      kp.gridDim = dim3(g.grid);
+      kp.blockDim = dim3(g.block);
+      kp.sharedMemBytes = uint32_t(g.lds);
+      kp.kernelParams = args;
+      kp.extra = nullptr;
+      const hipGraphNode_t *deps = gi >= used ? &cur[gi - used] : prev.data();
+      const size_t nd = gi >= used ? 1 : prev.size();
+      hipError_t e = hipGraphAddKernelNode(&cur[gi], gr, nd ? deps : nullptr, nd, &kp);
+      if (e != hipSuccess) return fail(e, "kernel node");
+    }
+    // the last group of every chain (used <= cur.size())
+    prev.assign(cur.end() - used, cur.end());
+  }
+  *out = gr;
   return false;
 }
 
@@ -930,15 +998,27 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
     hip_marshal_batch_destroy(hb);
     return nullptr;
   }
+  // every device call checked; the first failure is named in the message
+  const char *failed = nullptr;
+  hipError_t ferr = hipSuccess;
+  auto ok_ = [&](hipError_t e, const char *what) {
+    if (e != hipSuccess && !failed) {
+      failed = what;
+      ferr = e;
+    }
+    return e == hipSuccess;
+  };
   bool ok = true;
   if (!b.solo.empty()) {
-    ok = hipEventCreateWithFlags(&hb->solo.fork, hipEventDisableTiming) == hipSuccess;
+    ok = ok_(hipEventCreateWithFlags(&hb->solo.fork, hipEventDisableTiming), "hipEventCreate");
     for (int l = 0; ok && l < b.solo_lanes; l++) {
-      ok = hipStreamCreateWithFlags(&hb->solo.st[l], hipStreamNonBlocking) == hipSuccess &&
-           hipEventCreateWithFlags(&hb->solo.join[l], hipEventDisableTiming) == hipSuccess;
+      ok = ok_(hipStreamCreateWithFlags(&hb->solo.st[l], hipStreamNonBlocking),
+               "hipStreamCreate") &&
+           ok_(hipEventCreateWithFlags(&hb->solo.join[l], hipEventDisableTiming),
+               "hipEventCreate");
       if (ok && b.solo_scratch[l])
-        ok = hipMalloc(reinterpret_cast<void **>(&hb->solo.scr[l]), b.solo_scratch[l]) ==
-             hipSuccess;
+        ok = ok_(hipMalloc(reinterpret_cast<void **>(&hb->solo.scr[l]), b.solo_scratch[l]),
+                 "hipMalloc (side-stream scratch)");
     }
   }
   // branch streams for levels of more than one group
@@ -947,23 +1027,28 @@ hip_marshal_batch *hip_marshal_batch_create(const hip_marshal_desc *descs, int64
   const int want = int(std::min<int64_t>(kLevelLanes, tuning("batch_level_lanes")));
   if (ok && widest > 1 && want > 1) {
     const int n = int(std::min<size_t>(size_t(want), widest));
-    ok = hipEventCreateWithFlags(&hb->solo.lfork, hipEventDisableTiming) == hipSuccess;
+    ok = ok_(hipEventCreateWithFlags(&hb->solo.lfork, hipEventDisableTiming), "hipEventCreate");
     for (int l = 0; ok && l < n; l++)
-      ok = hipStreamCreateWithFlags(&hb->solo.lst[l], hipStreamNonBlocking) == hipSuccess &&
-           hipEventCreateWithFlags(&hb->solo.ljoin[l], hipEventDisableTiming) == hipSuccess;
+      ok = ok_(hipStreamCreateWithFlags(&hb->solo.lst[l], hipStreamNonBlocking),
+               "hipStreamCreate") &&
+           ok_(hipEventCreateWithFlags(&hb->solo.ljoin[l], hipEventDisableTiming),
+               "hipEventCreate");
     if (ok) hb->solo.level_lanes = n;
   }
-  if (ok && b.flag_words) ok = hipMalloc(&hb->flags, b.flag_words * 4) == hipSuccess;
+  if (ok && b.flag_words) ok = ok_(hipMalloc(&hb->flags, b.flag_words * 4), "hipMalloc (flags)");
   if (ok && b.arena_bytes)
-    ok = hipMalloc(reinterpret_cast<void **>(&hb->arena), b.arena_bytes) == hipSuccess;
-  if (ok && b.param_bytes) ok = hipMalloc(&hb->params, b.param_bytes) == hipSuccess;
+    ok = ok_(hipMalloc(reinterpret_cast<void **>(&hb->arena), b.arena_bytes),
+             "hipMalloc (arena)");
+  if (ok && b.param_bytes) ok = ok_(hipMalloc(&hb->params, b.param_bytes), "hipMalloc (params)");
   if (ok && b.param_bytes) {
     std::vector<unsigned char> host;
     build_image(b, hb->flags, host, hb->arena);
-    ok = hipMemcpy(hb->params, host.data(), b.param_bytes, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok_(hipMemcpy(hb->params, host.data(), b.param_bytes, hipMemcpyHostToDevice),
+             "hipMemcpy (work lists)");
   }
   if (!ok) {
-    fprintf(stderr, "[libmarshal] %s: device allocation or upload failed\n", who);
+    fprintf(stderr, "[libmarshal] %s: %s failed: %s\n", who, failed ? failed : "setup",
+            hipGetErrorString(ferr));
     hip_marshal_batch_destroy(hb);
     return nullptr;
   }
@@ -981,23 +1066,16 @@ bool hip_marshal_batch_run(hip_marshal_batch *hb, hipStream_t st, uint64_t *elap
   }
   if (hb->use_graph) {
     if (!hb->exec) {
-      // Captured once, on a private stream (the caller's may be the null stream).
-      hipStream_t cs = nullptr;
-      LM_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-      hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-      bool err = e != hipSuccess || enqueue_batch(b, hb->params, hb->flags, cs, &hb->solo);
       hipGraph_t g = nullptr;
-      if (e == hipSuccess) e = hipStreamEndCapture(cs, &g);
-      hipStreamDestroy(cs);
-      if (err || e != hipSuccess) {
-        if (g) hipGraphDestroy(g);
-        fprintf(stderr, "[libmarshal] hip_marshal_batch_run: capture failed\n");
-        return true;
-      }
+      if (build_group_graph(b, hb->params, hb->flags, hb->solo.level_lanes, &g)) return true;
       hb->graph = g;
       LM_CHECK_HIP(hipGraphInstantiate(&hb->exec, hb->graph, nullptr, nullptr, 0));
     }
+    // the solo lists start with the graph, on the side streams, and the run's stream waits
+    // for both
+    if (fork_solo(b, st, &hb->solo)) return true;
     LM_CHECK_HIP(hipGraphLaunch(hb->exec, st));
+    if (run_solo(b, &hb->solo) || wait_solo(b, st, &hb->solo)) return true;
   } else if (enqueue_batch(b, hb->params, hb->flags, st, &hb->solo)) {
     return true;
   }

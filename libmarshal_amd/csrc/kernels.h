@@ -154,6 +154,9 @@ hipError_t launch_rc(unsigned char *base, uint64_t M, uint64_t N, const RcPlan &
 
 // ---- utilities --------------------------------------------------------------------------
 hipError_t launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st);
+// The same fill as a kernel node of `graph` after `deps` (for graphs built node by node).
+hipError_t add_fill_u32_node(hipGraph_t graph, const hipGraphNode_t *deps, size_t ndeps,
+                             uint32_t *p, uint64_t words, uint32_t value, hipGraphNode_t *node);
 // Byte copy (16-byte vectors when both ends and the size allow) and a pitched 2-D copy.
 hipError_t launch_copy(unsigned char *dst, const unsigned char *src, uint64_t bytes,
                        hipStream_t st);

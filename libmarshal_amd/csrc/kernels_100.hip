@@ -65,12 +65,29 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t words, uint32_t value) {
   for (; i < words; i += stride) p[i] = value;
 }
 
+static uint32_t fill_blocks(uint64_t words) {
+  return uint32_t(std::min<uint64_t>((words + 255) / 256, uint64_t(device_info().cus) * 16));
+}
+
 hipError_t launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st) {
   if (words == 0) return hipSuccess;
-  uint64_t blocks = std::min<uint64_t>((words + 255) / 256, uint64_t(device_info().cus) * 16);
   KernelSpan span(st, "k_fill_u32", 0.0);
-  hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t(blocks)), dim3(256), 0, st, p, words, value);
+  hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks(words)), dim3(256), 0, st, p, words, value);
   return hipGetLastError();
+}
+
+hipError_t add_fill_u32_node(hipGraph_t graph, const hipGraphNode_t *deps, size_t ndeps,
+                             uint32_t *p, uint64_t words, uint32_t value, hipGraphNode_t *node) {
+  if (words == 0) return hipErrorInvalidValue;
+  void *args[] = {&p, &words, &value};
+  hipKernelNodeParams kp{};
+  kp.func = reinterpret_cast<void *>(k_fill_u32);
+  kp.gridDim = dim3(fill_blocks(words));
+  kp.blockDim = dim3(256);
+  kp.sharedMemBytes = 0;
+  kp.kernelParams = args;
+  kp.extra = nullptr;
+  return hipGraphAddKernelNode(node, graph, deps, ndeps, &kp);
 }
 
 static uint64_t next_pow2(uint64_t v) {

@@ -150,3 +150,68 @@ def test_threads_sharing_one_stream(hip, orc):
     for k in range(3):
         for (name, _, _, want), b in zip(per[k], bufs[k]):
             assert np.array_equal(b.cpu().numpy(), want), (k, name)
+
+
+def test_batches_built_and_run_beside_plain_calls(hip, orc):
+    """Three threads build batches (planning, allocation, graph capture on their own streams)
+    and replay them while a fourth makes plain calls that grow the shared pool and the
+    cycle-leader cache: a capture is local to its thread, so neither side may disturb the
+    other. Every buffer of every replay must match the oracle."""
+    import torch
+    from tests.test_gpu_batch import expected, make_mix
+    mixes = [make_mix(orc, 60, 16, 600, seed=300 + k) for k in range(3)]
+    wants = [[expected(orc, it) for it in mix] for mix in mixes]
+    plain = work_items(orc, seed=9000)
+    failures = []
+    started = threading.Barrier(4)
+
+    def batch_worker(k):
+        try:
+            stream = torch.cuda.Stream()
+            with torch.cuda.stream(stream):
+                bufs = [torch.from_numpy(it[0].copy()).cuda() for it in mixes[k]]
+            stream.synchronize()
+            started.wait(timeout=120)
+            batch = hip.Batch([(b, op, es, r, c, t)
+                               for b, (_, op, es, r, c, t) in zip(bufs, mixes[k])])
+            for replay in range(2):
+                if replay:
+                    with torch.cuda.stream(stream):
+                        for b, it in zip(bufs, mixes[k]):
+                            b.copy_(torch.from_numpy(it[0]).cuda())
+                assert batch.run(stream=stream) is False, (k, replay)
+                stream.synchronize()
+                for i, (b, w) in enumerate(zip(bufs, wants[k])):
+                    if not np.array_equal(b.cpu().numpy(), w):
+                        failures.append(f"batch {k} replay {replay} item {i} "
+                                        f"{mixes[k][i][1:]} {batch.route(i)}")
+            batch.close()
+        except BaseException as e:
+            failures.append(f"batch thread {k}: {type(e).__name__}: {e}")
+
+    def plain_worker():
+        try:
+            stream = torch.cuda.Stream()
+            with torch.cuda.stream(stream):
+                bufs = [torch.from_numpy(x.copy()).cuda() for (_, _, x, _) in plain]
+            stream.synchronize()
+            started.wait(timeout=120)
+            for (name, run, x, want), b in zip(plain, bufs):
+                assert run(hip, b, stream, None) is False, name
+            stream.synchronize()
+            for (name, run, x, want), b in zip(plain, bufs):
+                if not np.array_equal(b.cpu().numpy(), want):
+                    failures.append(f"plain {name}")
+        except BaseException as e:
+            failures.append(f"plain thread: {type(e).__name__}: {e}")
+
+    ts = [threading.Thread(target=batch_worker, args=(k,)) for k in range(3)]
+    ts.append(threading.Thread(target=plain_worker))
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=150)
+    assert not any(t.is_alive() for t in ts), "a worker thread did not finish"
+    torch.cuda.synchronize()
+    assert not failures, failures[:10]
+    assert hip.cluster_aborts() == 0
