@@ -94,7 +94,8 @@ Knob g_knobs[] = {
     {"pt010_wave", {1}, 0},      // 1: the 010 cycle-following form owns one tile per wave
     {"pt010_lead_order", {64}, 0},  // its leader-only form when every cycle is this short
     {"pt010_lead_blocks_per_cu", {256}},
-    {"pt010_lead_cached", {1}, 0},  // its leaders copied from the cached list of the a x B grid
+    {"pt010_lead_cached", {2}, 0},  // its leaders: 0 searched per workgroup, 1 the cached list
+                                    // of the a x B grid, 2 that list ordered for few sectors
     {"pt100_vw_max", {8}},
     {"pt100_blocks_per_cu", {32}},
     {"lds0100_min_inst", {32}},
@@ -214,12 +215,14 @@ struct LeaderEntry {
   uint32_t *list = nullptr;
   hipEvent_t ready = nullptr;
   size_t bytes = 0;
+  std::vector<uint32_t> host;  // an ordered list's source of its upload (kept until finalize)
   // set by a lookup outside capture that finds the search finished: under capture no event may
   // be queried (that would invalidate the capture), so only such an entry is used there
   bool done = false;
 };
 std::mutex g_lc_mu;
-std::map<std::tuple<int, uint64_t, uint64_t>, LeaderEntry> g_lc;
+// key: (device, R, C, 0) for a searched list, (device, a, B, es) for an ordered 010 list
+std::map<std::tuple<int, uint64_t, uint64_t, uint64_t>, LeaderEntry> g_lc;
 size_t g_lc_bytes = 0;
 constexpr size_t kLeaderCacheBytes = size_t(256) << 20;
 
@@ -232,7 +235,7 @@ const uint32_t *cached_leaders(uint64_t R, uint64_t C, const PlPlan &pc, hipStre
   hipStreamIsCapturing(st, &cs);
   const bool capturing = cs != hipStreamCaptureStatusNone;
   std::lock_guard<std::mutex> lk(g_lc_mu);
-  const auto key = std::make_tuple(dev, R, C);
+  const auto key = std::make_tuple(dev, R, C, uint64_t(0));
   auto it = g_lc.find(key);
   if (it != g_lc.end()) {
     if (capturing) return it->second.done ? it->second.list : nullptr;
@@ -257,6 +260,50 @@ const uint32_t *cached_leaders(uint64_t R, uint64_t C, const PlPlan &pc, hipStre
   g_lc[key] = e;
   g_lc_bytes += bytes;
   return e.list;
+}
+
+const uint32_t *cached_ordered_leaders(uint64_t a, uint64_t B, size_t es, const PlPlan &pc,
+                                       hipStream_t st, uint32_t *entries) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || a > 0xffffffffu || B > 0xffffffffu) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipStreamIsCapturing(st, &cs);
+  const bool capturing = cs != hipStreamCaptureStatusNone;
+  std::lock_guard<std::mutex> lk(g_lc_mu);
+  const auto key = std::make_tuple(dev, a, B, uint64_t(es));
+  auto it = g_lc.find(key);
+  if (it != g_lc.end()) {
+    LeaderEntry &f = it->second;
+    if (!f.list) return nullptr;  // this shape could not be ordered
+    *entries = uint32_t(f.bytes / 4);
+    if (capturing) return f.done ? f.list : nullptr;
+    if (!f.done && hipEventQuery(f.ready) == hipSuccess) f.done = true;
+    if (!f.done && hipStreamWaitEvent(st, f.ready, 0) != hipSuccess) return nullptr;
+    return f.list;
+  }
+  if (capturing) return nullptr;
+  LeaderEntry e;
+  e.host = order_010_leaders(uint32_t(a), uint32_t(B), es, pc.cycles);
+  if (e.host.empty()) {
+    g_lc[key] = LeaderEntry{};  // remembered, so the ordering is not tried again
+    return nullptr;
+  }
+  e.bytes = e.host.size() * 4;
+  if (g_lc_bytes + e.bytes > kLeaderCacheBytes) return nullptr;
+  if (hipMalloc(reinterpret_cast<void **>(&e.list), e.bytes) != hipSuccess) return nullptr;
+  if (hipEventCreateWithFlags(&e.ready, hipEventDisableTiming) != hipSuccess ||
+      hipMemcpyAsync(e.list, e.host.data(), e.bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipEventRecord(e.ready, st) != hipSuccess) {
+    hipStreamSynchronize(st);
+    hipFree(e.list);
+    if (e.ready) hipEventDestroy(e.ready);
+    return nullptr;
+  }
+  *entries = uint32_t(e.host.size());
+  const uint32_t *list = e.list;
+  g_lc_bytes += e.bytes;
+  g_lc[key] = std::move(e);
+  return list;
 }
 
 namespace {
@@ -1801,9 +1848,11 @@ void hip_marshal_finalize(void) {
     hipGetDevice(&cur);
     for (auto &kv : g_lc) {
       hipSetDevice(std::get<0>(kv.first));
-      hipEventSynchronize(kv.second.ready);
-      hipEventDestroy(kv.second.ready);
-      hipFree(kv.second.list);
+      if (kv.second.ready) {
+        hipEventSynchronize(kv.second.ready);
+        hipEventDestroy(kv.second.ready);
+      }
+      if (kv.second.list) hipFree(kv.second.list);
     }
     g_lc.clear();
     g_lc_bytes = 0;

@@ -106,6 +106,18 @@ bool plan_100_leaders(uint64_t R, uint64_t C, uint32_t max_order, PlPlan *out);
 // per (device, R, C) and ordered after its search on `st`; null when it cannot be had now
 // (allocation failed, cache full, or first use under stream capture).
 const uint32_t *cached_leaders(uint64_t R, uint64_t C, const PlPlan &pc, hipStream_t st);
+// Padding entry of an ordered leader list: the lane that draws it has no cycle in that round.
+constexpr uint32_t kNoLeader = 0xffffffffu;
+// One start slot per cycle (of length >= 2) of the 010 tile map s -> (s mod B) * a + s / B of
+// an a x B tile, ordered for the leader-only 010 with es-byte elements: groups of 64 (one wave
+// instruction each) whose lanes touch few distinct 64-byte sectors at every step of their
+// walks, padded with kNoLeader to a whole number of groups. `cycles` is the host's count (a
+// list that disagrees is not returned); empty when the tile is too large to order.
+std::vector<uint32_t> order_010_leaders(uint32_t a, uint32_t B, size_t es, uint64_t cycles);
+// That list on the device, kept per (device, a, B, es) like cached_leaders; *entries gets its
+// length (a multiple of 64). Null when it cannot be had now.
+const uint32_t *cached_ordered_leaders(uint64_t a, uint64_t B, size_t es, const PlPlan &pc,
+                                       hipStream_t st, uint32_t *entries);
 // scratch: cycles + 1 32-bit words (the leader list and its counter).
 hipError_t launch_100_leaders(unsigned char *base, uint32_t *scratch, uint64_t inst,
                               uint64_t R, uint64_t C, uint64_t S, const Pt100Plan &pl,

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 namespace lm {
 
@@ -94,7 +96,9 @@ __global__ __launch_bounds__(256) void k_010_pt_wave(Pt010Params p, uint32_t sh_
 // per tile): lane i takes leader i of a round of 64, loads the W slots of its cycle that follow
 // (all W loads in flight together), waits, and stores each value one slot on; the value that
 // returns to the leader is stored last. No visited bits: cycles are disjoint, so no two lanes
-// touch the same element.
+// touch the same element. A cached list may instead hold one start of any slot per cycle,
+// ordered so that the lanes of a round touch few sectors (order_010_leaders), with kNoLeader
+// entries for lanes that have no cycle in their round.
 template <int ES, int W>
 __global__ __launch_bounds__(256) void k_010_pt_lead(Pt010Params p, uint32_t order,
                                                       uint32_t max_leaders, FastDiv32 div_nl,
@@ -155,6 +159,7 @@ __global__ __launch_bounds__(256) void k_010_pt_lead(Pt010Params p, uint32_t ord
     {
       E *in = reinterpret_cast<E *>(p.base) + t * uint64_t(p.n);
       const uint32_t s = list[i - k * nl];
+      if (s == kNoLeader) continue;  // padding of an ordered list
       E carry = in[s];
       uint32_t c = s;
       bool done = false;
@@ -246,6 +251,201 @@ hipError_t launch_ptl_t(const Pt010Params &p, uint32_t order, uint32_t max_leade
 }
 
 }  // namespace
+
+namespace {
+// splitmix64: the descent's moves come from a fixed seed, so an order is reproducible.
+struct Mix64 {
+  uint64_t x;
+  uint64_t next() {
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  }
+  uint32_t below(uint32_t m) { return uint32_t(next() % m); }
+};
+
+// The distinct sectors of one (group, step) with their lane counts (at most 64 pairs).
+struct SectorSet {
+  std::vector<std::pair<uint32_t, uint32_t>> v;
+  int add(uint32_t s) {  // 1 when the sector is new to the set
+    for (auto &e : v)
+      if (e.first == s) {
+        e.second++;
+        return 0;
+      }
+    v.push_back({s, 1u});
+    return 1;
+  }
+  int remove(uint32_t s) {  // -1 when its last lane left
+    for (size_t i = 0; i < v.size(); i++)
+      if (v[i].first == s) {
+        if (--v[i].second == 0) {
+          v[i] = v.back();
+          v.pop_back();
+          return -1;
+        }
+        return 0;
+      }
+    return 0;
+  }
+};
+}  // namespace
+
+// Lanes of one wave instruction that touch the same 64-byte sector cost one L2 request
+// between them (the stores, written through the L1 cache, are most of this kernel's L2
+// requests). So the cycles are placed in groups of 64 lanes that share sectors at every step:
+//   1. any slot of a cycle may start its walk; each cycle takes the start whose walk gives
+//      the smallest key, where the key interleaves the bits of the walk's sector ids, high
+//      bits first, over all steps (so cycles whose walks stay in the same sectors at every
+//      step get close keys);
+//   2. the cycles, sorted by key, fill groups of 64 in turn;
+//   3. a bounded descent with a fixed seed swaps two cycles of different groups or moves a
+//      cycle's start, keeping only moves that lower the sum over groups and steps of
+//      distinct sectors.
+// On the 64 x 16 tile of 4-byte elements this sum falls from 305 (smallest slots in
+// increasing order) to 247 per tile.
+std::vector<uint32_t> order_010_leaders(uint32_t a, uint32_t B, size_t es, uint64_t cycles) {
+  constexpr uint32_t G = 64, kSector = 64, kMaxLen = 64, kMaxTries = 8;
+  const uint64_t n64 = uint64_t(a) * B;
+  if (a < 2 || B < 2 || es == 0 || n64 > (uint64_t(1) << 22) || cycles == 0 ||
+      cycles > (uint64_t(1) << 15))
+    return {};
+  const uint32_t n = uint32_t(n64), last = n - 1;
+  auto next = [&](uint32_t s) { return (s % B) * a + s / B; };
+  // every cycle of length >= 2 of [1, last), its slots in walk order
+  std::vector<uint8_t> seen(n, 0);
+  std::vector<uint32_t> slots, off{0};
+  for (uint32_t s = 1; s < last; s++) {
+    if (seen[s]) continue;
+    uint32_t t = s, len = 0;
+    do {
+      seen[t] = 1;
+      slots.push_back(t);
+      len++;
+      t = next(t);
+    } while (t != s && len <= kMaxLen);
+    if (t != s) return {};  // a cycle longer than kMaxLen: not ordered
+    if (len == 1) slots.pop_back();
+    else off.push_back(uint32_t(slots.size()));
+  }
+  const uint32_t nc = uint32_t(off.size() - 1);
+  if (nc != cycles) return {};
+  auto len_of = [&](uint32_t c) { return off[c + 1] - off[c]; };
+  uint32_t L = 0;
+  for (uint32_t c = 0; c < nc; c++) L = std::max(L, len_of(c));
+  auto sector = [&](uint32_t c, uint32_t rot, uint32_t j) {
+    return uint32_t(uint64_t(slots[off[c] + (rot + j) % len_of(c)]) * es / kSector);
+  };
+  const uint32_t max_sector = uint32_t((n64 * es - 1) / kSector);
+  uint32_t sb = 1;
+  while (sb < 32 && (max_sector >> sb) != 0) sb++;
+  const uint32_t kw = (L * sb + 63) / 64;  // 64-bit words per key
+  // 1. each cycle's start: the smallest key over (up to kMaxTries) candidate starts
+  std::vector<uint64_t> keys(size_t(nc) * kw), cand(kw);
+  std::vector<uint32_t> rot_of(nc, 0);
+  auto make_key = [&](uint32_t c, uint32_t rot, uint64_t *k) {
+    std::fill(k, k + kw, 0ull);
+    uint32_t pos = 0;
+    for (int b = int(sb) - 1; b >= 0; b--)
+      for (uint32_t j = 0; j < L; j++, pos++)
+        if (j < len_of(c) && ((sector(c, rot, j) >> b) & 1u))
+          k[pos / 64] |= uint64_t(1) << (63 - pos % 64);
+  };
+  for (uint32_t c = 0; c < nc; c++) {
+    const uint32_t len = len_of(c), tries = std::min(len, kMaxTries);
+    uint64_t *best = &keys[size_t(c) * kw];
+    make_key(c, 0, best);
+    for (uint32_t i = 1; i < tries; i++) {
+      const uint32_t rot = uint32_t(uint64_t(i) * len / tries);
+      make_key(c, rot, cand.data());
+      if (std::lexicographical_compare(cand.begin(), cand.end(), best, best + kw)) {
+        std::copy(cand.begin(), cand.end(), best);
+        rot_of[c] = rot;
+      }
+    }
+  }
+  // 2. sorted by key into groups
+  std::vector<uint32_t> order(nc);
+  for (uint32_t c = 0; c < nc; c++) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    const uint64_t *kx = &keys[size_t(x) * kw], *ky = &keys[size_t(y) * kw];
+    return std::lexicographical_compare(kx, kx + kw, ky, ky + kw);
+  });
+  const uint32_t ng = (nc + G - 1) / G;
+  std::vector<uint32_t> grp_of(nc);
+  std::vector<SectorSet> sets(size_t(ng) * L);
+  auto put = [&](uint32_t c, uint32_t g, uint32_t rot, bool add) {
+    int d = 0;
+    for (uint32_t j = 0; j < len_of(c); j++) {
+      SectorSet &ss = sets[size_t(g) * L + j];
+      d += add ? ss.add(sector(c, rot, j)) : ss.remove(sector(c, rot, j));
+    }
+    return d;
+  };
+  for (uint32_t i = 0; i < nc; i++) {
+    grp_of[order[i]] = i / G;
+    put(order[i], i / G, rot_of[order[i]], true);
+  }
+  // 3. descent (each move costs about 4 L set updates of up to 64 entries)
+  if (ng > 1 || L > 1) {
+    const uint64_t iters =
+        std::min<uint64_t>(uint64_t(50) * nc, uint64_t(40000000) / (uint64_t(L) * 256 + 1));
+    Mix64 rng{0x5eedull + n64 * 131 + es};
+    for (uint64_t it = 0; it < iters; it++) {
+      if ((rng.next() & 1) || ng == 1) {
+        const uint32_t c = rng.below(nc), len = len_of(c);
+        const uint32_t r2 = rng.below(len), g = grp_of[c], r = rot_of[c];
+        if (r2 == r) continue;
+        const int d = put(c, g, r, false) + put(c, g, r2, true);
+        if (d < 0) {
+          rot_of[c] = r2;
+        } else {
+          put(c, g, r2, false);
+          put(c, g, r, true);
+        }
+      } else {
+        const uint32_t x = rng.below(nc), y = rng.below(nc);
+        const uint32_t gx = grp_of[x], gy = grp_of[y];
+        if (gx == gy) continue;
+        const int d = put(x, gx, rot_of[x], false) + put(y, gy, rot_of[y], false) +
+                      put(x, gy, rot_of[x], true) + put(y, gx, rot_of[y], true);
+        if (d < 0) {
+          grp_of[x] = gy;
+          grp_of[y] = gx;
+        } else {
+          put(x, gy, rot_of[x], false);
+          put(y, gx, rot_of[y], false);
+          put(x, gx, rot_of[x], true);
+          put(y, gy, rot_of[y], true);
+        }
+      }
+    }
+  }
+  // the list: group by group, each padded to 64 entries
+  std::vector<uint32_t> list(size_t(ng) * G, kNoLeader), fill(ng, 0);
+  for (uint32_t i = 0; i < nc; i++) {
+    const uint32_t c = order[i], g = grp_of[c];
+    if (fill[g] >= G) return {};  // never: swaps keep every group's size
+    list[size_t(g) * G + fill[g]++] = slots[off[c] + rot_of[c]];
+  }
+  // self-check: the walks from the listed starts cover every slot of [1, last) that is not a
+  // fixed point exactly once
+  std::fill(seen.begin(), seen.end(), 0);
+  uint64_t covered = 0;
+  for (uint32_t s : list) {
+    if (s == kNoLeader) continue;
+    uint32_t t = s;
+    do {
+      if (t == 0 || t >= last || seen[t]) return {};
+      seen[t] = 1;
+      covered++;
+      t = next(t);
+    } while (t != s);
+  }
+  if (covered != slots.size()) return {};
+  return list;
+}
 
 uint32_t bs010_pitch(uint32_t B) { return (B & 1u) ? B : B + 1; }
 
@@ -405,9 +605,20 @@ hipError_t launch_010_pttwac(unsigned char *base, size_t es, uint64_t A, uint64_
     const uint32_t lblocks = uint32_t(std::max<uint64_t>(
         1, std::min<uint64_t>((A + 3) / 4, grid_blocks(tuning("pt010_lead_blocks_per_cu")))));
     const uint64_t nw = uint64_t(lblocks) * 4;
-    if ((A + nw - 1) / nw * pc.cycles >= (uint64_t(1) << 32)) return hipErrorInvalidValue;
-    const uint32_t ml = uint32_t(pc.cycles);
-    const uint32_t *cached = tuning("pt010_lead_cached") ? cached_leaders(a, B, pc, st) : nullptr;
+    // "pt010_lead_cached": 0 every workgroup searches the leaders; 1 they come from the cached
+    // list of the a x B grid; 2 from the list ordered for few sectors per instruction
+    // (cached per tile shape and element size), when it fits the LDS budget
+    const int64_t lead_src = tuning("pt010_lead_cached");
+    uint32_t ml = uint32_t(pc.cycles);
+    const uint32_t *cached = nullptr;
+    if (lead_src == 2) {
+      uint32_t entries = 0;
+      cached = cached_ordered_leaders(a, B, es, pc, st, &entries);
+      if (cached && (uint64_t(entries) + 4) * 4 <= current_lds_budget()) ml = entries;
+      else cached = nullptr;
+    }
+    if (!cached && lead_src >= 1) cached = cached_leaders(a, B, pc, st);
+    if ((A + nw - 1) / nw * ml >= (uint64_t(1) << 32)) return hipErrorInvalidValue;
     switch (es) {
       case 1: return launch_ptl_t<1>(p, pc.order, ml, lblocks, cached, st);
       case 2: return launch_ptl_t<2>(p, pc.order, ml, lblocks, cached, st);

@@ -120,7 +120,7 @@ def test_tuning_keys_defaults_and_limits():
                 "autotune": 0, "oop_order": 2, "cl_max_wgs": 16, "cl_min_bytes": 256 << 10,
                 "cl_min_inst": 64, "cl_fuse": 1, "cl_spin": 1 << 22,
                 "batch_arena_mb": 1024, "batch_arena_max_mb": 16,
-                "batch_level_lanes": 4, "pt010_lead_cached": 1}
+                "batch_level_lanes": 4, "pt010_lead_cached": 2}
     for k, v in defaults.items():
         assert lm.get_tuning(k) == v, k
     assert lm.get_tuning("no_such_key") == 0

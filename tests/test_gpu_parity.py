@@ -105,16 +105,22 @@ def test_010_pttwac_bug536_shapes(hip, orc):
                 check(hip.transpose_010_pttwac, x, want, h // t, t, w, R=R, P=1, elem_bytes=4)
 
 
-@pytest.mark.parametrize("mode", ["lead", "wave", "wg"])
+@pytest.mark.parametrize("mode", ["lead", "lead_search", "lead_cached", "wave", "wg"])
 @pytest.mark.parametrize("es", [1, 4, 8, 16])
 def test_010_pttwac_element_sizes(hip, orc, es, mode):
     """Every form of the cycle-following 010: leader-only when all cycles are short ("lead",
-    the default where it applies), visited bits with one tile per wave ("wave") and visited bits
-    with one tile per 256-lane workgroup ("wg"), forced with the knobs "pt010_lead_order" and
-    "pt010_wave"."""
-    keep = {k: hip.get_tuning(k) for k in ("pt010_wave", "pt010_lead_order")}
-    hip.set_tuning("pt010_lead_order", 64 if mode == "lead" else 0)
+    the default where it applies, with its leaders from the list ordered for few sectors per
+    instruction; searched by every workgroup, "lead_search"; or from the cached list in search
+    order, "lead_cached"), visited bits with one tile per wave ("wave") and visited bits with
+    one tile per 256-lane workgroup ("wg"), forced with the knobs "pt010_lead_order",
+    "pt010_lead_cached" and "pt010_wave"."""
+    keep = {k: hip.get_tuning(k) for k in ("pt010_wave", "pt010_lead_order", "pt010_lead_cached")}
+    hip.set_tuning("pt010_lead_order", 64 if mode.startswith("lead") else 0)
     hip.set_tuning("pt010_wave", 0 if mode == "wg" else 1)
+    if mode == "lead_search":
+        hip.set_tuning("pt010_lead_cached", 0)
+    elif mode == "lead_cached":
+        hip.set_tuning("pt010_lead_cached", 1)
     try:
         for (A, a, B) in ((50, 33, 47), (7, 64, 100), (3, 300, 211), (1000, 2, 3), (4099, 64, 16),
                           (257, 16, 64), (9, 8, 8), (5, 2, 1000)):
