@@ -269,30 +269,38 @@ const uint32_t *cached_ordered_leaders(uint64_t a, uint64_t B, size_t es, const 
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   hipStreamIsCapturing(st, &cs);
   const bool capturing = cs != hipStreamCaptureStatusNone;
-  std::lock_guard<std::mutex> lk(g_lc_mu);
   const auto key = std::make_tuple(dev, a, B, uint64_t(es));
-  auto it = g_lc.find(key);
-  if (it != g_lc.end()) {
-    LeaderEntry &f = it->second;
-    if (!f.list) return nullptr;  // this shape could not be ordered
+  // A known entry: ordered list (or a shape remembered as not ordered).
+  auto lookup = [&](LeaderEntry &f) -> const uint32_t * {
+    if (!f.list) return nullptr;
     *entries = uint32_t(f.bytes / 4);
     if (capturing) return f.done ? f.list : nullptr;
     if (!f.done && hipEventQuery(f.ready) == hipSuccess) f.done = true;
     if (!f.done && hipStreamWaitEvent(st, f.ready, 0) != hipSuccess) return nullptr;
     return f.list;
+  };
+  {
+    std::lock_guard<std::mutex> lk(g_lc_mu);
+    auto it = g_lc.find(key);
+    if (it != g_lc.end()) return lookup(it->second);
   }
   if (capturing) return nullptr;
-  LeaderEntry e;
-  e.host = order_010_leaders(uint32_t(a), uint32_t(B), es, pc.cycles);
-  if (e.host.empty()) {
-    g_lc[key] = LeaderEntry{};  // remembered, so the ordering is not tried again
+  // ordered outside the lock (milliseconds for large tiles); a thread that got there first wins
+  std::vector<uint32_t> host = order_010_leaders(uint32_t(a), uint32_t(B), es, pc.cycles);
+  std::lock_guard<std::mutex> lk(g_lc_mu);
+  auto it = g_lc.find(key);
+  if (it != g_lc.end()) return lookup(it->second);
+  const size_t bytes = host.size() * 4;
+  if (host.empty() || g_lc_bytes + bytes > kLeaderCacheBytes) {
+    g_lc[key] = LeaderEntry{};  // remembered: not ordered again until hip_marshal_finalize
     return nullptr;
   }
-  e.bytes = e.host.size() * 4;
-  if (g_lc_bytes + e.bytes > kLeaderCacheBytes) return nullptr;
-  if (hipMalloc(reinterpret_cast<void **>(&e.list), e.bytes) != hipSuccess) return nullptr;
+  LeaderEntry e;
+  e.host = std::move(host);
+  e.bytes = bytes;
+  if (hipMalloc(reinterpret_cast<void **>(&e.list), bytes) != hipSuccess) return nullptr;
   if (hipEventCreateWithFlags(&e.ready, hipEventDisableTiming) != hipSuccess ||
-      hipMemcpyAsync(e.list, e.host.data(), e.bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(e.list, e.host.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipEventRecord(e.ready, st) != hipSuccess) {
     hipStreamSynchronize(st);
     hipFree(e.list);
@@ -301,7 +309,7 @@ const uint32_t *cached_ordered_leaders(uint64_t a, uint64_t B, size_t es, const 
   }
   *entries = uint32_t(e.host.size());
   const uint32_t *list = e.list;
-  g_lc_bytes += e.bytes;
+  g_lc_bytes += bytes;
   g_lc[key] = std::move(e);
   return list;
 }

@@ -47,6 +47,12 @@ def work_items(orc, seed):
     add("AoS->ASTA 65536x16/64", lambda h, b, s, e: h.aos_asta(b, 65536, 16, 64, elapsed=e,
                                                                 elem_bytes=4, stream=s),
         x, orc.op_010(x, 1024, 64, 16, es=4))
+    # the cycle-following 010 on 64 x 16 tiles: every thread asks for the same ordered leader
+    # list at once on its first call
+    x = orc.tagged(2048 * 64 * 16, 4, seed=seed + 7)
+    add("PT010 2048x64x16 f32", lambda h, b, s, e: h.transpose_010_pttwac(
+        b, 2048, 64, 16, elapsed=e, R=1, P=1, elem_bytes=4, stream=s),
+        x, orc.op_010(x, 2048, 64, 16, es=4))
     # SoA -> ASTA: the 100 on the 16 x 1024 grid of 64-element super-elements
     x = orc.tagged(65536 * 16, 4, seed=seed + 4)
     add("SoA->ASTA 65536x16/64", lambda h, b, s, e: h.soa_asta(b, 65536, 16, 64, elapsed=e,
