@@ -134,6 +134,33 @@ def test_010_pttwac_element_sizes(hip, orc, es, mode):
             hip.set_tuning(k, v)
 
 
+def test_010_pttwac_random_tiles(hip, orc):
+    """Random tile shapes and element sizes through the cycle-following 010 with its default
+    leader source (lists ordered per tile shape and element size, any start slot per cycle,
+    padded rounds): every result bit-identical to the oracle, then the inverse restores."""
+    rng = np.random.default_rng(2024)
+    for k in range(40):
+        a, B = (int(v) for v in rng.integers(2, 160, size=2))
+        es = int(rng.choice([1, 2, 4, 8, 16]))
+        A = int(rng.integers(1, 6))
+        say("pttwac random", k, A, a, B, es)
+        x = orc.tagged(A * a * B, es, seed=k + 77)
+        want = orc.op_010(x, A, a, B, es=es)
+        check(hip.transpose_010_pttwac, x, want, A, a, B, R=1, P=1, elem_bytes=es)
+        check(hip.transpose_010_pttwac, want, x, A, B, a, R=1, P=1, elem_bytes=es)
+    # power-of-two sides: the tile map is a rotation of the slot's bits, every cycle short, so
+    # these all take the leader-only form
+    for k in range(24):
+        a, B = (1 << int(v) for v in rng.integers(1, 9, size=2))
+        es = int(rng.choice([1, 2, 4, 8, 16]))
+        A = int(rng.integers(1, 9))
+        say("pttwac pow2", k, A, a, B, es)
+        x = orc.tagged(A * a * B, es, seed=k + 177)
+        want = orc.op_010(x, A, a, B, es=es)
+        check(hip.transpose_010_pttwac, x, want, A, a, B, R=1, P=1, elem_bytes=es)
+        check(hip.transpose_010_pttwac, want, x, A, B, a, R=1, P=1, elem_bytes=es)
+
+
 def test_100_bug537_shapes(hip, orc):
     # test/cl_unittest.cc:171-223: 100(w, h/t, t), w in {197, 215}, t = 1..1024 powers of two
     for w, h0 in ((197, 35588), (215, 44609)):
