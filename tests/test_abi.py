@@ -176,3 +176,17 @@ def test_cycle_count_matches_the_oracle():
     assert lm.cycle_count(16, 1 << 18) == (381300, 11)   # 100(16, 2^18, 64): 2^24 structs
     assert lm.cycle_count(1 << 18, 16) == (381300, 11)   # and its inverse
     assert lm.cycle_count(1800, 7200, max_order=64) == (0, 0)  # order far above 64
+
+
+def test_c_caller_host_checks(tmp_path):
+    """A C++ program built against include/*.h and linked to the library (no Python in
+    between) calls the entry points that need no device: version, planner, knobs, argument
+    validation."""
+    from tests.capi import build as capi
+    exe = capi.build(str(tmp_path / "capi_check"))
+    env = dict(os.environ)  # built outside the tree: its relative rpath does not reach the .so
+    env["LD_LIBRARY_PATH"] = os.path.join(ROOT, "libmarshal_amd") + ":" + env.get(
+        "LD_LIBRARY_PATH", "")
+    r = subprocess.run([exe, "host"], capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "capi host ok" in r.stdout
