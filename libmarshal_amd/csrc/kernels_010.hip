@@ -306,7 +306,7 @@ struct SectorSet {
 // On the 64 x 16 tile of 4-byte elements this sum falls from 305 (smallest slots in
 // increasing order) to 247 per tile.
 std::vector<uint32_t> order_010_leaders(uint32_t a, uint32_t B, size_t es, uint64_t cycles) {
-  constexpr uint32_t G = 64, kSector = 64, kMaxLen = 64, kMaxTries = 8;
+  constexpr uint32_t G = 64, kSector = 64, kMaxLen = 64;
   const uint64_t n64 = uint64_t(a) * B;
   if (a < 2 || B < 2 || es == 0 || n64 > (uint64_t(1) << 22) || cycles == 0 ||
       cycles > (uint64_t(1) << 15))
@@ -340,28 +340,31 @@ std::vector<uint32_t> order_010_leaders(uint32_t a, uint32_t B, size_t es, uint6
   const uint32_t max_sector = uint32_t((n64 * es - 1) / kSector);
   uint32_t sb = 1;
   while (sb < 32 && (max_sector >> sb) != 0) sb++;
-  const uint32_t kw = (L * sb + 63) / 64;  // 64-bit words per key
-  // 1. each cycle's start: the smallest key over (up to kMaxTries) candidate starts
-  std::vector<uint64_t> keys(size_t(nc) * kw), cand(kw);
+  // 1. each cycle's start: the smallest key over all its slots. The key's bits, taken plane by
+  // plane from the sector ids' top bit down, step by step within a plane, are kept as one
+  // 64-bit word per plane (step j at bit 63 - j; L <= 64), so starting the walk r slots later
+  // rotates every word's top `len` bits left by r.
+  std::vector<uint64_t> keys(size_t(nc) * sb), base(sb), cand(sb);
   std::vector<uint32_t> rot_of(nc, 0);
-  auto make_key = [&](uint32_t c, uint32_t rot, uint64_t *k) {
-    std::fill(k, k + kw, 0ull);
-    uint32_t pos = 0;
-    for (int b = int(sb) - 1; b >= 0; b--)
-      for (uint32_t j = 0; j < L; j++, pos++)
-        if (j < len_of(c) && ((sector(c, rot, j) >> b) & 1u))
-          k[pos / 64] |= uint64_t(1) << (63 - pos % 64);
-  };
   for (uint32_t c = 0; c < nc; c++) {
-    const uint32_t len = len_of(c), tries = std::min(len, kMaxTries);
-    uint64_t *best = &keys[size_t(c) * kw];
-    make_key(c, 0, best);
-    for (uint32_t i = 1; i < tries; i++) {
-      const uint32_t rot = uint32_t(uint64_t(i) * len / tries);
-      make_key(c, rot, cand.data());
-      if (std::lexicographical_compare(cand.begin(), cand.end(), best, best + kw)) {
+    const uint32_t len = len_of(c);
+    std::fill(base.begin(), base.end(), 0ull);
+    for (uint32_t j = 0; j < len; j++) {
+      const uint32_t sec = sector(c, 0, j);
+      for (uint32_t b = 0; b < sb; b++)
+        if ((sec >> b) & 1u) base[sb - 1 - b] |= uint64_t(1) << (63 - j);
+    }
+    uint64_t *best = &keys[size_t(c) * sb];
+    std::copy(base.begin(), base.end(), best);
+    const uint64_t mask = len == 64 ? ~0ull : (uint64_t(1) << len) - 1;
+    for (uint32_t r = 1; r < len; r++) {
+      for (uint32_t w = 0; w < sb; w++) {
+        const uint64_t x = base[w] >> (64 - len);
+        cand[w] = (((x << r) | (x >> (len - r))) & mask) << (64 - len);
+      }
+      if (std::lexicographical_compare(cand.begin(), cand.end(), best, best + sb)) {
         std::copy(cand.begin(), cand.end(), best);
-        rot_of[c] = rot;
+        rot_of[c] = r;
       }
     }
   }
@@ -369,8 +372,8 @@ std::vector<uint32_t> order_010_leaders(uint32_t a, uint32_t B, size_t es, uint6
   std::vector<uint32_t> order(nc);
   for (uint32_t c = 0; c < nc; c++) order[c] = c;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-    const uint64_t *kx = &keys[size_t(x) * kw], *ky = &keys[size_t(y) * kw];
-    return std::lexicographical_compare(kx, kx + kw, ky, ky + kw);
+    const uint64_t *kx = &keys[size_t(x) * sb], *ky = &keys[size_t(y) * sb];
+    return std::lexicographical_compare(kx, kx + sb, ky, ky + sb);
   });
   const uint32_t ng = (nc + G - 1) / G;
   std::vector<uint32_t> grp_of(nc);
@@ -387,10 +390,11 @@ std::vector<uint32_t> order_010_leaders(uint32_t a, uint32_t B, size_t es, uint6
     grp_of[order[i]] = i / G;
     put(order[i], i / G, rot_of[order[i]], true);
   }
-  // 3. descent (each move costs about 4 L set updates of up to 64 entries)
+  // 3. descent (each move costs about 4 L set updates of up to 64 entries; at most ~10^7 such
+  // steps, a few milliseconds once per tile shape and element size)
   if (ng > 1 || L > 1) {
     const uint64_t iters =
-        std::min<uint64_t>(uint64_t(50) * nc, uint64_t(40000000) / (uint64_t(L) * 256 + 1));
+        std::min<uint64_t>(uint64_t(50) * nc, uint64_t(10000000) / (uint64_t(L) * 256 + 1));
     Mix64 rng{0x5eedull + n64 * 131 + es};
     for (uint64_t it = 0; it < iters; it++) {
       if ((rng.next() & 1) || ng == 1) {
