@@ -304,7 +304,7 @@ struct SectorSet {
 //      cycle's start, keeping only moves that lower the sum over groups and steps of
 //      distinct sectors.
 // On the 64 x 16 tile of 4-byte elements this sum falls from 305 (smallest slots in
-// increasing order) to 247 per tile.
+// increasing order) to about 250 per tile.
 std::vector<uint32_t> order_010_leaders(uint32_t a, uint32_t B, size_t es, uint64_t cycles) {
   constexpr uint32_t G = 64, kSector = 64, kMaxLen = 64;
   const uint64_t n64 = uint64_t(a) * B;
