@@ -31,14 +31,22 @@ def _compile(src):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
-    return obj
+    # compiler warnings are shown (the host pass's note that it does not know the packed-FP32
+    # feature is expected and left out)
+    warnings = [ln for ln in r.stderr.splitlines()
+                if "warning:" in ln and "packed-fp32-ops" not in ln]
+    return obj, warnings
 
 
 def build(verbose=False):
     os.makedirs(os.path.join(CSRC, "_build"), exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        done = list(ex.map(_compile, srcs))
+    objs = [o for o, _ in done]
+    for _, warnings in done:
+        for w in warnings:
+            print(w, file=sys.stderr)
     tmp = OUT + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)

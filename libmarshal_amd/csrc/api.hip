@@ -232,7 +232,7 @@ const uint32_t *cached_leaders(uint64_t R, uint64_t C, const PlPlan &pc, hipStre
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  hipStreamIsCapturing(st, &cs);
+  (void)hipStreamIsCapturing(st, &cs);
   const bool capturing = cs != hipStreamCaptureStatusNone;
   std::lock_guard<std::mutex> lk(g_lc_mu);
   const auto key = std::make_tuple(dev, R, C, uint64_t(0));
@@ -252,9 +252,9 @@ const uint32_t *cached_leaders(uint64_t R, uint64_t C, const PlPlan &pc, hipStre
   if (hipEventCreateWithFlags(&e.ready, hipEventDisableTiming) != hipSuccess ||
       launch_find_leaders(e.list, R, C, pc, st) != hipSuccess ||
       hipEventRecord(e.ready, st) != hipSuccess) {
-    hipStreamSynchronize(st);
-    hipFree(e.list);
-    if (e.ready) hipEventDestroy(e.ready);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(e.list);
+    if (e.ready) (void)hipEventDestroy(e.ready);
     return nullptr;
   }
   g_lc[key] = e;
@@ -267,7 +267,7 @@ const uint32_t *cached_ordered_leaders(uint64_t a, uint64_t B, size_t es, const 
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || a > 0xffffffffu || B > 0xffffffffu) return nullptr;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  hipStreamIsCapturing(st, &cs);
+  (void)hipStreamIsCapturing(st, &cs);
   const bool capturing = cs != hipStreamCaptureStatusNone;
   const auto key = std::make_tuple(dev, a, B, uint64_t(es));
   // A known entry: ordered list (or a shape remembered as not ordered).
@@ -302,9 +302,9 @@ const uint32_t *cached_ordered_leaders(uint64_t a, uint64_t B, size_t es, const 
   if (hipEventCreateWithFlags(&e.ready, hipEventDisableTiming) != hipSuccess ||
       hipMemcpyAsync(e.list, e.host.data(), bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipEventRecord(e.ready, st) != hipSuccess) {
-    hipStreamSynchronize(st);
-    hipFree(e.list);
-    if (e.ready) hipEventDestroy(e.ready);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(e.list);
+    if (e.ready) (void)hipEventDestroy(e.ready);
     return nullptr;
   }
   *entries = uint32_t(e.host.size());
@@ -325,7 +325,7 @@ struct Lease {
   // Returns a pointer to at least `bytes` of flag space, ordered after earlier users.
   uint32_t *get(size_t bytes) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    hipStreamIsCapturing(st, &cs);
+    (void)hipStreamIsCapturing(st, &cs);
     if (cs != hipStreamCaptureStatusNone && p.cap < bytes) {
       // Growing would allocate during capture; reserve first (hip_marshal_reserve).
       fprintf(stderr, "[libmarshal] scratch pool too small under stream capture (need %zu "
@@ -334,17 +334,21 @@ struct Lease {
     }
     if (pool_grow(p, bytes)) return nullptr;
     if (cs != hipStreamCaptureStatusNone) p.captured = true;
-    if (p.has_ev && p.last != st && cs == hipStreamCaptureStatusNone)
-      hipStreamWaitEvent(st, p.ev, 0);
+    if (p.has_ev && p.last != st && cs == hipStreamCaptureStatusNone &&
+        hipStreamWaitEvent(st, p.ev, 0) != hipSuccess) {
+      // without the wait this stream could clear bits another stream's stage still uses
+      fprintf(stderr, "[libmarshal] scratch pool: ordering after its previous user failed\n");
+      return nullptr;
+    }
     ok = true;
     return static_cast<uint32_t *>(p.ptr);
   }
   ~Lease() {
     if (!ok) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    hipStreamIsCapturing(st, &cs);
+    (void)hipStreamIsCapturing(st, &cs);
     if (cs != hipStreamCaptureStatusNone) return;
-    if (!p.ev) hipEventCreateWithFlags(&p.ev, hipEventDisableTiming);
+    if (!p.ev) (void)hipEventCreateWithFlags(&p.ev, hipEventDisableTiming);
     if (p.ev && hipEventRecord(p.ev, st) == hipSuccess) {
       p.has_ev = true;
       p.last = st;
@@ -369,13 +373,13 @@ struct Timer {
       : st(s), id(name), bytes(moved_bytes), out(elapsed), on(g_profiling.load() != 0) {
     if (!on) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    hipStreamIsCapturing(st, &cs);
+    (void)hipStreamIsCapturing(st, &cs);
     if (cs != hipStreamCaptureStatusNone) { on = false; return; }
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
       on = false;
       return;
     }
-    hipEventRecord(e0, st);
+    (void)hipEventRecord(e0, st);
   }
   bool finish(bool err) {
     if (!on) return err;
@@ -399,15 +403,15 @@ struct Timer {
         fprintf(stderr, "\n");
       }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     on = false;
     return err;
   }
   ~Timer() {
     if (on) {
-      hipEventDestroy(e0);
-      hipEventDestroy(e1);
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
     }
   }
 };
@@ -1516,7 +1520,7 @@ static bool autotune_full(hipStream_t st, unsigned char *buf, size_t es, uint64_
   if (bytes <= (uint64_t(tuning("st_scratch_mb")) << 20) &&
       hipMalloc(&snap, bytes) == hipSuccess &&
       hipMemcpyAsync(snap, buf, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {
-    hipFree(snap);
+    (void)hipFree(snap);
     snap = nullptr;
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1542,18 +1546,21 @@ static bool autotune_full(hipStream_t st, unsigned char *buf, size_t es, uint64_
       best = i;
     }
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (err) {
-    hipStreamSynchronize(st);
-    if (snap) hipMemcpy(buf, snap, bytes, hipMemcpyDeviceToDevice);
+    (void)hipStreamSynchronize(st);
+    const bool restored =
+        snap && hipMemcpy(buf, snap, bytes, hipMemcpyDeviceToDevice) == hipSuccess;
     fprintf(stderr, "[libmarshal] autotune %llux%llu: a candidate failed; the buffer %s\n",
             (unsigned long long)M, (unsigned long long)N,
-            snap ? "was restored" : "was not restored (larger than the scratch cap)");
+            restored ? "was restored"
+            : snap   ? "was not restored (the copy back failed)"
+                     : "was not restored (larger than the scratch cap)");
   }
   if (snap) {
-    hipStreamSynchronize(st);
-    hipFree(snap);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(snap);
   }
   if (err || best == SIZE_MAX) return true;
   Choice c = cand[best];
@@ -1654,7 +1661,7 @@ static bool autotune_step(hipStream_t st, const hip_marshal_desc &d, ClassChoice
   if (bytes <= (uint64_t(tuning("st_scratch_mb")) << 20) &&
       hipMalloc(&snap, bytes) == hipSuccess &&
       hipMemcpyAsync(snap, buf, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {
-    hipFree(snap);
+    (void)hipFree(snap);
     snap = nullptr;
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1695,21 +1702,20 @@ static bool autotune_step(hipStream_t st, const hip_marshal_desc &d, ClassChoice
       best = int(i);
     }
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (err) {
-    hipStreamSynchronize(st);
-    if (snap) {
-      hipMemcpy(buf, snap, bytes, hipMemcpyDeviceToDevice);
+    (void)hipStreamSynchronize(st);
+    if (snap && hipMemcpy(buf, snap, bytes, hipMemcpyDeviceToDevice) == hipSuccess) {
       fprintf(stderr, "[libmarshal] autotune: a candidate failed; the buffer was restored\n");
     } else {
       fprintf(stderr, "[libmarshal] autotune: a candidate failed; the buffer was not "
-                      "restored (larger than the scratch cap)\n");
+                      "restored (%s)\n", snap ? "the copy back failed" : "larger than the scratch cap");
     }
   }
   if (snap) {
-    hipStreamSynchronize(st);
-    hipFree(snap);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(snap);
   }
   if (err) return true;
   if (best < 0) {  // nothing to move (unit tiles) or no form can run: the model decides
@@ -1853,38 +1859,38 @@ void hip_marshal_finalize(void) {
   {
     std::lock_guard<std::mutex> lk(g_lc_mu);
     int cur = 0;
-    hipGetDevice(&cur);
+    (void)hipGetDevice(&cur);
     for (auto &kv : g_lc) {
-      hipSetDevice(std::get<0>(kv.first));
+      (void)hipSetDevice(std::get<0>(kv.first));
       if (kv.second.ready) {
-        hipEventSynchronize(kv.second.ready);
-        hipEventDestroy(kv.second.ready);
+        (void)hipEventSynchronize(kv.second.ready);
+        (void)hipEventDestroy(kv.second.ready);
       }
-      if (kv.second.list) hipFree(kv.second.list);
+      if (kv.second.list) (void)hipFree(kv.second.list);
     }
     g_lc.clear();
     g_lc_bytes = 0;
-    hipSetDevice(cur);
+    (void)hipSetDevice(cur);
   }
   for (int d = 0; d < 64; d++) {
     Pool &p = g_pool[d];
     std::lock_guard<std::mutex> lk(p.mu);
     if (!p.ptr && !p.ev && p.retired.empty()) continue;
     int cur = 0;
-    hipGetDevice(&cur);
-    hipSetDevice(d);
-    if (p.has_ev) hipEventSynchronize(p.ev);
-    if (p.ptr) hipFree(p.ptr);
-    for (void *q : p.retired) hipFree(q);
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(d);
+    if (p.has_ev) (void)hipEventSynchronize(p.ev);
+    if (p.ptr) (void)hipFree(p.ptr);
+    for (void *q : p.retired) (void)hipFree(q);
     p.retired.clear();
     p.captured = false;
-    if (p.ev) hipEventDestroy(p.ev);
+    if (p.ev) (void)hipEventDestroy(p.ev);
     p.ptr = nullptr;
     p.cap = 0;
     p.ev = nullptr;
     p.has_ev = false;
     p.last = nullptr;
-    hipSetDevice(cur);
+    (void)hipSetDevice(cur);
   }
 }
 
@@ -1952,7 +1958,7 @@ bool hip_transpose_es(hipStream_t st, void *src, size_t es, int64_t A, int64_t a
       tuned = g_tuned.count(std::make_tuple(M, N, es, lds_budget(), true, planning_restricted())) != 0;
     }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    hipStreamIsCapturing(st, &cs);
+    (void)hipStreamIsCapturing(st, &cs);
     Choice c;
     if (!tuned && cs == hipStreamCaptureStatusNone && M > 1 && N > 1)
       autotune_full(st, buf, es, M, N, true, &c);

@@ -404,7 +404,7 @@ static bool build_group_graph(const BatchPlan &b, void *params, uint32_t *flags,
   LM_CHECK_HIP(hipGraphCreate(&gr, 0));
   auto fail = [&](hipError_t e, const char *what) {
     fprintf(stderr, "[libmarshal] batch graph: %s: %s\n", what, hipGetErrorString(e));
-    hipGraphDestroy(gr);
+    (void)hipGraphDestroy(gr);
     return true;
   };
   std::vector<hipGraphNode_t> prev;  // nodes the next level waits for
@@ -972,8 +972,8 @@ bool autotune_classes(const hip_marshal_desc *descs, int64_t count) {
     int form = 0;
     err = hip_marshal_autotune_desc(st, &d, &form);
   }
-  hipStreamSynchronize(st);
-  hipStreamDestroy(st);
+  (void)hipStreamSynchronize(st);
+  (void)hipStreamDestroy(st);
   return err;
 }
 }  // namespace lm
@@ -1085,8 +1085,8 @@ bool hip_marshal_batch_run(hip_marshal_batch *hb, hipStream_t st, uint64_t *elap
     LM_CHECK_HIP(hipEventSynchronize(e1));
     LM_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
     *elapsed_ns += uint64_t(double(ms) * 1e6 + 0.5);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   }
   return false;
 }
@@ -1117,24 +1117,24 @@ const char *hip_marshal_batch_route(const hip_marshal_batch *hb, int64_t i) {
 
 void hip_marshal_batch_destroy(hip_marshal_batch *hb) {
   if (!hb) return;
-  if (hb->exec) hipGraphExecDestroy(hb->exec);
-  if (hb->graph) hipGraphDestroy(hb->graph);
-  if (hb->params) hipFree(hb->params);
-  if (hb->flags) hipFree(hb->flags);
-  if (hb->arena) hipFree(hb->arena);
+  if (hb->exec) (void)hipGraphExecDestroy(hb->exec);
+  if (hb->graph) (void)hipGraphDestroy(hb->graph);
+  if (hb->params) (void)hipFree(hb->params);
+  if (hb->flags) (void)hipFree(hb->flags);
+  if (hb->arena) (void)hipFree(hb->arena);
   for (int l = 0; l < kSoloLanes; l++) {
-    if (hb->solo.st[l]) hipStreamSynchronize(hb->solo.st[l]);
-    if (hb->solo.scr[l]) hipFree(hb->solo.scr[l]);
-    if (hb->solo.join[l]) hipEventDestroy(hb->solo.join[l]);
-    if (hb->solo.st[l]) hipStreamDestroy(hb->solo.st[l]);
+    if (hb->solo.st[l]) (void)hipStreamSynchronize(hb->solo.st[l]);
+    if (hb->solo.scr[l]) (void)hipFree(hb->solo.scr[l]);
+    if (hb->solo.join[l]) (void)hipEventDestroy(hb->solo.join[l]);
+    if (hb->solo.st[l]) (void)hipStreamDestroy(hb->solo.st[l]);
   }
-  if (hb->solo.fork) hipEventDestroy(hb->solo.fork);
+  if (hb->solo.fork) (void)hipEventDestroy(hb->solo.fork);
   for (int l = 0; l < kLevelLanes; l++) {
-    if (hb->solo.lst[l]) hipStreamSynchronize(hb->solo.lst[l]);
-    if (hb->solo.ljoin[l]) hipEventDestroy(hb->solo.ljoin[l]);
-    if (hb->solo.lst[l]) hipStreamDestroy(hb->solo.lst[l]);
+    if (hb->solo.lst[l]) (void)hipStreamSynchronize(hb->solo.lst[l]);
+    if (hb->solo.ljoin[l]) (void)hipEventDestroy(hb->solo.ljoin[l]);
+    if (hb->solo.lst[l]) (void)hipStreamDestroy(hb->solo.lst[l]);
   }
-  if (hb->solo.lfork) hipEventDestroy(hb->solo.lfork);
+  if (hb->solo.lfork) (void)hipEventDestroy(hb->solo.lfork);
   delete hb;
 }
 

@@ -342,7 +342,7 @@ bool plan_100_segments(uint64_t inst, uint64_t R, uint64_t C, uint64_t S, uint64
       if (hipMalloc(&d, tb->host.size() * 4) != hipSuccess) return false;
       if (hipMemcpy(d, tb->host.data(), tb->host.size() * 4, hipMemcpyHostToDevice) !=
           hipSuccess) {
-        hipFree(d);
+        (void)hipFree(d);
         return false;
       }
       tb->dev[dev] = d;
@@ -374,18 +374,18 @@ void free_segment_tables() {
     if (t.joinable()) t.join();
   std::lock_guard<std::mutex> lk(g_seg_mu);
   int cur = 0;
-  hipGetDevice(&cur);
+  (void)hipGetDevice(&cur);
   for (auto &kv : g_seg)
     for (int d = 0; d < 64; d++)
       if (kv.second->dev[d]) {
-        hipSetDevice(d);
-        hipDeviceSynchronize();
-        hipFree(kv.second->dev[d]);
+        (void)hipSetDevice(d);
+        (void)hipDeviceSynchronize();
+        (void)hipFree(kv.second->dev[d]);
         kv.second->dev[d] = nullptr;
       }
   g_seg.clear();
   g_seg_bytes = 0;
-  hipSetDevice(cur);
+  (void)hipSetDevice(cur);
 }
 
 namespace {

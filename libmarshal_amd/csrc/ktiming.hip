@@ -58,7 +58,7 @@ KernelSpan::KernelSpan(hipStream_t st, const char *name, double bytes) : st_(st)
 KernelSpan::~KernelSpan() {
   if (idx_ < 0) return;
   std::lock_guard<std::mutex> lk(g_mu);
-  if (size_t(idx_) < g_recs.size()) hipEventRecord(g_recs[size_t(idx_)].e1, st_);
+  if (size_t(idx_) < g_recs.size()) (void)hipEventRecord(g_recs[size_t(idx_)].e1, st_);
 }
 
 }  // namespace lm
@@ -70,7 +70,7 @@ extern "C" {
 void hip_marshal_set_kernel_timing(int on) {
   std::lock_guard<std::mutex> lk(g_mu);
   for (Rec &r : g_recs) {
-    hipEventSynchronize(r.e1);
+    (void)hipEventSynchronize(r.e1);
     g_free.push_back(r.e0);
     g_free.push_back(r.e1);
   }

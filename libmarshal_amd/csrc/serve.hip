@@ -111,8 +111,8 @@ bool grow(void **p, size_t *cap, size_t want, bool pinned, std::atomic<size_t> *
   size_t n = want >= (size_t(64) << 20) && !pinned ? want
                                                    : std::max<size_t>(want + want / 4, size_t(1) << 20);
   if (*p) {
-    if (pinned) hipHostFree(*p);
-    else hipFree(*p);
+    if (pinned) (void)hipHostFree(*p);
+    else (void)hipFree(*p);
     if (!pinned && held) *held -= *cap;
     *p = nullptr;
     *cap = 0;
@@ -200,15 +200,17 @@ void start_on_slot(Server &s, Slot &sl, const std::vector<int64_t> &ids,
   s.max_in_batch = std::max<int64_t>(s.max_in_batch, int64_t(ids.size()));
   lk.unlock();
   int prev = -1;
-  hipGetDevice(&prev);
-  if (prev != s.device) hipSetDevice(s.device);
-  const bool ok = plain ? launch_solo(s, sl, plain_list, plain_scratch, solo, alone)
-                        : launch_batch(s, sl, descs);
-  if (prev != s.device && prev >= 0) hipSetDevice(prev);
+  (void)hipGetDevice(&prev);
+  // the server's device must be current for its launches; if it cannot be made so, the
+  // requests fail rather than run elsewhere
+  const bool dev_ok = prev == s.device || hipSetDevice(s.device) == hipSuccess;
+  const bool ok = dev_ok && (plain ? launch_solo(s, sl, plain_list, plain_scratch, solo, alone)
+                                   : launch_batch(s, sl, descs));
+  if (prev != s.device && prev >= 0) (void)hipSetDevice(prev);
   lk.lock();
   sl.launching = false;
   if (!ok) {  // mark failed once the stream has drained
-    hipStreamSynchronize(sl.st);
+    (void)hipStreamSynchronize(sl.st);
     for (int64_t id : ids) {
       s.reqs[id].state = -1;
       s.failed++;
@@ -260,9 +262,9 @@ bool retire(Server &s) {
 void release_idle(Server &s) {
   for (Slot &sl : s.slots) {
     if (sl.busy) continue;
-    if (sl.flags) hipFree(sl.flags);
-    if (sl.params) hipFree(sl.params);
-    if (sl.scr) hipFree(sl.scr);
+    if (sl.flags) (void)hipFree(sl.flags);
+    if (sl.params) (void)hipFree(sl.params);
+    if (sl.scr) (void)hipFree(sl.scr);
     s.held -= sl.flags_cap + sl.params_cap + sl.scr_cap;
     sl.flags = nullptr;
     sl.params = nullptr;
@@ -270,7 +272,7 @@ void release_idle(Server &s) {
     sl.flags_cap = sl.params_cap = sl.scr_cap = 0;
   }
   if (s.solo_running == 0 && s.solo_scr) {
-    hipFree(s.solo_scr);
+    (void)hipFree(s.solo_scr);
     s.held -= s.solo_cap;
     s.solo_scr = nullptr;
     s.solo_cap = 0;
@@ -341,7 +343,7 @@ void admit(Server &s, std::unique_lock<std::mutex> &lk) {
 
 void scheduler(Server *sp) {
   Server &s = *sp;
-  hipSetDevice(s.device);
+  (void)hipSetDevice(s.device);
   std::unique_lock<std::mutex> lk(s.mu);
   while (true) {
     retire(s);
@@ -383,7 +385,7 @@ hip_marshal_server *hip_marshal_server_create(size_t hbm_budget_bytes, int strea
   s.budget = hbm_budget_bytes;
   s.max_batch = max_batch;
   s.scratch_cap = uint64_t(tuning("serve_scratch_mb")) << 20;
-  hipGetDevice(&s.device);
+  (void)hipGetDevice(&s.device);
   s.slots.resize(size_t(streams));
   for (Slot &sl : s.slots) {
     hipError_t e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
@@ -539,18 +541,18 @@ void hip_marshal_server_destroy(hip_marshal_server *h) {
     s.cv.notify_all();
   }
   if (s.th.joinable()) s.th.join();
-  if (s.solo_scr) hipFree(s.solo_scr);
+  if (s.solo_scr) (void)hipFree(s.solo_scr);
   for (Slot &sl : s.slots) {
-    if (sl.st) hipStreamSynchronize(sl.st);
-    if (sl.flags) hipFree(sl.flags);
-    if (sl.scr) hipFree(sl.scr);
-    if (sl.params) hipFree(sl.params);
-    if (sl.host) hipHostFree(sl.host);
+    if (sl.st) (void)hipStreamSynchronize(sl.st);
+    if (sl.flags) (void)hipFree(sl.flags);
+    if (sl.scr) (void)hipFree(sl.scr);
+    if (sl.params) (void)hipFree(sl.params);
+    if (sl.host) (void)hipHostFree(sl.host);
     sl.flags = nullptr;
     sl.scr = nullptr;
     sl.params = nullptr;
-    if (sl.ev) hipEventDestroy(sl.ev);
-    if (sl.st) hipStreamDestroy(sl.st);
+    if (sl.ev) (void)hipEventDestroy(sl.ev);
+    if (sl.st) (void)hipStreamDestroy(sl.st);
   }
   delete h;
 }
