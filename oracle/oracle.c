@@ -132,7 +132,7 @@ void orc_golden_transpose(void *buf, void *tmp, size_t es, i64 M, i64 N) {
  * barrier; in[i] = store[i]). Returns 0, or -1 on allocation failure. */
 int orc_inplace_010_bs(void *buf, size_t es, i64 A, i64 a, i64 B) {
   u64 n = (u64)a * B;
-  char *store = (char *)malloc(n * es ? n * es : 1);
+  char *store = (char *)malloc(n * es != 0 ? n * es : 1);
   if (!store) return -1;
   for (i64 t = 0; t < A; t++) {
     char *in = (char *)buf + (u64)t * n * es;
@@ -430,7 +430,7 @@ typedef struct {
 static void *bs_worker(void *p) {
   bs_job *j = (bs_job *)p;
   u64 n = (u64)j->a * j->B;
-  char *store = (char *)malloc(n * j->es ? n * j->es : 1);
+  char *store = (char *)malloc(n * j->es != 0 ? n * j->es : 1);
   if (!store) { j->err = -1; return NULL; }
   for (i64 t = j->tid; t < j->A; t += j->nt) {
     char *in = j->buf + (u64)t * n * j->es;
