@@ -158,7 +158,8 @@ def test_threads_sharing_one_stream(hip, orc):
             assert np.array_equal(b.cpu().numpy(), want), (k, name)
 
 
-def test_batches_built_and_run_beside_plain_calls(hip, orc):
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batches_built_and_run_beside_plain_calls(hip, orc, use_graph):
     """Three threads build batches (planning, allocation, graph capture on their own streams)
     and replay them while a fourth makes plain calls that grow the shared pool and the
     cycle-leader cache: a capture is local to its thread, so neither side may disturb the
@@ -179,7 +180,8 @@ def test_batches_built_and_run_beside_plain_calls(hip, orc):
             stream.synchronize()
             started.wait(timeout=120)
             batch = hip.Batch([(b, op, es, r, c, t)
-                               for b, (_, op, es, r, c, t) in zip(bufs, mixes[k])])
+                               for b, (_, op, es, r, c, t) in zip(bufs, mixes[k])],
+                              use_graph=use_graph)
             for replay in range(2):
                 if replay:
                     with torch.cuda.stream(stream):
